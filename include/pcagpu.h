@@ -1,0 +1,207 @@
+/*
+ * pcagpu.h -- C ABI of the MI355X search path for the PCA-16 superoptimizer.
+ *
+ * The reference has no library interface: each GPU solver is a CLI whose
+ * main() owns an in-process host<->device seam.  The entry points below are
+ * that seam, cut out as plain C so that a host DFS (C, C++, Python ctypes)
+ * can drive hand-written HIP kernels for gfx950.  Plain pointers and sizes
+ * only; no HIP or torch types cross this boundary.
+ *
+ * Conventions: functions return 0 (or a count) on success and a negative
+ * PCAGPU_E* code on failure; the library never exits the process.  Handles
+ * are independent (no global state), so several tasks can be resident at
+ * once; one handle must not be used from two threads at the same time.
+ * The caller owns input memory; the library copies what it keeps.
+ */
+#ifndef PCAGPU_H
+#define PCAGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PCAGPU_MAX_TESTS   32   /* tasks/spec.h:17 MAX_TESTS                */
+#define PCAGPU_MAX_PORTS   8    /* tasks/spec.h:18 MAX_IO_PORTS             */
+#define PCAGPU_MAX_KDEPTH  8    /* gpu/hybrid.cu:29 MAX_KERNEL_DEPTH        */
+#define PCAGPU_MAX_PROG    16   /* gpu/hybrid.cu:30 MAX_PROG_LEN            */
+
+/* error codes */
+#define PCAGPU_OK              0
+#define PCAGPU_EINVAL         -1   /* bad argument / unsupported task        */
+#define PCAGPU_ENOMEM         -2   /* device or host allocation failed       */
+#define PCAGPU_EHIP           -3   /* a HIP runtime call failed              */
+#define PCAGPU_ESTATE         -4   /* call out of order                      */
+#define PCAGPU_EOVERFLOW      -5   /* a fixed-capacity buffer overflowed      */
+#define PCAGPU_ECOLLISION     -6   /* two distinct OEP states shared a 64-bit
+                                      key: the exact path refuses to guess   */
+#define PCAGPU_ENODEV         -7   /* no GPU visible                          */
+#define PCAGPU_EIO            -8   /* task file unreadable                    */
+
+/* Task record, byte-compatible with task_spec_t (tasks/spec.h:20-34). */
+typedef struct {
+    uint16_t in[PCAGPU_MAX_PORTS];
+    uint16_t out[PCAGPU_MAX_PORTS];
+} pca_test_case_t;
+
+typedef struct {
+    char            name[64];
+    uint64_t        max_cycles;
+    uint8_t         input_ports[PCAGPU_MAX_PORTS];
+    int             num_inputs;
+    uint8_t         output_ports[PCAGPU_MAX_PORTS];
+    int             num_outputs;
+    pca_test_case_t tests[PCAGPU_MAX_TESTS];
+    int             num_tests;
+} pca_task_t;
+
+/* CPU->GPU unit of work, same 18-byte layout as prefix_t
+ * (gpu/hybrid.cu:56-60): the first num_insns kernel instructions and the
+ * live-register mask after them (bit i = r_i holds a written value). */
+#pragma pack(push, 1)
+typedef struct {
+    uint16_t insns[PCAGPU_MAX_KDEPTH];
+    uint8_t  num_insns;
+    uint8_t  live;
+} pca_prefix_t;
+#pragma pack(pop)
+
+/* A verified program at the target depth, in canonical (reference DFS)
+ * order: ordered by (prefix_rank, cand[], out_reg). */
+typedef struct {
+    uint64_t prefix_rank;            /* DFS rank of its split-depth prefix   */
+    uint16_t prog[PCAGPU_MAX_PROG];  /* scaffold + kernel + OUT + HLT        */
+    uint8_t  len;                    /* words in prog                        */
+    uint8_t  out_reg;                /* register written by OUT              */
+    uint8_t  kernel_len;             /* kernel instructions                  */
+    uint8_t  cpu_verified;           /* 1: task_evaluate passed every test   */
+    uint16_t cand[PCAGPU_MAX_KDEPTH];/* candidate-list index at each kernel
+                                        position (the DFS rank path)         */
+} pca_solution_t;
+
+typedef struct {
+    uint64_t candidates;      /* (prefix, suffix, out_reg) triples that pass
+                                 the static grammar/liveness filters         */
+    uint64_t suffixes;        /* (prefix, suffix) pairs evaluated            */
+    uint64_t tests_evaluated; /* per-candidate test evaluations (early exit) */
+    uint64_t prefixes;        /* prefixes submitted                          */
+    uint64_t raw_hits;        /* device hits before exact OEP resolution     */
+    uint64_t kernel_ns;       /* device time of the scan kernels (HIP events)*/
+    uint64_t launches;        /* scan kernel launches                        */
+    uint64_t frontier_nodes;  /* wavefront: children generated               */
+    uint64_t frontier_kept;   /* wavefront: survivors after exact OEP        */
+    uint64_t expand_ns;       /* wavefront kernels' device time              */
+    uint64_t oep_collisions;  /* 64-bit key matches with a different state   */
+} pcagpu_stats_t;
+
+typedef struct {
+    int      device;          /* HIP device ordinal (default 0)               */
+    uint32_t batch_capacity;  /* prefixes per flush (default 1<<20)           */
+    uint32_t hit_capacity;    /* device hit records per depth (default 1<<22) */
+    uint32_t waves_per_cu;    /* persistent scan occupancy (0 = autotuned)    */
+    uint32_t reserved[4];
+} pcagpu_opts_t;
+
+typedef struct pcagpu pcagpu_t;
+
+/* ---- host helpers (the reference's C API for load/verify/print) ---- */
+
+/* tasks/spec.c:80-184 task_load: same parser and quirks. */
+int  pcagpu_task_load(const char *path, pca_task_t *task);
+/* Same, from an in-memory JSON string. */
+int  pcagpu_task_parse(const char *json, pca_task_t *task);
+/* tasks/spec.c:188-215 task_evaluate: number of tests passed. */
+int  pcagpu_task_evaluate(const pca_task_t *task, const uint16_t *prog, int len);
+/* src/asm.c:420-503 pca_disasm: identical text. */
+void pcagpu_disasm(uint16_t insn, char *buf, size_t len);
+/* tools/enumerate.c:206-267 gen_candidates (branch mode, use_branches=1):
+ * the ordered candidate list at kernel position pos of a depth-D tree. */
+int  pcagpu_gen_candidates(uint16_t *out, int max, uint8_t live, int pos, int kernel_depth);
+
+/* ---- device handle: replaces the constant uploads and buffers of
+ *      gpu/hybrid.cu:592-621, :662-680 ---- */
+int  pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **out);
+void pcagpu_close(pcagpu_t *h);
+const char *pcagpu_device_name(pcagpu_t *h);
+
+/* Start kernel depth D with the CPU prefixes ending at split s
+ * (gpu/hybrid.cu:699-745): resets the depth's hit list. 0 <= s <= D. */
+int  pcagpu_begin_depth(pcagpu_t *h, int target_depth, int split_depth);
+
+/* Queue n split-depth prefixes whose DFS ranks are first_rank..+n-1
+ * (gpu/hybrid.cu:507-521 emit).  Copies them; flushes automatically when the
+ * batch is full.  Ranks must increase across calls within a depth. */
+int  pcagpu_submit(pcagpu_t *h, const pca_prefix_t *prefixes, uint32_t n, uint64_t first_rank);
+
+/* Evaluate every queued prefix against every suffix of length D - s
+ * (gpu/hybrid.cu:470-505 flush).  Blocks until done.  Returns the number of
+ * raw device hits so far in this depth (>= 0) or an error. */
+int  pcagpu_flush(pcagpu_t *h);
+
+/* Finish the depth: exact OEP resolution of the raw hits (reference
+ * first-visitor semantics with exact keys), canonical ordering, CPU
+ * re-verification (gpu/hybrid.cu:784-816).  Writes up to cap solutions and
+ * returns the total count (which may exceed cap), or an error. */
+long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap);
+
+/* The raw device hits of the current depth before OEP resolution: the
+ * brute-force suffix semantics of gpu/hybrid.cu:163-275 restricted to the
+ * reference grammar.  Each row is (prefix index, flags, match mask, then the
+ * candidate-list indices at positions s..D-1), 14 uint16 per row:
+ *   row[0..3] prefix index (little-endian u64), row[4] flags (bit 0: cannot
+ *   pass: fault or cycle limit), row[5] mask of registers r1..r7 equal to the
+ *   target on every test, row[6..] indices (k of them, rest 0).
+ * Rows come in canonical order.  Call after pcagpu_flush and before
+ * pcagpu_end_depth.  Returns the total row count. */
+long pcagpu_raw_hits(pcagpu_t *h, uint16_t *rows, long cap);
+
+int  pcagpu_get_stats(pcagpu_t *h, pcagpu_stats_t *stats);
+void pcagpu_reset_stats(pcagpu_t *h);
+
+/* ---- the GPU wavefront producer (gpu/wavefront.cu:330-414 expand) ----
+ * Exact-OEP level-synchronous expansion of the depth-D tree down to split
+ * s.  Survivors come back in DFS order with their ranks 0..n-1; BR children
+ * are exempt from OEP as in tools/enumerate.c:435.  Returns the survivor
+ * count (writes up to cap of them) or an error. */
+long pcagpu_frontier(pcagpu_t *h, int target_depth, int split_depth,
+                     pca_prefix_t *out, long cap);
+
+/* ---- the CPU prefix producer (gpu/hybrid.cu:524-588 hybrid_enumerate) ----
+ * Branch-mode DFS of the depth-D tree down to split s with exact OEP keys
+ * and no early stop; no GPU needed.  Writes up to cap surviving prefixes in
+ * DFS order; returns the count.  nodes_out / pruned_out (may be NULL) get the
+ * prefix levels' tried and OEP-pruned counters. */
+long pcagpu_host_frontier(const pca_task_t *task, int target_depth, int split_depth,
+                          pca_prefix_t *out, long cap, uint64_t *nodes_out, uint64_t *pruned_out);
+
+/* ---- whole IDDFS solve (the drop-in for the three CLIs' main loops) ---- */
+typedef struct {
+    int      max_depth;       /* last kernel depth (<= 8)                      */
+    int      start_depth;     /* first kernel depth (default 0)                */
+    int      split;           /* -1 = auto (D-1), else fixed split            */
+    int      producer;        /* 0 = GPU wavefront, 1 = host DFS (hybrid.cu)  */
+    int      verbose;         /* print per-depth progress to stdout           */
+    double   time_limit;      /* seconds, 0 = none: stop between flushes      */
+} pcagpu_solve_opts_t;
+
+typedef struct {
+    int      optimal_depth;   /* -1 = none found up to max_depth              */
+    long     num_solutions;   /* total at the optimal depth                   */
+    int      depths_done;     /* last depth completely searched               */
+    int      timed_out;
+    double   seconds;
+    uint64_t candidates[PCAGPU_MAX_KDEPTH + 1]; /* per depth                  */
+    uint64_t prefixes[PCAGPU_MAX_KDEPTH + 1];
+} pcagpu_result_t;
+
+/* Runs IDDFS on an open handle.  Writes up to cap solutions of the optimal
+ * depth.  Returns 0 or an error. */
+int  pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *opts, pcagpu_result_t *res,
+                  pca_solution_t *out, long cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

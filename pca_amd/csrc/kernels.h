@@ -1,0 +1,88 @@
+// kernels.h -- device-side data layout shared by the kernels and the host
+// driver in pcagpu.cpp.  Compiled only by hipcc.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/pcagpu.h"
+#include "isa.h"
+
+namespace pca {
+
+constexpr int MAXT = PCAGPU_MAX_TESTS;
+constexpr int MAXLEV = PCAGPU_MAX_KDEPTH + 1;   // levels 0..8
+constexpr int LMAX = 3776;                      // == CAND_LMAX
+constexpr int SCAN_WAVES = 4;                   // waves per workgroup
+constexpr int SCAN_WG = SCAN_WAVES * 64;
+
+// Per-task constants, passed by value (lands in the kernel-argument
+// segment, read through the scalar cache).
+struct DevTask {
+    int T;
+    int n_in;
+    int max_cycles;          // clamped to 255
+    int pad;
+    uint16_t tgt[MAXT];      // expected output port value per test
+    TestState root[MAXT];    // per-test state after the IN scaffold
+};
+
+// One device hit: a (prefix, suffix) whose state has a register equal to the
+// target on every test.  flags bit 0: the program cannot pass (a test
+// faulted or ran out of cycles) -- it still takes part in exact OEP.
+struct HitRec {
+    uint32_t prefix;         // index in the batch
+    uint8_t mask;            // registers r1..r7 matching the target (any liveness)
+    uint8_t flags;
+    uint16_t j[PCAGPU_MAX_KDEPTH];   // candidate-list index at positions s..D-1
+    uint16_t pad;
+};
+static_assert(sizeof(HitRec) == 24, "HitRec layout");
+
+struct CandDev {
+    const uint16_t *lists;   // [128][LMAX]
+    const int *count;        // [128]
+    const int *split;        // [128]
+};
+
+struct ScanArgs {
+    const pca_prefix_t *prefixes;
+    uint32_t n_prefix;
+    const uint64_t *unit_off;    // k >= 2: exclusive prefix sums of the position-s list sizes
+    uint64_t n_units;
+    int D, s, k;
+    CandDev cand;
+    HitRec *hits;
+    uint32_t hit_cap;
+    uint32_t *hit_count;
+    unsigned long long *counters;   // [0] candidates [1] suffixes [2] tests [3] parents
+    unsigned long long *work;       // persistent work counter
+};
+
+// Frontier (wavefront) expansion arguments.
+struct ExpandArgs {
+    const pca_prefix_t *parents;
+    uint64_t n_parents;
+    const uint64_t *child_off;       // [n_parents + 1] exclusive scan of child counts
+    int D, pos;                      // children are placed at kernel position pos
+    CandDev cand;
+    unsigned long long *table;       // [2 * cap]: key, min child index
+    uint64_t table_mask;             // cap - 1
+    uint32_t *keep_bits;             // one bit per child
+    uint64_t *keep_count;            // [n_parents] survivors per parent
+    const uint64_t *keep_off;        // [n_parents + 1] exclusive scan of keep_count
+    pca_prefix_t *out;
+    unsigned long long *counters;    // [0] collisions [1] probe overflow
+    unsigned long long *work;
+};
+
+void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
+void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st);
+void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st);
+void launch_expand_insert(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);
+void launch_expand_survive(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);
+void launch_expand_compact(const ExpandArgs &a, int grid, hipStream_t st);
+// exclusive scan in place over n+1 u64 values (the last one receives the total)
+void launch_exclusive_scan(uint64_t *data, uint64_t n, uint64_t *tmp, hipStream_t st);
+uint64_t exclusive_scan_tmp_words(uint64_t n);
+
+}  // namespace pca

@@ -1,0 +1,661 @@
+// kernels.hip -- gfx950 kernels of the search path.
+//
+//   scan_kernel      the hot path: every suffix of length k below a batch of
+//                    split-depth prefixes (gpu/hybrid.cu:163-275 suffix_kernel)
+//   expand_*         exact-OEP wavefront expansion of one tree level
+//                    (gpu/wavefront.cu:330-414 expand_kernel), min-rank winners
+//   scan_u64_*       exclusive prefix sums built from wave-level scans
+//
+// Work is organised per wave64: a wave owns one parent state at a time (the
+// per-test registers live in LDS, one row per test), lanes = candidates of the
+// last kernel position, and each lane exits the test loop on its first
+// mismatch; the wave leaves the loop when a ballot finds no lane still alive.
+#include "kernels.h"
+
+namespace pca {
+
+namespace {
+
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int cand_count(const CandDev &c, uint32_t live, int mf)
+{
+    return c.count[(live >> 1) & 127] + (mf > 0 ? 6 * mf : 0);
+}
+
+// The reference order at one position: ALU ++ CMP ++ BR(mf) ++ LDI ++ ADDI
+// (tools/enumerate.c:206-267), indexed arithmetically.
+__device__ __forceinline__ uint32_t cand_at(const CandDev &c, uint32_t live, int mf, int j)
+{
+    const int m = (live >> 1) & 127;
+    const int a = c.split[m];
+    if (j < a) return c.lists[m * LMAX + j];
+    const int nbr = mf > 0 ? 6 * mf : 0;
+    if (j < a + nbr) {
+        const int q = j - a;
+        return enc_i(OP_BR, 1 + q / mf, 1 + q % mf);
+    }
+    return c.lists[m * LMAX + j - nbr];
+}
+
+// Register value written by a last-position candidate (flags are not needed
+// there: only OUT r; HLT follow).
+__device__ __forceinline__ uint32_t alu_value(uint32_t op, uint32_t a, uint32_t b, uint32_t imm)
+{
+    switch (op) {
+    case OP_ADD: return (a + b) & 0xFFFFu;
+    case OP_SUB: return (a - b) & 0xFFFFu;
+    case OP_MUL: return (a * b) & 0xFFFFu;
+    case OP_DIV: return sdiv16(a, b);
+    case OP_MOD: return smod16(a, b);
+    case OP_AND: return a & b;
+    case OP_OR: return a | b;
+    case OP_XOR: return a ^ b;
+    case OP_SHL: return (a << (b & 15u)) & 0xFFFFu;
+    case OP_SHR: return a >> (b & 15u);
+    case OP_ASR: return (uint32_t)((int32_t)(int16_t)(uint16_t)a >> (b & 15u)) & 0xFFFFu;
+    case OP_LDI: return imm;
+    case OP_ADDI: return (a + ((uint32_t)(int32_t)(int8_t)(uint8_t)imm)) & 0xFFFFu;
+    default: return 0;
+    }
+}
+
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+// Replays a prefix from the root into an LDS level (lanes = tests).
+__device__ __forceinline__ void replay_prefix(const DevTask &task, const pca_prefix_t *p, int n,
+                                              TestState *dst, int lane)
+{
+    if (lane < task.T) {
+        TestState st = task.root[lane];
+        for (int q = 0; q < n; q++) step_test(st, p->insns[q], (uint32_t)q);
+        dst[lane] = st;
+    }
+}
+
+__device__ __forceinline__ void step_level(const DevTask &task, const TestState *src, TestState *dst,
+                                           uint32_t insn, int pos, int lane)
+{
+    if (lane < task.T) {
+        TestState st = src[lane];
+        step_test(st, insn, (uint32_t)pos);
+        dst[lane] = st;
+    }
+}
+
+struct WaveCounters {
+    unsigned long long cand = 0, suf = 0, tests = 0, parents = 0;
+};
+
+// Scan every last-position candidate below one parent state P (kernel
+// position D-1).  Emits hits whose state has a target-matching register.
+__device__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P, uint32_t live,
+                            uint32_t prefix_idx, const uint16_t *jpath, int lane, WaveCounters &wc)
+{
+    const int T = task.T;
+    const uint32_t pos = (uint32_t)(a.D - 1);
+    // ---- parent summary, lanes = tests
+    const bool act = lane < T;
+    TestState st;
+    if (act) st = P[lane];
+    const uint32_t tgt = act ? task.tgt[lane] : 0u;
+    uint32_t pm = 0;
+    for (int r = 1; r < 8; r++) pm |= (__ballot(act && st.r[r] != tgt) == 0ull ? 1u : 0u) << r;
+    // tests on which the last instruction does not execute (skipped by a
+    // pending BR, or the VM already stopped)
+    const bool noexec = act && (st.frozen || st.resume > pos);
+    const uint64_t skipm = __ballot(noexec);
+    const bool cyc = act && ((uint32_t)st.exec + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
+    const uint32_t dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
+    wc.parents += lane == 0;
+
+    const int m = (live >> 1) & 127;
+    const int n = a.cand.count[m];
+    const uint16_t *L = a.cand.lists + m * LMAX;
+    for (int base = 0; base < n; base += 64) {
+        const int j = base + lane;
+        const bool valid = j < n;
+        const uint32_t insn = valid ? L[j] : 0u;
+        const uint32_t op = insn_op(insn), rd = insn_rd(insn), rs = insn_rs(insn), rt = insn_rt(insn);
+        const uint32_t imm = insn_imm(insn);
+        const uint32_t la = live_after(live, insn);
+        wc.cand += valid ? (unsigned)__popc(la & 0xFEu) : 0u;
+        wc.suf += valid;
+        const bool is_cmp = op == OP_CMP;
+        const bool divmod = op == OP_DIV || op == OP_MOD;
+        const uint32_t other = is_cmp ? pm : (pm & ~(1u << rd));
+        const uint32_t sa = op == OP_ADDI ? rd : rs;
+        bool rd_ok = valid && !is_cmp;
+        bool fault = false;
+        bool need = rd_ok;
+        for (int t = 0; t < T; t++) {
+            if (!__any(need)) break;
+            if (need) {
+                uint32_t v;
+                if ((skipm >> t) & 1ull) {
+                    v = P[t].r[rd];
+                } else {
+                    const uint32_t x = P[t].r[sa], y = P[t].r[rt];
+                    if (divmod && y == 0) {
+                        fault = true;      // the VM stops: rd keeps its value
+                        v = P[t].r[rd];
+                    } else {
+                        v = alu_value(op, x, y, imm);
+                    }
+                }
+                rd_ok = rd_ok && v == task.tgt[t];
+                wc.tests++;
+                need = rd_ok || (divmod && other != 0u && !fault);
+            }
+        }
+        const uint32_t match = valid ? (other | (rd_ok ? (1u << rd) : 0u)) & 0xFEu : 0u;
+        if (match) {
+            const uint32_t slot = atomicAdd(a.hit_count, 1u);
+            if (slot < a.hit_cap) {
+                HitRec h;
+                h.prefix = prefix_idx;
+                h.mask = (uint8_t)match;
+                h.flags = (uint8_t)((dead || fault) ? 1u : 0u);
+                for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
+                for (int q = 0; q + 1 < a.k; q++) h.j[q] = jpath[q];
+                h.j[a.k - 1] = (uint16_t)j;
+                h.pad = 0;
+                a.hits[slot] = h;
+            }
+        }
+    }
+}
+
+// k == 0: the prefix is the whole kernel; check its registers directly.
+__device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestState *P, uint32_t live,
+                            uint32_t prefix_idx, int lane, WaveCounters &wc)
+{
+    const int T = task.T;
+    const bool act = lane < T;
+    TestState st;
+    if (act) st = P[lane];
+    const uint32_t tgt = act ? task.tgt[lane] : 0u;
+    uint32_t pm = 0;
+    for (int r = 1; r < 8; r++) pm |= (__ballot(act && st.r[r] != tgt) == 0ull ? 1u : 0u) << r;
+    const bool cyc = act && ((uint32_t)st.exec + 2u > (uint32_t)task.max_cycles);
+    const uint32_t dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
+    if (lane == 0) {
+        wc.parents++;
+        wc.cand += (unsigned)__popc(live & 0xFEu);
+        wc.suf++;
+        if (pm & 0xFEu) {
+            const uint32_t slot = atomicAdd(a.hit_count, 1u);
+            if (slot < a.hit_cap) {
+                HitRec h;
+                h.prefix = prefix_idx;
+                h.mask = (uint8_t)(pm & 0xFEu);
+                h.flags = (uint8_t)dead;
+                for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
+                h.pad = 0;
+                a.hits[slot] = h;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
+{
+    __shared__ TestState s_st[SCAN_WAVES][MAXLEV][MAXT];
+    __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
+    __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveCounters wc;
+    uint16_t jpath[PCAGPU_MAX_KDEPTH];
+    for (;;) {
+        unsigned long long u = 0;
+        if (lane == 0) u = atomicAdd(a.work, 1ull);
+        u = __shfl(u, 0);
+        if (u >= a.n_units) break;
+        uint32_t pi;
+        int js = 0;
+        if (a.k >= 2) {
+            // largest pi with unit_off[pi] <= u
+            uint32_t lo = 0, hi = a.n_prefix;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.unit_off[mid] <= u) lo = mid; else hi = mid;
+            }
+            pi = lo;
+            js = (int)(u - a.unit_off[pi]);
+        } else {
+            pi = (uint32_t)u;
+        }
+        pi = uni(pi);
+        js = (int)uni((uint32_t)js);
+        const pca_prefix_t *P = &a.prefixes[pi];
+        const uint32_t live0 = uni(P->live);
+        replay_prefix(task, P, a.s, s_st[w][0], lane);
+        wave_sync();
+        if (a.k == 0) {
+            leaf_parent(task, a, s_st[w][0], live0, pi, lane, wc);
+            continue;
+        }
+        if (a.k == 1) {
+            scan_parent(task, a, s_st[w][0], live0, pi, jpath, lane, wc);
+            wave_sync();
+            continue;
+        }
+        // k >= 2: level 1 is fixed by the unit; levels 2..k-1 by an odometer
+        const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, js));
+        step_level(task, s_st[w][0], s_st[w][1], c1, a.s, lane);
+        uint32_t lv1 = live_after(live0, c1);
+        if (lane == 0) {
+            s_live[w][1] = lv1;
+            s_idx[w][1] = (uint32_t)js;
+            s_idx[w][2] = 0;
+        }
+        wave_sync();
+        int L = 1;
+        for (;;) {
+            if (L == a.k - 1) {
+                for (int q = 1; q <= L; q++) jpath[q - 1] = (uint16_t)uni(s_idx[w][q]);
+                scan_parent(task, a, s_st[w][L], uni(s_live[w][L]), pi, jpath, lane, wc);
+                wave_sync();
+                L--;
+                if (L < 1) break;
+                if (lane == 0) s_idx[w][L + 1]++;
+                wave_sync();
+                continue;
+            }
+            const int pos = a.s + L;
+            const int mf = a.D - pos - 1;
+            const uint32_t lvL = uni(s_live[w][L]);
+            const int n = cand_count(a.cand, lvL, mf);
+            const int ix = (int)uni(s_idx[w][L + 1]);
+            if (ix >= n) {
+                L--;
+                if (L < 1) break;
+                if (lane == 0) s_idx[w][L + 1]++;
+                wave_sync();
+                continue;
+            }
+            const uint32_t c = uni(cand_at(a.cand, lvL, mf, ix));
+            step_level(task, s_st[w][L], s_st[w][L + 1], c, pos, lane);
+            if (lane == 0) {
+                s_live[w][L + 1] = live_after(lvL, c);
+                if (L + 2 < MAXLEV) s_idx[w][L + 2] = 0;
+            }
+            wave_sync();
+            L++;
+        }
+    }
+    const unsigned long long c0 = wave_sum(wc.cand), c1 = wave_sum(wc.suf), c2 = wave_sum(wc.tests),
+                             c3 = wave_sum(wc.parents);
+    if (lane == 0) {
+        atomicAdd(&a.counters[0], c0);
+        atomicAdd(&a.counters[1], c1);
+        atomicAdd(&a.counters[2], c2);
+        atomicAdd(&a.counters[3], c3);
+    }
+}
+
+// ------------------------------------------------------------ wavefront
+__device__ __forceinline__ uint64_t mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return x;
+}
+
+// Child state on one test: parent row P with candidate c at kernel position
+// pos, with the exact skip / fault rules of step_test.
+__device__ __forceinline__ void child_row(const TestState &p, uint32_t c, uint32_t pos, TestState &out)
+{
+    out = p;
+    step_test(out, c, pos);
+}
+
+__device__ __forceinline__ uint64_t row_words_hash(uint64_t h, const TestState &s)
+{
+    const uint64_t w0 = (uint64_t)s.r[1] | ((uint64_t)s.r[2] << 16) | ((uint64_t)s.r[3] << 32) |
+                        ((uint64_t)s.r[4] << 48);
+    const uint64_t w1 = (uint64_t)s.r[5] | ((uint64_t)s.r[6] << 16) | ((uint64_t)s.r[7] << 32) |
+                        ((uint64_t)s.flags << 48);
+    h = mix64(h ^ w0);
+    return mix64(h ^ w1 ^ 0x5851F42D4C957F2Dull);
+}
+
+__device__ __forceinline__ uint64_t upper_parent(const uint64_t *off, uint64_t n, uint64_t idx)
+{
+    uint64_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (off[mid] <= idx) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void expand_count_kernel(ExpandArgs a, uint64_t *counts)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n_parents) return;
+    counts[i] = (uint64_t)cand_count(a.cand, a.parents[i].live, a.D - a.pos - 1);
+}
+
+// Pass 1: every non-BR child inserts its 64-bit key with atomicMin of its
+// global child index (= DFS rank order), so the slot ends at the first
+// visitor whatever the arrival order.
+__global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, ExpandArgs a)
+{
+    __shared__ TestState s_st[SCAN_WAVES][MAXT];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mf = a.D - a.pos - 1;
+    for (;;) {
+        unsigned long long u = 0;
+        if (lane == 0) u = atomicAdd(a.work, 1ull);
+        u = __shfl(u, 0);
+        if (u >= a.n_parents) break;
+        const pca_prefix_t *P = &a.parents[u];
+        const uint32_t live = uni(P->live);
+        replay_prefix(task, P, a.pos, s_st[w], lane);
+        wave_sync();
+        const int n = cand_count(a.cand, live, mf);
+        const uint64_t base = a.child_off[u];
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const int j = j0 + lane;
+            if (j >= n) continue;
+            const uint32_t c = cand_at(a.cand, live, mf, j);
+            if (insn_op(c) == OP_BR) continue;
+            uint64_t h = 0x243F6A8885A308D3ull;
+            for (int t = 0; t < task.T; t++) {
+                TestState cs;
+                child_row(s_st[w][t], c, (uint32_t)a.pos, cs);
+                h = row_words_hash(h, cs);
+            }
+            h |= 1ull;
+            const unsigned long long idx = base + (unsigned long long)j;
+            uint64_t slot = (h >> 1) & a.table_mask;
+            for (uint64_t probe = 0;; probe++) {
+                if (probe > a.table_mask) { atomicAdd(&a.counters[1], 1ull); break; }
+                unsigned long long *k = &a.table[2 * slot];
+                unsigned long long prev = atomicCAS(k, 0ull, (unsigned long long)h);
+                if (prev == 0ull || prev == h) {
+                    atomicMin(&a.table[2 * slot + 1], idx);
+                    break;
+                }
+                slot = (slot + 1) & a.table_mask;
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Pass 2: a child survives if it is a BR or the first visitor of its key.
+// A later visitor is confirmed against the winner's full state vector before
+// it is dropped; a mismatch is a 64-bit collision and is counted (the host
+// refuses the level).  Survivor bits are packed with a wave ballot.
+__global__ __launch_bounds__(SCAN_WG) void expand_survive_kernel(DevTask task, ExpandArgs a)
+{
+    __shared__ TestState s_st[SCAN_WAVES][MAXT];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int mf = a.D - a.pos - 1;
+    for (;;) {
+        unsigned long long u = 0;
+        if (lane == 0) u = atomicAdd(a.work, 1ull);
+        u = __shfl(u, 0);
+        if (u >= a.n_parents) break;
+        const pca_prefix_t *P = &a.parents[u];
+        const uint32_t live = uni(P->live);
+        replay_prefix(task, P, a.pos, s_st[w], lane);
+        wave_sync();
+        const int n = cand_count(a.cand, live, mf);
+        const uint64_t base = a.child_off[u];
+        unsigned long long kept = 0;
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const int j = j0 + lane;
+            bool keep = false;
+            if (j < n) {
+                const uint32_t c = cand_at(a.cand, live, mf, j);
+                if (insn_op(c) == OP_BR) {
+                    keep = true;
+                } else {
+                    uint64_t h = 0x243F6A8885A308D3ull;
+                    for (int t = 0; t < task.T; t++) {
+                        TestState cs;
+                        child_row(s_st[w][t], c, (uint32_t)a.pos, cs);
+                        h = row_words_hash(h, cs);
+                    }
+                    h |= 1ull;
+                    const unsigned long long idx = base + (unsigned long long)j;
+                    uint64_t slot = (h >> 1) & a.table_mask;
+                    unsigned long long win = ~0ull;
+                    for (uint64_t probe = 0; probe <= a.table_mask; probe++) {
+                        const unsigned long long k = a.table[2 * slot];
+                        if (k == h) { win = a.table[2 * slot + 1]; break; }
+                        if (k == 0ull) break;
+                        slot = (slot + 1) & a.table_mask;
+                    }
+                    if (win == idx) {
+                        keep = true;
+                    } else if (win != ~0ull) {
+                        // confirm: rebuild the winner's state test by test
+                        const uint64_t wp = upper_parent(a.child_off, a.n_parents, win);
+                        const pca_prefix_t *WP = &a.parents[wp];
+                        const uint32_t wc = cand_at(a.cand, WP->live, mf, (int)(win - a.child_off[wp]));
+                        bool same = true;
+                        for (int t = 0; t < task.T && same; t++) {
+                            TestState ws = task.root[t];
+                            for (int q = 0; q < a.pos; q++) step_test(ws, WP->insns[q], (uint32_t)q);
+                            step_test(ws, wc, (uint32_t)a.pos);
+                            TestState cs;
+                            child_row(s_st[w][t], c, (uint32_t)a.pos, cs);
+                            for (int r = 1; r < 8; r++) same = same && ws.r[r] == cs.r[r];
+                            same = same && ws.flags == cs.flags;
+                        }
+                        if (!same) {
+                            atomicAdd(&a.counters[0], 1ull);
+                            keep = true;
+                        }
+                    } else {
+                        atomicAdd(&a.counters[1], 1ull);
+                        keep = true;
+                    }
+                }
+            }
+            const unsigned long long b = __ballot(keep);
+            kept += (unsigned long long)__popcll(b);
+            const uint64_t bit0 = base + (uint64_t)j0;
+            // 64 consecutive children -> up to two 32-bit words
+            if (lane == 0) {
+                const uint64_t wi = bit0 >> 5;
+                const uint32_t sh = (uint32_t)(bit0 & 31u);
+                const uint64_t valid = (n - j0) >= 64 ? ~0ull : ((1ull << (n - j0)) - 1ull);
+                const unsigned long long bv = b & valid;
+                atomicOr(&a.keep_bits[wi], (uint32_t)(bv << sh));
+                atomicOr(&a.keep_bits[wi + 1], (uint32_t)((bv << sh) >> 32));
+                if (sh) atomicOr(&a.keep_bits[wi + 2], (uint32_t)(bv >> (64 - sh)));
+            }
+        }
+        if (lane == 0) a.keep_count[u] = kept;
+        wave_sync();
+    }
+}
+
+// Pass 3: order-preserving compaction.  A wave owns a parent; the output
+// slot of a survivor is the parent's base plus a ballot prefix count.
+__global__ __launch_bounds__(SCAN_WG) void expand_compact_kernel(ExpandArgs a)
+{
+    const int lane = threadIdx.x & 63;
+    const int mf = a.D - a.pos - 1;
+    for (;;) {
+        unsigned long long u = 0;
+        if (lane == 0) u = atomicAdd(a.work, 1ull);
+        u = __shfl(u, 0);
+        if (u >= a.n_parents) break;
+        const pca_prefix_t P = a.parents[u];
+        const int n = cand_count(a.cand, P.live, mf);
+        const uint64_t base = a.child_off[u];
+        uint64_t outp = a.keep_off[u];
+        for (int j0 = 0; j0 < n; j0 += 64) {
+            const int j = j0 + lane;
+            bool keep = false;
+            if (j < n) {
+                const uint64_t bit = base + (uint64_t)j;
+                keep = (a.keep_bits[bit >> 5] >> (bit & 31u)) & 1u;
+            }
+            const unsigned long long b = __ballot(keep);
+            if (keep) {
+                const unsigned long long below = b & ((1ull << lane) - 1ull);
+                pca_prefix_t c = P;
+                const uint32_t insn = cand_at(a.cand, P.live, mf, j);
+                c.insns[a.pos] = (uint16_t)insn;
+                c.num_insns = (uint8_t)(a.pos + 1);
+                c.live = (uint8_t)live_after(P.live, insn);
+                a.out[outp + (uint64_t)__popcll(below)] = c;
+            }
+            outp += (uint64_t)__popcll(b);
+        }
+    }
+}
+
+// ---------------------------------------------------------- prefix sums
+constexpr int SCAN_ITEMS = 16;
+constexpr int SCAN_BLOCK = 256;
+constexpr uint64_t SCAN_TILE = (uint64_t)SCAN_ITEMS * SCAN_BLOCK;
+
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v, int lane)
+{
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t t = __shfl_up(v, o);
+        if (lane >= o) v += t;
+    }
+    return v;
+}
+
+// block-wide exclusive scan of one value per thread; returns the block total
+__device__ uint64_t block_excl_scan(uint64_t v, uint64_t *out_excl)
+{
+    __shared__ uint64_t s_w[SCAN_BLOCK / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t inc = wave_incl_scan(v, lane);
+    if (lane == 63) s_w[w] = inc;
+    __syncthreads();
+    uint64_t wbase = 0, total = 0;
+    for (int i = 0; i < SCAN_BLOCK / 64; i++) {
+        if (i < w) wbase += s_w[i];
+        total += s_w[i];
+    }
+    *out_excl = wbase + inc - v;
+    __syncthreads();
+    return total;
+}
+
+__global__ void scan_tile_sums(const uint64_t *data, uint64_t n, uint64_t *sums)
+{
+    const uint64_t t0 = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++)
+        if (t0 + i < n) s += data[t0 + i];
+    uint64_t ex;
+    const uint64_t tot = block_excl_scan(s, &ex);
+    if (threadIdx.x == 0) sums[blockIdx.x] = tot;
+}
+
+__global__ void scan_tile_apply(uint64_t *data, uint64_t n, const uint64_t *sums_excl)
+{
+    const uint64_t t0 = (uint64_t)blockIdx.x * SCAN_TILE + (uint64_t)threadIdx.x * SCAN_ITEMS;
+    uint64_t vals[SCAN_ITEMS];
+    uint64_t s = 0;
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        vals[i] = (t0 + i < n) ? data[t0 + i] : 0;
+        s += vals[i];
+    }
+    uint64_t ex;
+    block_excl_scan(s, &ex);
+    uint64_t run = ex + (sums_excl ? sums_excl[blockIdx.x] : 0);
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        if (t0 + i < n) data[t0 + i] = run;
+        run += vals[i];
+    }
+}
+
+__global__ void table_init_kernel(unsigned long long *t, uint64_t cap)
+{
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < cap) {
+        t[2 * i] = 0ull;
+        t[2 * i + 1] = ~0ull;
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------- launchers
+void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st)
+{
+    hipLaunchKernelGGL(table_init_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table, cap);
+}
+
+void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(scan_kernel, dim3(grid), dim3(SCAN_WG), 0, st, task, a);
+}
+
+void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st)
+{
+    const uint64_t blocks = (a.n_parents + 255) / 256;
+    if (blocks) hipLaunchKernelGGL(expand_count_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, counts);
+}
+
+void launch_expand_insert(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(expand_insert_kernel, dim3(grid), dim3(SCAN_WG), 0, st, task, a);
+}
+
+void launch_expand_survive(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(expand_survive_kernel, dim3(grid), dim3(SCAN_WG), 0, st, task, a);
+}
+
+void launch_expand_compact(const ExpandArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(expand_compact_kernel, dim3(grid), dim3(SCAN_WG), 0, st, a);
+}
+
+uint64_t exclusive_scan_tmp_words(uint64_t n)
+{
+    // one level of tile sums per recursion step, plus slack
+    uint64_t words = 0, m = n + 1;
+    while (m > SCAN_TILE) {
+        m = (m + SCAN_TILE - 1) / SCAN_TILE;
+        words += m + 1;
+    }
+    return words + 1;
+}
+
+// Exclusive scan of data[0..n] (n+1 values, the last one is usually 0 and
+// receives the grand total).
+void launch_exclusive_scan(uint64_t *data, uint64_t n, uint64_t *tmp, hipStream_t st)
+{
+    const uint64_t len = n + 1;
+    const uint64_t tiles = (len + SCAN_TILE - 1) / SCAN_TILE;
+    if (tiles <= 1) {
+        hipLaunchKernelGGL(scan_tile_apply, dim3(1), dim3(SCAN_BLOCK), 0, st, data, len, (const uint64_t *)nullptr);
+        return;
+    }
+    uint64_t *sums = tmp;
+    hipLaunchKernelGGL(scan_tile_sums, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, data, len, sums);
+    launch_exclusive_scan(sums, tiles - 1, tmp + tiles + 1, st);
+    hipLaunchKernelGGL(scan_tile_apply, dim3((unsigned)tiles), dim3(SCAN_BLOCK), 0, st, data, len, sums);
+}
+
+}  // namespace pca

@@ -1,0 +1,176 @@
+"""Seeded synthetic task generators for the benchmark workloads.
+
+The pseudo-random source is xoshiro256** seeded through splitmix64
+(Blackman & Vigna), the same published generator the reference's task
+generator uses (tools/gen_tasks.c:35-75), so seeds mean the same thing.
+
+Workloads:
+  negate_16      one input x, target (-x) mod 2^16; 16 tests: 0, 1, 0x7FFF,
+                 0x8000, 0xFFFF and 11 seeded uniform values.
+  is_power2_32   one input x, target 1 iff x is a nonzero power of two;
+                 32 tests: the 16 powers of two, 0, 0xFFFF and 14 seeded
+                 uniform non-powers (no duplicates).
+  random_task    target = outputs of a seeded random straight-line program
+                 (length 3-8, opcodes uniform over the 25, registers r1-r7,
+                 immediates uniform uint16 reduced to the imm8 field), 1 or 2
+                 inputs, 8-32 seeded uniform tests.
+"""
+from ._lib import Task
+
+M64 = (1 << 64) - 1
+
+
+class Xoshiro256:
+    def __init__(self, seed):
+        s = []
+        x = seed & M64
+        for _ in range(4):
+            x = (x + 0x9E3779B97F4A7C15) & M64
+            z = x
+            z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+            z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+            s.append(z ^ (z >> 31))
+        self.s = s
+
+    @staticmethod
+    def _rotl(x, k):
+        return ((x << k) | (x >> (64 - k))) & M64
+
+    def next(self):
+        s = self.s
+        result = (self._rotl((s[1] * 5) & M64, 7) * 9) & M64
+        t = (s[1] << 17) & M64
+        s[2] ^= s[0]
+        s[3] ^= s[1]
+        s[1] ^= s[2]
+        s[0] ^= s[3]
+        s[2] ^= t
+        s[3] = self._rotl(s[3], 45)
+        return result
+
+    def u16(self):
+        return self.next() & 0xFFFF
+
+    def randint(self, lo, hi):
+        return lo + self.next() % (hi - lo + 1)
+
+
+def negate_16(seed=42):
+    rng = Xoshiro256(seed)
+    xs = [0, 1, 0x7FFF, 0x8000, 0xFFFF] + [rng.u16() for _ in range(11)]
+    return Task.from_io("negate_16", [(x,) for x in xs], [(-x) & 0xFFFF for x in xs], max_cycles=100)
+
+
+def is_power2(x):
+    return 1 if x != 0 and (x & (x - 1)) == 0 else 0
+
+
+def is_power2_32(seed=42):
+    rng = Xoshiro256(seed)
+    xs = [1 << i for i in range(16)] + [0, 0xFFFF]
+    while len(xs) < 32:
+        x = rng.u16()
+        if is_power2(x) or x in xs:
+            continue
+        xs.append(x)
+    return Task.from_io("is_power2_32", [(x,) for x in xs], [is_power2(x) for x in xs], max_cycles=100)
+
+
+# ---- straight-line random programs -----------------------------------------
+def _interp(prog, ins, max_cycles=64):
+    """Straight-line interpreter of the PCA-16 VM semantics for the
+    generator (programs here have no branches or calls)."""
+    r = [0] * 8
+    flags = 0
+    mem = [0] * 256
+    ports = [0] * 8
+    for i, v in enumerate(ins):
+        ports[i] = v
+    pc = 0
+    for _ in range(max_cycles):
+        if pc >= len(prog):
+            return None
+        w = prog[pc]
+        pc += 1
+        op, rd, rs, rt, imm = (w >> 11) & 31, (w >> 8) & 7, (w >> 5) & 7, (w >> 2) & 7, w & 0xFF
+        a, b = r[rs], r[rt]
+        if op == 1:
+            return ports[0]
+        if op in (2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12):
+            if op in (5, 6) and b == 0:
+                return None
+            sa = a - 0x10000 if a & 0x8000 else a
+            sb = b - 0x10000 if b & 0x8000 else b
+            if op == 2: v = a + b
+            elif op == 3: v = a - b
+            elif op == 4: v = a * b
+            elif op == 5: v = int(sa / sb)
+            elif op == 6: v = sa - int(sa / sb) * sb
+            elif op == 7: v = a & b
+            elif op == 8: v = a | b
+            elif op == 9: v = a ^ b
+            elif op == 10: v = a << (b & 15)
+            elif op == 11: v = a >> (b & 15)
+            else: v = sa >> (b & 15)
+            r[rd] = v & 0xFFFF
+        elif op == 14:
+            r[rd] = mem[r[rs] & 0xFF]
+        elif op == 15:
+            mem[r[rd] & 0xFF] = r[rs]
+        elif op == 16:
+            r[rd] = imm
+        elif op == 17:
+            r[rd] = (r[rd] & 0xFF) | (imm << 8)
+        elif op == 18:
+            r[rd] = (r[rd] + (imm - 256 if imm & 0x80 else imm)) & 0xFFFF
+        elif op == 23:
+            r[rd] = ports[imm & 7]
+        elif op == 24:
+            ports[imm & 7] = r[rd]
+        elif op in (0, 13, 19, 20, 21, 22):
+            pass   # NOP / CMP / control flow: no register effect on this line
+        else:
+            return None
+        r[0] = 0
+    return None
+
+
+def random_task(seed, index):
+    """Task #index of the seeded synthetic suite (see module docstring).
+
+    Control-flow opcodes (BR, JMP, CALL, RET) are drawn like the others but
+    executed as no-ops on the generating straight line, and an immediate
+    drawn uniform over uint16 is reduced to its low byte, the width of the
+    imm8 field.  The generating program's OUT r, 0 of a random register
+    and HLT end it."""
+    rng = Xoshiro256((seed * 1000003 + index) & M64)
+    n_in = rng.randint(1, 2)
+    length = rng.randint(3, 8)
+    prog = [(23 << 11) | ((i + 1) << 8) | i for i in range(n_in)]
+    for _ in range(length):
+        op = rng.randint(0, 24)
+        rd, rs, rt = rng.randint(1, 7), rng.randint(1, 7), rng.randint(1, 7)
+        imm = rng.u16() & 0xFF
+        if op in (1, 23, 24):     # keep HLT/IN/OUT for the scaffold and tail
+            op = 0
+        if op in (16, 17, 18, 19):
+            w = (op << 11) | (rd << 8) | imm
+        else:
+            w = (op << 11) | (rd << 8) | (rs << 5) | (rt << 2)
+        prog.append(w)
+    out_reg = rng.randint(1, 7)
+    prog += [(24 << 11) | (out_reg << 8), 1 << 11]
+    n_tests = rng.randint(8, 32)
+    ins, outs = [], []
+    tries = 0
+    while len(ins) < n_tests and tries < 4 * n_tests:
+        tries += 1
+        x = tuple(rng.u16() for _ in range(n_in))
+        y = _interp(prog, x)
+        if y is None:
+            continue
+        ins.append(x)
+        outs.append(y)
+    if not ins:
+        ins, outs = [tuple(0 for _ in range(n_in))], [0]
+    return Task.from_io(f"syn_{seed}_{index}", ins, outs, max_cycles=100)

@@ -1,0 +1,129 @@
+"""Parity of the HIP path (through the C ABI) against the CPU oracle.
+
+Every comparison is bit-exact: frontiers, raw kernel hits, and the final
+canonically ordered program sets.
+"""
+import random
+
+import pytest
+
+from conftest import golden_task_path
+
+pytestmark = pytest.mark.gpu
+
+TASKS = ["negate", "double", "square", "add", "abs", "relu", "sign", "max"]
+
+
+def _pair(pca, oracle, name):
+    t = pca.Task.load(golden_task_path(name))
+    return t, oracle.task_from_bytes(bytes(t))
+
+
+@pytest.mark.parametrize("name,D,s", [("abs", 3, 2), ("abs", 3, 1), ("max", 3, 2), ("add", 2, 1),
+                                      ("relu", 3, 2), ("sign", 2, 2), ("negate", 3, 2), ("negate", 4, 2)])
+def test_wavefront_frontier_matches_oracle(pca, oracle, name, D, s):
+    t, ot = _pair(pca, oracle, name)
+    print(f"[frontier] {name} D={D} s={s}", flush=True)
+    want = oracle.frontier(ot, D, s)
+    with pca.Solver(t) as sv:
+        got = sv.frontier(D, s)
+    assert got == want
+
+
+def _oracle_suffix_rows(oracle, ot, D, s, prefixes, idxs):
+    rows = []
+    for i in idxs:
+        ins, live = prefixes[i]
+        hits, n, _ = oracle.suffix_hits(ot, D, list(ins) + [0] * (8 - len(ins)), s, live)
+        assert n == len(hits)
+        for h in hits:
+            rows.append((i, tuple(h[:-1]), h[-1]))
+    return rows
+
+
+def _gpu_suffix_rows(pca, sv, D, s, prefixes, idxs):
+    sub = [prefixes[i] for i in idxs]
+    raw = sv.raw_depth(D, s, sub)
+    rows = []
+    for pidx, flags, mask, js in raw:
+        if flags & 1:
+            continue
+        ins, live = sub[pidx]
+        # live after the suffix, from the grammar order
+        lv = live
+        for q, j in enumerate(js):
+            c = pca.gen_candidates(lv, s + q, D)[j]
+            op, rd = (c >> 11) & 31, (c >> 8) & 7
+            if op not in (13, 19) and rd != 0:
+                lv |= 1 << rd
+        for r in range(1, 8):
+            if mask >> r & 1 and lv >> r & 1:
+                rows.append((idxs[pidx], js, r))
+    return rows
+
+
+@pytest.mark.parametrize("name,D", [("abs", 3), ("max", 3), ("relu", 3), ("add", 2), ("negate", 3),
+                                    ("sign", 3)])
+def test_scan_k1_raw_hits_match_bruteforce(pca, oracle, name, D):
+    t, ot = _pair(pca, oracle, name)
+    s = D - 1
+    prefixes = oracle.frontier(ot, D, s)
+    rng = random.Random(7)
+    idxs = sorted(set(list(range(min(40, len(prefixes)))) + rng.sample(range(len(prefixes)),
+                                                                       min(60, len(prefixes)))))
+    print(f"[scan k=1] {name} D={D} prefixes={len(idxs)}", flush=True)
+    want = _oracle_suffix_rows(oracle, ot, D, s, prefixes, idxs)
+    with pca.Solver(t) as sv:
+        got = _gpu_suffix_rows(pca, sv, D, s, prefixes, idxs)
+    assert got == want
+
+
+@pytest.mark.parametrize("name,D,s", [("abs", 3, 1), ("relu", 3, 1)])
+def test_scan_k2_raw_hits_match_bruteforce(pca, oracle, name, D, s):
+    t, ot = _pair(pca, oracle, name)
+    prefixes = oracle.frontier(ot, D, s)
+    rng = random.Random(11)
+    idxs = sorted(rng.sample(range(len(prefixes)), min(2, len(prefixes))))
+    print(f"[scan k=2] {name} D={D} s={s} prefixes={idxs}", flush=True)
+    want = _oracle_suffix_rows(oracle, ot, D, s, prefixes, idxs)
+    with pca.Solver(t) as sv:
+        got = _gpu_suffix_rows(pca, sv, D, s, prefixes, idxs)
+    assert got == want
+
+
+def _exact_fixture(name, maxd):
+    import json
+    import os
+    from conftest import GOLDEN
+    path = os.path.join(GOLDEN, "exact", f"{name}_d{maxd}.json")
+    with open(path) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("name,maxd", [("negate", 3), ("double", 3), ("square", 3), ("add", 3)])
+@pytest.mark.parametrize("producer", ["gpu", "host"])
+def test_solve_matches_oracle_live(pca, oracle, name, maxd, producer):
+    t, ot = _pair(pca, oracle, name)
+    print(f"[solve] {name} producer={producer}", flush=True)
+    want = oracle.solve(ot, maxd, oracle.OEP_EXACT, max_solutions=0)
+    with pca.Solver(t) as sv:
+        res = sv.solve(maxd, producer=producer, cap=1 << 16)
+    assert res.optimal_depth == want["optimal_depth"]
+    assert res.num_solutions == want["num_solutions"]
+    assert [s[0] for s in res.solutions] == want["solutions"]
+    assert all(s[4] for s in res.solutions)
+
+
+@pytest.mark.parametrize("name,maxd", [("abs", 3), ("relu", 3), ("sign", 3), ("clamp", 2), ("popcount", 2),
+                                       ("is_power2", 2), ("max", 2), ("negate_16", 5), ("is_power2_32", 2)])
+@pytest.mark.parametrize("producer", ["gpu", "host"])
+def test_solve_matches_exact_fixture(pca, name, maxd, producer):
+    want = _exact_fixture(name, maxd)
+    t = pca.Task.load(golden_task_path(name))
+    print(f"[solve-fixture] {name} d{maxd} producer={producer}", flush=True)
+    with pca.Solver(t) as sv:
+        res = sv.solve(maxd, producer=producer, cap=1 << 16)
+    assert res.optimal_depth == want["optimal_depth"]
+    assert res.num_solutions == want["num_solutions"]
+    assert [s[0] for s in res.solutions] == want["solutions"]
+    assert all(s[4] for s in res.solutions)
