@@ -1,0 +1,154 @@
+"""Benchmark of the MI355X search path (one JSON line on stdout).
+
+Workload (the hybrid CPU-prefix / GPU-suffix configuration on 1 GPU):
+is_power2 with 32 tests (16 powers of two, 0, 0xFFFF, 14 seeded non-powers),
+IDDFS window at kernel depth 8 with split depth 5: the host DFS (exact OEP)
+emits depth-5 prefixes in DFS order and the device evaluates every 3-long
+suffix below them with every output register.
+
+A step = one flush of a batch of B split-depth prefixes through the scan
+kernel (inputs already resident in HBM when the timed region starts).
+value = candidates/s, a candidate being one (prefix, suffix, out_reg) that
+passes the grammar / liveness / dead-instruction filters (the same counter
+as the CPU baseline's leaf tries).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--batch B]
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_INT_OPS = 256 * 4 * 16 * 2.4e9     # gfx950 VALU lane-ops/s (256 CU x 4 SIMD x 16 lanes x 2.4 GHz)
+OPS_PER_TEST = 9                         # one last-position VM step (8) + its target compare (1)
+OPS_PER_PARENT_TEST = 15                 # parent step (8) + 7 out-register match compares
+
+
+def cpu_baseline(task_path, seconds):
+    """The oracle's restatement of the reference CPU solver (branch mode,
+    32-bit OEP table as in tools/enumerate.c, single thread), timed over a
+    bounded window of the same kernel-depth-8 search."""
+    exe = os.path.join(ROOT, "oracle", "oracle_enumerate")
+    if not os.path.exists(exe):
+        return None
+    out = subprocess.run([exe, task_path, "--start-depth", "8", "-d", "8", "--ref32", "--no-shadow",
+                          "--time-limit", str(seconds), "--json"], capture_output=True, text=True)
+    try:
+        rec = json.loads(out.stdout.strip().splitlines()[-1])
+    except Exception:
+        return None
+    secs = max(rec["seconds"], 1e-9)
+    leaf = rec["leaf_tried"][8]
+    tried = rec["tried"][8]
+    return {"value": leaf / secs, "unit": "candidates/s", "cores": 1, "kind": "port",
+            "sample": f"is_power2_32, kernel depth 8, first {secs:.1f} s of the branch-mode DFS "
+                      f"(restated tools/enumerate.c, 1 thread of {os.cpu_count()}); "
+                      f"reference tried counter {tried / secs:.4g}/s",
+            "tried_per_s": tried / secs}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=8)
+    ap.add_argument("--depth", type=int, default=8)
+    ap.add_argument("--split", type=int, default=5)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
+              "(the search path shards by prefix batches, see DESIGN.md)", file=sys.stderr)
+        return 2
+
+    import pca_amd
+    from pca_amd import taskgen
+    task = taskgen.is_power2_32(42)
+    task_path = os.path.join(ROOT, "tests", "golden", "tasks", "is_power2_32.json")
+    D, s, B = args.depth, args.split, args.batch
+    need = B * (args.warmup + args.steps)
+    prefixes, _, _ = pca_amd.host_frontier(task, D, s, cap=need)
+    if len(prefixes) < need:
+        raise RuntimeError("not enough prefixes")
+
+    with pca_amd.Solver(task) as sv:
+        lib = pca_amd.lib()
+        import ctypes as C
+        arr = (pca_amd.Prefix * need)()
+        for i, (ins, live) in enumerate(prefixes):
+            for q, v in enumerate(ins):
+                arr[i].insns[q] = v
+            arr[i].num_insns = s
+            arr[i].live = live
+
+        def step(i):
+            rc = lib.pcagpu_begin_depth(sv._h, D, s)
+            assert rc == 0, rc
+            ptr = C.cast(C.addressof(arr) + i * B * C.sizeof(pca_amd.Prefix), C.POINTER(pca_amd.Prefix))
+            rc = lib.pcagpu_submit(sv._h, ptr, B, i * B)
+            assert rc == 0, rc
+            rc = lib.pcagpu_flush(sv._h)
+            assert rc >= 0, rc
+            n = lib.pcagpu_end_depth(sv._h, None, 0)
+            assert n >= 0, n
+
+        for i in range(args.warmup):
+            step(i)
+            print(f"warmup step {i} done", file=sys.stderr, flush=True)
+        sv.reset_stats()
+        t0 = time.perf_counter()
+        for i in range(args.warmup, args.warmup + args.steps):
+            step(i)
+            print(f"step {i - args.warmup} done", file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        st = sv.stats()
+        dev = sv.device_name
+
+    wall = t1 - t0
+    cand = st["candidates"]
+    launches = max(st["launches"], 1)
+    kernel_s = st["kernel_ns"] * 1e-9
+    avg_launch = kernel_s / launches
+    ops = OPS_PER_TEST * st["tests_evaluated"] + OPS_PER_PARENT_TEST * task.num_tests * st["parents"]
+    achieved = ops / launches / avg_launch
+    line = {
+        "metric": "candidates/s (1 GPU)",
+        "value": cand / wall,
+        "unit": "candidates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u16",
+        "data": "synthetic (seeded is_power2_32 tests, DFS-ordered depth-5 prefixes)",
+        "config": {"workload": f"is_power2_32 hybrid IDDFS window: kernel depth {D}, split {s}, "
+                               f"{D - s}-insn suffix, {B} prefixes per step",
+                   "tests": task.num_tests, "kernel_depth": D, "split": s, "prefixes_per_step": B,
+                   "device": dev},
+        "roofline": {"bound": "valu-int", "achieved": achieved / 1e12, "peak": PEAK_INT_OPS / 1e12,
+                     "unit": "Tint-ops/s", "frac": achieved / PEAK_INT_OPS, "traffic": None},
+        "kernel": {"name": "scan_kernel", "avg_launch_ms": avg_launch * 1e3, "launches": launches,
+                   "candidates": cand, "suffixes": st["suffixes"], "tests_evaluated": st["tests_evaluated"],
+                   "parents": st["parents"], "raw_hits": st["raw_hits"], "kernel_candidates_per_s": cand / kernel_s if kernel_s else None},
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(task_path, args.cpu_seconds)
+        line["cpu_baseline"] = cb
+        if cb:
+            line["speedup_vs_cpu"] = line["value"] / cb["value"]
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

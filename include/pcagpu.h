@@ -94,6 +94,7 @@ typedef struct {
     uint64_t frontier_kept;   /* wavefront: survivors after exact OEP        */
     uint64_t expand_ns;       /* wavefront kernels' device time              */
     uint64_t oep_collisions;  /* 64-bit key matches with a different state   */
+    uint64_t parents;         /* parent states scanned (last-position lists) */
 } pcagpu_stats_t;
 
 typedef struct {
@@ -170,8 +171,9 @@ long pcagpu_frontier(pcagpu_t *h, int target_depth, int split_depth,
 
 /* ---- the CPU prefix producer (gpu/hybrid.cu:524-588 hybrid_enumerate) ----
  * Branch-mode DFS of the depth-D tree down to split s with exact OEP keys
- * and no early stop; no GPU needed.  Writes up to cap surviving prefixes in
- * DFS order; returns the count.  nodes_out / pruned_out (may be NULL) get the
+ * and no early stop; no GPU needed.  Writes the first cap surviving prefixes
+ * in DFS order and stops there when out is given (returns min(total, cap));
+ * with out == NULL it walks the whole level and returns the total.  nodes_out / pruned_out (may be NULL) get the
  * prefix levels' tried and OEP-pruned counters. */
 long pcagpu_host_frontier(const pca_task_t *task, int target_depth, int split_depth,
                           pca_prefix_t *out, long cap, uint64_t *nodes_out, uint64_t *pruned_out);

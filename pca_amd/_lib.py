@@ -126,7 +126,7 @@ class Solution(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "candidates", "suffixes", "tests_evaluated", "prefixes", "raw_hits", "kernel_ns",
-        "launches", "frontier_nodes", "frontier_kept", "expand_ns", "oep_collisions")]
+        "launches", "frontier_nodes", "frontier_kept", "expand_ns", "oep_collisions", "parents")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -218,7 +218,5 @@ def host_frontier(task, target_depth, split, cap=1 << 22):
     n = lib().pcagpu_host_frontier(C.byref(task), target_depth, split, buf, cap, C.byref(nodes), C.byref(pruned))
     if n < 0:
         raise PcaGpuError(n, "host_frontier")
-    if n > cap:
-        raise PcaGpuError(-5, "host_frontier")
     out = [(tuple(buf[i].insns[:split]), buf[i].live) for i in range(n)]
     return out, nodes.value, pruned.value

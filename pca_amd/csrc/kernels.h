@@ -44,10 +44,22 @@ struct CandDev {
     const int *split;        // [128]
 };
 
+// Work units of a scan launch.  k <= 1: unit = prefix.  k >= 2: units are
+// grouped; a group is a prefix (k == 2, unit = position-s index) or a
+// (prefix, position-s index) pair (k >= 3, unit = position s+1 index), so
+// every unit is one parent state of roughly equal cost.
+struct ScanGroup {
+    uint32_t prefix;
+    uint16_t js;
+    uint16_t pad;
+};
+
 struct ScanArgs {
     const pca_prefix_t *prefixes;
     uint32_t n_prefix;
-    const uint64_t *unit_off;    // k >= 2: exclusive prefix sums of the position-s list sizes
+    const ScanGroup *groups;
+    const uint64_t *group_off;   // [n_groups + 1] exclusive prefix sums of unit counts
+    uint32_t n_groups;
     uint64_t n_units;
     int D, s, k;
     CandDev cand;

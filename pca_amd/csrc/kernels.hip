@@ -45,42 +45,22 @@ __device__ __forceinline__ uint32_t cand_at(const CandDev &c, uint32_t live, int
     return c.lists[m * LMAX + j - nbr];
 }
 
-// Register value written by a last-position candidate (flags are not needed
-// there: only OUT r; HLT follow).
-__device__ __forceinline__ uint32_t alu_value(uint32_t op, uint32_t a, uint32_t b, uint32_t imm)
-{
-    switch (op) {
-    case OP_ADD: return (a + b) & 0xFFFFu;
-    case OP_SUB: return (a - b) & 0xFFFFu;
-    case OP_MUL: return (a * b) & 0xFFFFu;
-    case OP_DIV: return sdiv16(a, b);
-    case OP_MOD: return smod16(a, b);
-    case OP_AND: return a & b;
-    case OP_OR: return a | b;
-    case OP_XOR: return a ^ b;
-    case OP_SHL: return (a << (b & 15u)) & 0xFFFFu;
-    case OP_SHR: return a >> (b & 15u);
-    case OP_ASR: return (uint32_t)((int32_t)(int16_t)(uint16_t)a >> (b & 15u)) & 0xFFFFu;
-    case OP_LDI: return imm;
-    case OP_ADDI: return (a + ((uint32_t)(int32_t)(int8_t)(uint8_t)imm)) & 0xFFFFu;
-    default: return 0;
-    }
-}
-
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
     return v;
 }
 
-// Replays a prefix from the root into an LDS level (lanes = tests).
+// Replays a prefix from the root into an LDS level (lanes = tests).  The
+// state is stepped in place in LDS so that the register index chosen by the
+// instruction addresses LDS, not a private array.
 __device__ __forceinline__ void replay_prefix(const DevTask &task, const pca_prefix_t *p, int n,
                                               TestState *dst, int lane)
 {
     if (lane < task.T) {
-        TestState st = task.root[lane];
+        TestState &st = dst[lane];
+        st = task.root[lane];
         for (int q = 0; q < n; q++) step_test(st, p->insns[q], (uint32_t)q);
-        dst[lane] = st;
     }
 }
 
@@ -88,9 +68,9 @@ __device__ __forceinline__ void step_level(const DevTask &task, const TestState 
                                            uint32_t insn, int pos, int lane)
 {
     if (lane < task.T) {
-        TestState st = src[lane];
+        TestState &st = dst[lane];
+        st = src[lane];
         step_test(st, insn, (uint32_t)pos);
-        dst[lane] = st;
     }
 }
 
@@ -98,10 +78,97 @@ struct WaveCounters {
     unsigned long long cand = 0, suf = 0, tests = 0, parents = 0;
 };
 
+// Value a last-position candidate of opcode OP writes (flags are not needed
+// there: only OUT r; HLT follow).
+template <uint32_t OP>
+__device__ __forceinline__ uint32_t op_value(uint32_t x, uint32_t y, uint32_t imm)
+{
+    if (OP == OP_ADD) return (x + y) & 0xFFFFu;
+    if (OP == OP_SUB) return (x - y) & 0xFFFFu;
+    if (OP == OP_MUL) return (x * y) & 0xFFFFu;
+    if (OP == OP_DIV) return sdiv16(x, y);
+    if (OP == OP_MOD) return smod16(x, y);
+    if (OP == OP_AND) return x & y;
+    if (OP == OP_OR) return x | y;
+    if (OP == OP_XOR) return x ^ y;
+    if (OP == OP_SHL) return (x << (y & 15u)) & 0xFFFFu;
+    if (OP == OP_SHR) return x >> (y & 15u);
+    if (OP == OP_ASR) return (uint32_t)((int32_t)(int16_t)(uint16_t)x >> (y & 15u)) & 0xFFFFu;
+    if (OP == OP_LDI) return imm;
+    if (OP == OP_ADDI) return (x + ((uint32_t)(int32_t)(int8_t)(uint8_t)imm)) & 0xFFFFu;
+    return 0;
+}
+
+// Test loop of the lanes whose candidate has opcode OP (`mine`).  A lane
+// stays in while its rd still equals the target on every test so far (or,
+// for DIV/MOD with another matching register, until a fault is ruled out);
+// the wave leaves as soon as a ballot finds no lane in.
+template <uint32_t OP>
+__device__ __forceinline__ void test_loop(const DevTask &task, const TestState *P, uint64_t skipm, bool mine,
+                                          uint32_t rd, uint32_t sa, uint32_t rt, uint32_t imm, uint32_t other,
+                                          bool &rd_ok, bool &fault, uint32_t &tests)
+{
+    constexpr bool DIVMOD = OP == OP_DIV || OP == OP_MOD;
+    constexpr bool TWO = OP != OP_LDI && OP != OP_ADDI;
+    const int T = task.T;
+    bool need = mine;
+    for (int t = 0; t < T; t++) {
+        if (!__any(need)) break;
+        if (need) {
+            uint32_t v;
+            if ((skipm >> t) & 1ull) {
+                v = P[t].r[rd];
+            } else {
+                const uint32_t x = OP == OP_LDI ? 0u : P[t].r[sa];
+                const uint32_t y = TWO ? P[t].r[rt] : 0u;
+                if (DIVMOD && y == 0u) {
+                    fault = true;      // the VM stops: rd keeps its value
+                    v = P[t].r[rd];
+                } else {
+                    v = op_value<OP>(x, y, imm);
+                }
+            }
+            rd_ok = rd_ok && v == task.tgt[t];
+            tests++;
+            // rd must be checked on every test (a faulted test keeps the old
+            // rd); with another matching register, a DIV/MOD lane also stays
+            // until a fault is ruled out, since a fault makes it unpassable
+            need = rd_ok || (DIVMOD && other != 0u && !fault);
+        }
+    }
+}
+
+// Stage A of a chunk: tests 0 and 1 for the lanes of opcode OP, straight
+// line (no early exit), four LDS reads in flight together.
+template <uint32_t OP>
+__device__ __forceinline__ void pair_eval(const DevTask &task, const TestState *P, uint64_t skipm, bool mine,
+                                          uint32_t rd, uint32_t sa, uint32_t rt, uint32_t imm, bool &rd_ok)
+{
+    constexpr bool DIVMOD = OP == OP_DIV || OP == OP_MOD;
+    constexpr bool TWO = OP != OP_LDI && OP != OP_ADDI;
+    if (!mine) return;
+    uint32_t v[2];
+#pragma unroll
+    for (int t = 0; t < 2; t++) {
+        const uint32_t old = P[t].r[rd];
+        const uint32_t x = OP == OP_LDI ? 0u : P[t].r[sa];
+        const uint32_t y = TWO ? P[t].r[rt] : 0u;
+        uint32_t r = (DIVMOD && y == 0u) ? old : op_value<OP>(x, DIVMOD && y == 0u ? 1u : y, imm);
+        v[t] = ((skipm >> t) & 1ull) ? old : r;
+    }
+    rd_ok = rd_ok && v[0] == task.tgt[0] && v[1] == task.tgt[1];
+}
+
 // Scan every last-position candidate below one parent state P (kernel
 // position D-1).  Emits hits whose state has a target-matching register.
-__device__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P, uint32_t live,
-                            uint32_t prefix_idx, const uint16_t *jpath, int lane, WaveCounters &wc)
+//
+// Stage A runs tests 0-1 on all 64 lanes of a chunk; the few candidates
+// still alive are pushed (ballot prefix count) to a per-wave LDS queue, and
+// stage B runs the full early-exit test loop 64 queued survivors at a time,
+// so lanes that died on test 0 do not idle through the slow lanes' tests.
+__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P,
+                                            uint32_t live, uint32_t prefix_idx, const uint16_t *jpath, int lane,
+                                            WaveCounters &wc, uint16_t *queue)
 {
     const int T = task.T;
     const uint32_t pos = (uint32_t)(a.D - 1);
@@ -123,41 +190,47 @@ __device__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestSt
     const int m = (live >> 1) & 127;
     const int n = a.cand.count[m];
     const uint16_t *L = a.cand.lists + m * LMAX;
-    for (int base = 0; base < n; base += 64) {
-        const int j = base + lane;
-        const bool valid = j < n;
+    uint32_t n_cand = 0, n_suf = 0, n_tests = 0;
+    uint32_t qn = 0;   // queued survivors (wave-uniform)
+    const bool two = T >= 2;
+
+    // full evaluation of queue entries [0, cnt): lanes = entries
+    auto drain = [&](uint32_t cnt) {
+        const bool valid = (uint32_t)lane < cnt;
+        const uint32_t j = valid ? queue[lane] : 0u;
         const uint32_t insn = valid ? L[j] : 0u;
         const uint32_t op = insn_op(insn), rd = insn_rd(insn), rs = insn_rs(insn), rt = insn_rt(insn);
         const uint32_t imm = insn_imm(insn);
-        const uint32_t la = live_after(live, insn);
-        wc.cand += valid ? (unsigned)__popc(la & 0xFEu) : 0u;
-        wc.suf += valid;
-        const bool is_cmp = op == OP_CMP;
-        const bool divmod = op == OP_DIV || op == OP_MOD;
-        const uint32_t other = is_cmp ? pm : (pm & ~(1u << rd));
+        const uint32_t other = pm & ~(1u << rd);
         const uint32_t sa = op == OP_ADDI ? rd : rs;
-        bool rd_ok = valid && !is_cmp;
+        bool rd_ok = valid;
         bool fault = false;
-        bool need = rd_ok;
-        for (int t = 0; t < T; t++) {
-            if (!__any(need)) break;
-            if (need) {
-                uint32_t v;
-                if ((skipm >> t) & 1ull) {
-                    v = P[t].r[rd];
-                } else {
-                    const uint32_t x = P[t].r[sa], y = P[t].r[rt];
-                    if (divmod && y == 0) {
-                        fault = true;      // the VM stops: rd keeps its value
-                        v = P[t].r[rd];
-                    } else {
-                        v = alu_value(op, x, y, imm);
-                    }
-                }
-                rd_ok = rd_ok && v == task.tgt[t];
-                wc.tests++;
-                need = rd_ok || (divmod && other != 0u && !fault);
+        bool pending = valid;
+        for (;;) {
+            const uint64_t pb = __ballot(pending);
+            if (pb == 0ull) break;
+            const uint32_t op0 = __builtin_amdgcn_readlane(op, (int)__ffsll((long long)pb) - 1);
+            const bool mine = pending && op == op0;
+            switch (op0) {
+#define PCA_CASE(OPC) \
+    case OPC: test_loop<OPC>(task, P, skipm, mine, rd, sa, rt, imm, other, rd_ok, fault, n_tests); break;
+            PCA_CASE(OP_ADD)
+            PCA_CASE(OP_SUB)
+            PCA_CASE(OP_MUL)
+            PCA_CASE(OP_DIV)
+            PCA_CASE(OP_MOD)
+            PCA_CASE(OP_AND)
+            PCA_CASE(OP_OR)
+            PCA_CASE(OP_XOR)
+            PCA_CASE(OP_SHL)
+            PCA_CASE(OP_SHR)
+            PCA_CASE(OP_ASR)
+            PCA_CASE(OP_LDI)
+            PCA_CASE(OP_ADDI)
+#undef PCA_CASE
+            default: break;
             }
+            pending = pending && !mine;
         }
         const uint32_t match = valid ? (other | (rd_ok ? (1u << rd) : 0u)) & 0xFEu : 0u;
         if (match) {
@@ -174,7 +247,96 @@ __device__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestSt
                 a.hits[slot] = h;
             }
         }
+    };
+
+    uint32_t insn_next = lane < n ? L[lane] : 0u;
+    for (int base = 0; base < n; base += 64) {
+        const int j = base + lane;
+        const bool valid = j < n;
+        const uint32_t insn = insn_next;
+        insn_next = j + 64 < n ? L[j + 64] : 0u;    // prefetch the next chunk
+        const uint32_t op = insn_op(insn), rd = insn_rd(insn), rs = insn_rs(insn), rt = insn_rt(insn);
+        const uint32_t imm = insn_imm(insn);
+        const uint32_t la = live_after(live, insn);
+        n_cand += valid ? (uint32_t)__popc(la & 0xFEu) : 0u;
+        n_suf += valid;
+        const bool is_cmp = op == OP_CMP;
+        const uint32_t other = is_cmp ? pm : (pm & ~(1u << rd));
+        bool alive;
+        if (is_cmp || !valid) {
+            // CMP writes no register: its hits are the parent's matches
+            if (valid && other) {
+                const uint32_t slot = atomicAdd(a.hit_count, 1u);
+                if (slot < a.hit_cap) {
+                    HitRec h;
+                    h.prefix = prefix_idx;
+                    h.mask = (uint8_t)(other & 0xFEu);
+                    h.flags = (uint8_t)dead;
+                    for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
+                    for (int q = 0; q + 1 < a.k; q++) h.j[q] = jpath[q];
+                    h.j[a.k - 1] = (uint16_t)j;
+                    h.pad = 0;
+                    a.hits[slot] = h;
+                }
+            }
+            alive = false;
+        } else if (two && other == 0u) {
+            const uint32_t sa = op == OP_ADDI ? rd : rs;
+            bool rd_ok = true;
+            bool pending = true;
+            for (;;) {
+                const uint64_t pb = __ballot(pending);
+                if (pb == 0ull) break;
+                const uint32_t op0 = __builtin_amdgcn_readlane(op, (int)__ffsll((long long)pb) - 1);
+                const bool mine = pending && op == op0;
+                switch (op0) {
+#define PCA_CASE(OPC) \
+    case OPC: pair_eval<OPC>(task, P, skipm, mine, rd, sa, rt, imm, rd_ok); break;
+                PCA_CASE(OP_ADD)
+                PCA_CASE(OP_SUB)
+                PCA_CASE(OP_MUL)
+                PCA_CASE(OP_DIV)
+                PCA_CASE(OP_MOD)
+                PCA_CASE(OP_AND)
+                PCA_CASE(OP_OR)
+                PCA_CASE(OP_XOR)
+                PCA_CASE(OP_SHL)
+                PCA_CASE(OP_SHR)
+                PCA_CASE(OP_ASR)
+                PCA_CASE(OP_LDI)
+                PCA_CASE(OP_ADDI)
+#undef PCA_CASE
+                default: break;
+                }
+                pending = pending && !mine;
+            }
+            n_tests += 2;
+            alive = rd_ok;
+        } else {
+            alive = true;   // another register already matches (or T < 2): full test
+        }
+        // push survivors to the queue in lane order
+        const uint64_t sb = __ballot(alive);
+        if (alive) queue[qn + (uint32_t)__popcll(sb & ((1ull << lane) - 1ull))] = (uint16_t)j;
+        qn += (uint32_t)__popcll(sb);
+        if (qn >= 64) {
+            wave_sync();
+            drain(64);
+            wave_sync();
+            const uint32_t rest = qn - 64;
+            if ((uint32_t)lane < rest) queue[lane] = queue[64 + lane];
+            qn = rest;
+            wave_sync();
+        }
     }
+    if (qn) {
+        wave_sync();
+        drain(qn);
+        wave_sync();
+    }
+    wc.cand += n_cand;
+    wc.suf += n_suf;
+    wc.tests += n_tests;
 }
 
 // k == 0: the prefix is the whole kernel; check its registers directly.
@@ -209,90 +371,119 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
     }
 }
 
+constexpr int SCAN_CHUNK = 8;   // units taken per atomic by one wave
+
 __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
 {
     __shared__ TestState s_st[SCAN_WAVES][MAXLEV][MAXT];
     __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
     __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
+    __shared__ uint16_t s_queue[SCAN_WAVES][128];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = a.k;
+    // level of the parent a unit designates: 0 (prefix), 1 (+ position s),
+    // 2 (+ position s+1)
+    const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
     WaveCounters wc;
     uint16_t jpath[PCAGPU_MAX_KDEPTH];
+    uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
     for (;;) {
-        unsigned long long u = 0;
-        if (lane == 0) u = atomicAdd(a.work, 1ull);
-        u = __shfl(u, 0);
-        if (u >= a.n_units) break;
-        uint32_t pi;
-        int js = 0;
-        if (a.k >= 2) {
-            // largest pi with unit_off[pi] <= u
-            uint32_t lo = 0, hi = a.n_prefix;
+        unsigned long long u0 = 0;
+        if (lane == 0) u0 = atomicAdd(a.work, (unsigned long long)SCAN_CHUNK);
+        u0 = __shfl(u0, 0);
+        if (u0 >= a.n_units) break;
+        const unsigned long long u1 = u0 + SCAN_CHUNK < a.n_units ? u0 + SCAN_CHUNK : a.n_units;
+        uint32_t g = 0;
+        if (k >= 2) {
+            uint32_t lo = 0, hi = a.n_groups;   // largest g with group_off[g] <= u0
             while (hi - lo > 1) {
                 const uint32_t mid = (lo + hi) >> 1;
-                if (a.unit_off[mid] <= u) lo = mid; else hi = mid;
+                if (a.group_off[mid] <= u0) lo = mid; else hi = mid;
             }
-            pi = lo;
-            js = (int)(u - a.unit_off[pi]);
-        } else {
-            pi = (uint32_t)u;
+            g = uni(lo);
         }
-        pi = uni(pi);
-        js = (int)uni((uint32_t)js);
-        const pca_prefix_t *P = &a.prefixes[pi];
-        const uint32_t live0 = uni(P->live);
-        replay_prefix(task, P, a.s, s_st[w][0], lane);
-        wave_sync();
-        if (a.k == 0) {
-            leaf_parent(task, a, s_st[w][0], live0, pi, lane, wc);
-            continue;
-        }
-        if (a.k == 1) {
-            scan_parent(task, a, s_st[w][0], live0, pi, jpath, lane, wc);
-            wave_sync();
-            continue;
-        }
-        // k >= 2: level 1 is fixed by the unit; levels 2..k-1 by an odometer
-        const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, js));
-        step_level(task, s_st[w][0], s_st[w][1], c1, a.s, lane);
-        uint32_t lv1 = live_after(live0, c1);
-        if (lane == 0) {
-            s_live[w][1] = lv1;
-            s_idx[w][1] = (uint32_t)js;
-            s_idx[w][2] = 0;
-        }
-        wave_sync();
-        int L = 1;
-        for (;;) {
-            if (L == a.k - 1) {
-                for (int q = 1; q <= L; q++) jpath[q - 1] = (uint16_t)uni(s_idx[w][q]);
-                scan_parent(task, a, s_st[w][L], uni(s_live[w][L]), pi, jpath, lane, wc);
+        for (unsigned long long u = u0; u < u1; u++) {
+            uint32_t pi, js = 0, js1 = 0;
+            if (k >= 2) {
+                while (a.group_off[g + 1] <= u) g++;
+                const ScanGroup G = a.groups[g];
+                pi = uni(G.prefix);
+                const uint32_t j = uni((uint32_t)(u - a.group_off[g]));
+                if (k == 2) js = j;
+                else { js = uni(G.js); js1 = j; }
+            } else {
+                pi = uni((uint32_t)u);
+            }
+            const pca_prefix_t *P = &a.prefixes[pi];
+            const uint32_t live0 = uni(P->live);
+            if (pi != cur_i) {
+                replay_prefix(task, P, a.s, s_st[w][0], lane);
+                cur_i = pi;
+                cur_js = 0xFFFFFFFFu;
                 wave_sync();
-                L--;
-                if (L < 1) break;
-                if (lane == 0) s_idx[w][L + 1]++;
-                wave_sync();
+            }
+            if (k == 0) {
+                leaf_parent(task, a, s_st[w][0], live0, pi, lane, wc);
                 continue;
             }
-            const int pos = a.s + L;
-            const int mf = a.D - pos - 1;
-            const uint32_t lvL = uni(s_live[w][L]);
-            const int n = cand_count(a.cand, lvL, mf);
-            const int ix = (int)uni(s_idx[w][L + 1]);
-            if (ix >= n) {
-                L--;
-                if (L < 1) break;
-                if (lane == 0) s_idx[w][L + 1]++;
-                wave_sync();
-                continue;
+            if (lane == 0) s_live[w][0] = live0;
+            if (k >= 2) {
+                if (js != cur_js) {
+                    const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, (int)js));
+                    step_level(task, s_st[w][0], s_st[w][1], c1, a.s, lane);
+                    if (lane == 0) s_live[w][1] = live_after(live0, c1);
+                    cur_js = js;
+                }
+                if (lane == 0) s_idx[w][1] = js;
+                if (k >= 3) {
+                    wave_sync();
+                    const uint32_t lv1 = uni(s_live[w][1]);
+                    const uint32_t c2 = uni(cand_at(a.cand, lv1, a.D - a.s - 2, (int)js1));
+                    step_level(task, s_st[w][1], s_st[w][2], c2, a.s + 1, lane);
+                    if (lane == 0) {
+                        s_live[w][2] = live_after(lv1, c2);
+                        s_idx[w][2] = js1;
+                    }
+                }
             }
-            const uint32_t c = uni(cand_at(a.cand, lvL, mf, ix));
-            step_level(task, s_st[w][L], s_st[w][L + 1], c, pos, lane);
-            if (lane == 0) {
-                s_live[w][L + 1] = live_after(lvL, c);
-                if (L + 2 < MAXLEV) s_idx[w][L + 2] = 0;
-            }
+            if (lane == 0 && Lu + 1 < MAXLEV) s_idx[w][Lu + 1] = 0;
             wave_sync();
-            L++;
+            // depth-first over positions s+Lu .. D-2, scanning position D-1
+            int L = Lu;
+            for (;;) {
+                if (L == k - 1) {
+                    for (int q = 1; q <= L; q++) jpath[q - 1] = (uint16_t)uni(s_idx[w][q]);
+                    scan_parent(task, a, s_st[w][L], uni(s_live[w][L]), pi, jpath, lane, wc, s_queue[w]);
+                    wave_sync();
+                    if (L == Lu) break;
+                    L--;
+                    if (lane == 0) s_idx[w][L + 1]++;
+                    wave_sync();
+                    continue;
+                }
+                const int pos = a.s + L;
+                const int mf = a.D - pos - 1;
+                const uint32_t lvL = uni(s_live[w][L]);
+                const int n = cand_count(a.cand, lvL, mf);
+                const int ix = (int)uni(s_idx[w][L + 1]);
+                if (ix >= n) {
+                    if (L == Lu) break;
+                    L--;
+                    if (lane == 0) s_idx[w][L + 1]++;
+                    wave_sync();
+                    continue;
+                }
+                const uint32_t c = uni(cand_at(a.cand, lvL, mf, ix));
+                step_level(task, s_st[w][L], s_st[w][L + 1], c, pos, lane);
+                if (lane == 0) {
+                    s_live[w][L + 1] = live_after(lvL, c);
+                    if (L + 2 < MAXLEV) s_idx[w][L + 2] = 0;
+                }
+                wave_sync();
+                L++;
+            }
+            // levels above Lu were overwritten only at L > Lu; level 1 stays
+            // valid for the cached js
         }
     }
     const unsigned long long c0 = wave_sum(wc.cand), c1 = wave_sum(wc.suf), c2 = wave_sum(wc.tests),

@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <new>
@@ -75,17 +76,19 @@ struct pcagpu {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pcagpu_opts_t opts;
-    uint32_t waves_per_cu = 8;
+    uint32_t waves_per_cu = 20;
 
     DevBuf<uint16_t> d_lists;
     DevBuf<int> d_count, d_split;
     DevBuf<pca_prefix_t> d_batch;
-    DevBuf<uint64_t> d_unit_off;
+    DevBuf<ScanGroup> d_groups;
+    DevBuf<uint64_t> d_goff;
     DevBuf<HitRec> d_hits;
     DevBuf<uint32_t> d_hit_count;
     DevBuf<unsigned long long> d_counters, d_work;
     pca_prefix_t *h_batch = nullptr;
-    uint64_t *h_unit_off = nullptr;
+    ScanGroup *h_groups = nullptr;
+    uint64_t *h_goff = nullptr;
     uint32_t batch_cap = 0;
     uint32_t hit_cap = 0;
 
@@ -108,14 +111,50 @@ struct pcagpu {
 
 namespace {
 
+constexpr uint32_t kGroupMax = 1u << 20;   // scan groups per launch
+
+// Test order on the device.  Tests whose inputs look random (about half the
+// bits set, far from 0 and 0xFFFF) come first: few candidates map such an
+// input to the target by accident, while inputs like 0 or 1 are matched by
+// many constant-ish candidates.  Equal-target tests are interleaved
+// round-robin so a near-constant candidate fails within two tests.  The
+// order only decides when a lane stops testing, never which programs pass.
 int fill_devtask(pcagpu *h)
 {
     std::memset(&h->dt, 0, sizeof(h->dt));
     h->dt.T = h->T;
     h->dt.n_in = h->task.num_inputs;
     h->dt.max_cycles = (int)std::min<uint64_t>(h->task.max_cycles, 255);
-    for (int t = 0; t < h->T; t++) h->dt.tgt[t] = h->task.tests[t].out[0];
-    root_states(&h->task, h->dt.root);
+    TestState roots[PCAGPU_MAX_TESTS];
+    root_states(&h->task, roots);
+    auto score = [&](int t) {
+        int sc = 0;
+        for (int i = 0; i < h->task.num_inputs; i++) {
+            const uint32_t x = h->task.tests[t].in[i];
+            const int pc = __builtin_popcount(x);
+            sc += 64 - 8 * std::abs(pc - 8) + (int)(std::min<uint32_t>(x, 65536u - x) >> 10);
+        }
+        return sc;
+    };
+    std::map<uint16_t, std::vector<int>> by_val;
+    for (int t = 0; t < h->T; t++) by_val[h->task.tests[t].out[0]].push_back(t);
+    std::vector<std::vector<int>> groups;
+    for (auto &kv : by_val) {
+        std::vector<int> g = kv.second;
+        std::stable_sort(g.begin(), g.end(), [&](int x, int y) { return score(x) > score(y); });
+        groups.push_back(g);
+    }
+    std::stable_sort(groups.begin(), groups.end(), [&](const std::vector<int> &x, const std::vector<int> &y) {
+        return score(x[0]) > score(y[0]);
+    });
+    std::vector<int> order;
+    for (size_t r = 0; order.size() < (size_t)h->T; r++)
+        for (auto &g : groups)
+            if (r < g.size()) order.push_back(g[r]);
+    for (int t = 0; t < h->T; t++) {
+        h->dt.tgt[t] = h->task.tests[order[t]].out[0];
+        h->dt.root[t] = roots[order[t]];
+    }
     return 0;
 }
 
@@ -126,12 +165,14 @@ int grid_for(pcagpu *h, uint64_t units)
     return (int)std::max<uint64_t>(wgs, 1);
 }
 
-int run_scan(pcagpu *h, uint32_t n, uint64_t units, uint32_t hits_before)
+int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
 {
     ScanArgs a;
     a.prefixes = h->d_batch.p;
     a.n_prefix = n;
-    a.unit_off = h->d_unit_off.p;
+    a.groups = h->d_groups.p;
+    a.group_off = h->d_goff.p;
+    a.n_groups = n_groups;
     a.n_units = units;
     a.D = h->D;
     a.s = h->s;
@@ -142,7 +183,12 @@ int run_scan(pcagpu *h, uint32_t n, uint64_t units, uint32_t hits_before)
     a.hit_count = h->d_hit_count.p;
     a.counters = h->d_counters.p;
     a.work = h->d_work.p;
-    HIP_TRY(hipMemcpyAsync(h->d_hit_count.p, &hits_before, sizeof(uint32_t), hipMemcpyHostToDevice, h->stream));
+    if (n_groups) {
+        HIP_TRY(hipMemcpyAsync(h->d_groups.p, h->h_groups, (size_t)n_groups * sizeof(ScanGroup),
+                               hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipMemcpyAsync(h->d_goff.p, h->h_goff, ((size_t)n_groups + 1) * sizeof(uint64_t),
+                               hipMemcpyHostToDevice, h->stream));
+    }
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     launch_scan(h->dt, a, grid_for(h, units), h->stream);
@@ -153,6 +199,42 @@ int run_scan(pcagpu *h, uint32_t n, uint64_t units, uint32_t hits_before)
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);
     h->stats.launches++;
+    return 0;
+}
+
+// All scan launches of one flush: k <= 1 in one launch; k >= 2 in launches
+// of at most kGroupMax groups.
+int scan_batch(pcagpu *h, uint32_t n)
+{
+    if (h->k <= 1) return run_scan(h, n, 0, n);
+    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
+    uint32_t g = 0;
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t live = h->h_batch[i].live;
+        const int ns = h->ct.count(live, mf_s);
+        if (h->k == 2) {
+            h->h_groups[g] = ScanGroup{i, 0, 0};
+            h->h_goff[g] = acc;
+            acc += (uint64_t)ns;
+            g++;
+        } else {
+            for (int js = 0; js < ns; js++) {
+                const uint32_t c = h->ct.at(live, mf_s, js);
+                h->h_groups[g] = ScanGroup{i, (uint16_t)js, 0};
+                h->h_goff[g] = acc;
+                acc += (uint64_t)h->ct.count(live_after(live, c), mf_s1);
+                g++;
+            }
+        }
+        if (g + (uint32_t)CAND_LMAX + 64 > kGroupMax || i + 1 == n) {
+            h->h_goff[g] = acc;
+            int rc = run_scan(h, n, g, acc);
+            if (rc) return rc;
+            g = 0;
+            acc = 0;
+        }
+    }
     return 0;
 }
 
@@ -214,11 +296,13 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     fill_devtask(h);
     if ((rc = h->d_lists.ensure(h->ct.list.size())) || (rc = h->d_count.ensure(128)) ||
         (rc = h->d_split.ensure(128)) || (rc = h->d_batch.ensure(h->batch_cap)) ||
-        (rc = h->d_unit_off.ensure((size_t)h->batch_cap + 1)) || (rc = h->d_hits.ensure(h->hit_cap)) ||
+        (rc = h->d_groups.ensure(kGroupMax)) || (rc = h->d_goff.ensure(kGroupMax + 1)) ||
+        (rc = h->d_hits.ensure(h->hit_cap)) ||
         (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(8)) || (rc = h->d_work.ensure(1)))
         return fail(rc);
     if (hipHostMalloc((void **)&h->h_batch, (size_t)h->batch_cap * sizeof(pca_prefix_t)) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_unit_off, ((size_t)h->batch_cap + 1) * sizeof(uint64_t)) != hipSuccess)
+        hipHostMalloc((void **)&h->h_groups, (size_t)kGroupMax * sizeof(ScanGroup)) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_goff, ((size_t)kGroupMax + 1) * sizeof(uint64_t)) != hipSuccess)
         return fail(PCAGPU_ENOMEM);
     if (hipMemcpy(h->d_lists.p, h->ct.list.data(), h->ct.list.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(h->d_count.p, h->ct.n, sizeof(h->ct.n), hipMemcpyHostToDevice) != hipSuccess ||
@@ -238,7 +322,8 @@ void pcagpu_close(pcagpu_t *h)
     h->d_count.release();
     h->d_split.release();
     h->d_batch.release();
-    h->d_unit_off.release();
+    h->d_groups.release();
+    h->d_goff.release();
     h->d_hits.release();
     h->d_hit_count.release();
     h->d_counters.release();
@@ -251,7 +336,8 @@ void pcagpu_close(pcagpu_t *h)
     h->d_table.release();
     h->d_keep_bits.release();
     if (h->h_batch) (void)hipHostFree(h->h_batch);
-    if (h->h_unit_off) (void)hipHostFree(h->h_unit_off);
+    if (h->h_groups) (void)hipHostFree(h->h_groups);
+    if (h->h_goff) (void)hipHostFree(h->h_goff);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -284,25 +370,13 @@ int pcagpu_flush(pcagpu_t *h)
     if (h->batch_n == 0) return (int)std::min<size_t>(h->raw.size(), 0x7FFFFFFF);
     (void)hipSetDevice(h->device);
     const uint32_t n = h->batch_n;
-    uint64_t units = n;
-    if (h->k >= 2) {
-        const int mf = h->D - h->s - 1;
-        uint64_t acc = 0;
-        for (uint32_t i = 0; i < n; i++) {
-            h->h_unit_off[i] = acc;
-            acc += (uint64_t)h->ct.count(h->h_batch[i].live, mf);
-        }
-        h->h_unit_off[n] = acc;
-        units = acc;
-        HIP_TRY(hipMemcpyAsync(h->d_unit_off.p, h->h_unit_off, ((size_t)n + 1) * sizeof(uint64_t),
-                               hipMemcpyHostToDevice, h->stream));
-    }
     HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
                            h->stream));
     uint32_t hits = 0;
     int rc;
     for (;;) {
-        if ((rc = run_scan(h, n, units, 0))) return rc;
+        HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), h->stream));
+        if ((rc = scan_batch(h, n))) return rc;
         HIP_TRY(hipMemcpy(&hits, h->d_hit_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
         if (hits <= h->hit_cap) break;
         // grow and redo this batch: no hit may be dropped
@@ -330,6 +404,7 @@ int pcagpu_flush(pcagpu_t *h)
     h->stats.candidates += c[0];
     h->stats.suffixes += c[1];
     h->stats.tests_evaluated += c[2];
+    h->stats.parents += c[3];
     HIP_TRY(hipMemset(h->d_counters.p, 0, 4 * sizeof(unsigned long long)));
     h->stats.raw_hits += hits;
     h->batch_first += n;
@@ -415,12 +490,14 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
     std::vector<char> keep(nodes.size(), 1);
     if (k >= 1) {
         ExactKeySet set(T * 17 > 0 ? T * 17 : 1);
-        std::vector<TestState> st((size_t)(T > 0 ? T : 1));
+        std::vector<TestState> root((size_t)(T > 0 ? T : 1)), st((size_t)(T > 0 ? T : 1));
+        root_states(&h->task, root.data());
         std::vector<uint8_t> key((size_t)T * 17 + 1);
         for (size_t i = 0; i < nodes.size(); i++) {
-            root_states(&h->task, st.data());
-            for (int t = 0; t < T; t++)
+            for (int t = 0; t < T; t++) {
+                st[t] = root[t];
                 for (int q = 0; q < D; q++) step_test(st[t], nodes[i].insns[q], (uint32_t)q);
+            }
             oep_key_from_states(st.data(), T, key.data());
             keep[i] = (char)set.insert(key.data());
         }
@@ -475,7 +552,9 @@ extern "C" long pcagpu_host_frontier(const pca_task_t *task, int D, int s, pca_p
                  [&](const pca_prefix_t &p) {
                      if (out && n < cap) out[n] = p;
                      n++;
-                     return true;
+                     // with an output buffer, stop once it is full: the first
+                     // cap prefixes of the DFS order are what the caller wants
+                     return !(out && n >= cap);
                  },
                  &st);
     if (nodes_out) *nodes_out = st.nodes;

@@ -127,3 +127,55 @@ def test_solve_matches_exact_fixture(pca, name, maxd, producer):
     assert res.num_solutions == want["num_solutions"]
     assert [s[0] for s in res.solutions] == want["solutions"]
     assert all(s[4] for s in res.solutions)
+
+
+def _oracle_all_raw(pca, oracle, ot, D, s, prefixes, idxs):
+    """Every child (k = 1) whose final state has a register r1..r7 equal to
+    the target on all tests, with that mask and whether the program cannot
+    pass (a test faults or times out) -- the device hit semantics."""
+    T = ot.num_tests
+    tgt = [ot.tests[t].out[0] for t in range(T)]
+    rows = []
+    for bi, i in enumerate(idxs):
+        ins, live = prefixes[i]
+        cands = pca.gen_candidates(live, s, D)
+        for j, c in enumerate(cands):
+            kern = list(ins) + [c]
+            st = oracle.replay_state(ot, kern)
+            mask = 0
+            for r in range(1, 8):
+                if all(st[t][r] == tgt[t] for t in range(T)):
+                    mask |= 1 << r
+            if not mask:
+                continue
+            prog = [(23 << 11) | ((q + 1) << 8) | ot.input_ports[q] for q in range(ot.num_inputs)]
+            prog += kern + [(24 << 11) | (1 << 8) | ot.output_ports[0], 1 << 11]
+            # dead: some test does not halt (fault / cycle limit) whatever is output
+            dead = 0
+            for t in range(T):
+                single = oracle.OrTask()
+                import ctypes as C
+                C.memmove(C.byref(single), C.byref(ot), C.sizeof(single))
+                single.tests[0] = ot.tests[t]
+                single.num_tests = 1
+                single.tests[0].out[0] = st[t][1]
+                if oracle.evaluate(single, prog) != 1:
+                    dead = 1
+                    break
+            rows.append((bi, (j,), mask, dead))
+    return rows
+
+
+@pytest.mark.parametrize("name,D", [("ident0", 2), ("ident0", 3), ("is_power2", 3), ("abs", 3)])
+def test_scan_raw_hits_with_shadow_hits(pca, oracle, name, D):
+    t, ot = _pair(pca, oracle, name)
+    s = D - 1
+    prefixes = oracle.frontier(ot, D, s)
+    rng = random.Random(5)
+    idxs = sorted(rng.sample(range(len(prefixes)), min(40, len(prefixes))))
+    print(f"[raw+shadow] {name} D={D} prefixes={len(idxs)}", flush=True)
+    want = _oracle_all_raw(pca, oracle, ot, D, s, prefixes, idxs)
+    with pca.Solver(t) as sv:
+        raw = sv.raw_depth(D, s, [prefixes[i] for i in idxs])
+    got = [(p, js, mask, flags & 1) for p, flags, mask, js in raw]
+    assert got == want
