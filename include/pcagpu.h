@@ -158,6 +158,13 @@ long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap);
  * pcagpu_end_depth.  Returns the total row count. */
 long pcagpu_raw_hits(pcagpu_t *h, uint16_t *rows, long cap);
 
+/* Exhaustive check of the 16-bit ALU (src/vm.c:79-136 semantics): the sum
+ * over a in [0, 2^16), b in [b_lo, b_hi) of a 64-bit mix of (a, b, value,
+ * flags, fault) for opcode op (ADD..ASR, CMP).  h == NULL computes it on
+ * the host, otherwise on h's device.  No reference counterpart: it exists
+ * to pin the device arithmetic against the CPU oracle. */
+int  pcagpu_alu_digest(pcagpu_t *h, int op, uint32_t b_lo, uint32_t b_hi, uint64_t *out);
+
 int  pcagpu_get_stats(pcagpu_t *h, pcagpu_stats_t *stats);
 void pcagpu_reset_stats(pcagpu_t *h);
 

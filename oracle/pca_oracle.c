@@ -1153,3 +1153,30 @@ void or_replay_state(const or_task_t *task, const uint16_t *kernel, int len, uin
         out[k * 9 + 8] = vm.flags;
     }
 }
+
+/* ---- exhaustive ALU digest (pins the VM's ALU semantics) ----
+ * Sum over a in [0, 2^16) and b in [b_lo, b_hi) of a 64-bit mix of
+ * (a, b, value, flags-after, fault) with flags-in 0.  Order independent,
+ * so any other implementation can produce the same figure in any order. */
+static uint64_t dg_mix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ULL;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBULL;
+    return x ^ (x >> 31);
+}
+
+uint64_t or_alu_digest(int op, uint32_t b_lo, uint32_t b_hi)
+{
+    uint64_t h = 0;
+    for (uint32_t b = b_lo; b < b_hi; b++)
+        for (uint32_t a = 0; a < 65536; a++) {
+            uint8_t f;
+            int fault;
+            uint16_t v = or_alu(op, (uint16_t)a, (uint16_t)b, 0, &f, &fault);
+            if (fault) { v = 0; f = 0; }
+            h += dg_mix((uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)v << 32) | ((uint64_t)f << 48) |
+                        ((uint64_t)fault << 56));
+        }
+    return h;
+}

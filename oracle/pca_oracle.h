@@ -158,6 +158,9 @@ long or_suffix_hits(const or_task_t *task, int D, const uint16_t *prefix,
                     int s, uint8_t live, uint16_t *rows, long cap,
                     uint64_t *candidates_out);
 
+/* Exhaustive ALU digest over a in [0,2^16), b in [b_lo,b_hi): see the .c. */
+uint64_t or_alu_digest(int op, uint32_t b_lo, uint32_t b_hi);
+
 /* Replay state of a partial kernel (scaffold + kernel + HLT, 100 cycles),
  * exactly as the reference OEP hashes it: per test r0..r7 then flags.
  * out must hold num_tests*9 uint16 (flags stored in the 9th). */

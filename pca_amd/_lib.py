@@ -166,6 +166,7 @@ SIGNATURES = [
     ("pcagpu_flush", C.c_int, [C.c_void_p]),
     ("pcagpu_end_depth", C.c_long, [C.c_void_p, C.POINTER(Solution), C.c_long]),
     ("pcagpu_raw_hits", C.c_long, [C.c_void_p, C.POINTER(C.c_uint16), C.c_long]),
+    ("pcagpu_alu_digest", C.c_int, [C.c_void_p, C.c_int, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("pcagpu_get_stats", C.c_int, [C.c_void_p, C.POINTER(Stats)]),
     ("pcagpu_reset_stats", None, [C.c_void_p]),
     ("pcagpu_frontier", C.c_long, [C.c_void_p, C.c_int, C.c_int, C.POINTER(Prefix), C.c_long]),
@@ -207,6 +208,15 @@ def gen_candidates(live, pos, kernel_depth):
     out = (C.c_uint16 * 4096)()
     n = lib().pcagpu_gen_candidates(out, 4096, live, pos, kernel_depth)
     return list(out[:n])
+
+
+def alu_digest(op, b_lo, b_hi, solver=None):
+    """Exhaustive ALU digest, on the host (solver None) or the solver's GPU."""
+    out = C.c_uint64()
+    rc = lib().pcagpu_alu_digest(solver._h if solver is not None else None, op, b_lo, b_hi, C.byref(out))
+    if rc < 0:
+        raise PcaGpuError(rc, "alu_digest")
+    return out.value
 
 
 def host_frontier(task, target_depth, split, cap=1 << 22):

@@ -17,8 +17,8 @@ LIB = os.path.join(HERE, "libpcagpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["kernels.hip", "pcagpu.cpp", "host_core.cpp", "host_dfs.cpp"]
-CLIS = {"gpu_hybrid": "cli_hybrid.cpp", "gpu_enumerate_oep": "cli_enumerate_oep.cpp",
-        "gpu_wavefront": "cli_wavefront.cpp"}
+# the three drop-in command lines, one source built in three modes
+CLIS = {"gpu_hybrid": 0, "gpu_enumerate_oep": 1, "gpu_wavefront": 2}
 
 # -packed-fp32-ops: no packed FP32 VALU forms in any kernel of this library.
 COMMON = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=gfx950",
@@ -39,12 +39,10 @@ def build(verbose=True):
         _run([HIPCC] + COMMON + lang + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
     _run([HIPCC, "-shared", "--offload-arch=gfx950", "-o", LIB] + objs)
-    for exe, src in CLIS.items():
-        path = os.path.join(CSRC, src)
-        if os.path.exists(path):
-            _run([HIPCC, "-O2", "-std=c++17", "-I" + os.path.join(ROOT, "include"), path,
-                  "-o", os.path.join(HERE, exe), "-L" + HERE, "-lpcagpu",
-                  "-Wl,-rpath,$ORIGIN"])
+    for exe, mode in CLIS.items():
+        _run(["g++", "-O2", "-std=c++17", "-DPCA_CLI_MODE=%d" % mode, "-I" + os.path.join(ROOT, "include"),
+              os.path.join(CSRC, "cli.cpp"), "-o", os.path.join(HERE, exe), "-L" + HERE, "-lpcagpu",
+              "-Wl,-rpath,$ORIGIN"])
     for o in objs:
         os.remove(o)
     return LIB

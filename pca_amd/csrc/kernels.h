@@ -70,6 +70,36 @@ struct ScanArgs {
     unsigned long long *work;       // persistent work counter
 };
 
+// A level-p node whose state (registers and flags on every test) hashes to a
+// query state; the host confirms it byte for byte.
+struct MatchRec {
+    uint32_t prefix;
+    uint32_t query;
+    uint16_t j[PCAGPU_MAX_KDEPTH];
+};
+static_assert(sizeof(MatchRec) == 24, "MatchRec layout");
+
+struct MatchArgs {
+    const pca_prefix_t *prefixes;
+    uint32_t n_prefix;
+    const ScanGroup *groups;
+    const uint64_t *group_off;
+    uint32_t n_groups;
+    uint64_t n_units;
+    int D, s, kk;                        // nodes at position p = s + kk - 1
+    CandDev cand;
+    const unsigned long long *q0;        // open-addressed set of test-0 row hashes
+    uint64_t q0_mask;
+    const unsigned long long *qf;        // [2 * cap] full-state hash, query id
+    uint64_t qf_mask;
+    MatchRec *out;
+    uint32_t cap;
+    uint32_t *count;
+    unsigned long long *work;
+};
+
+void launch_match(const DevTask &task, const MatchArgs &a, int grid, hipStream_t st);
+
 // Frontier (wavefront) expansion arguments.
 struct ExpandArgs {
     const pca_prefix_t *parents;
@@ -87,6 +117,7 @@ struct ExpandArgs {
     unsigned long long *work;
 };
 
+void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out, hipStream_t st);
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
 void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st);
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st);

@@ -779,6 +779,183 @@ __global__ void scan_tile_apply(uint64_t *data, uint64_t n, const uint64_t *sums
     }
 }
 
+// ------------------------------------------------------------ state match
+__device__ __forceinline__ bool set_has(const unsigned long long *t, uint64_t mask, uint64_t key)
+{
+    uint64_t slot = (key >> 1) & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        const unsigned long long k = t[slot];
+        if (k == key) return true;
+        if (k == 0ull) return false;
+        slot = (slot + 1) & mask;
+    }
+    return false;
+}
+
+__device__ __forceinline__ uint32_t map_get(const unsigned long long *t, uint64_t mask, uint64_t key)
+{
+    uint64_t slot = (key >> 1) & mask;
+    for (uint64_t probe = 0; probe <= mask; probe++) {
+        const unsigned long long k = t[2 * slot];
+        if (k == key) return (uint32_t)t[2 * slot + 1];
+        if (k == 0ull) return 0xFFFFFFFFu;
+        slot = (slot + 1) & mask;
+    }
+    return 0xFFFFFFFFu;
+}
+
+// Every node at kernel position p = s + kk - 1 (all prefixes, all
+// suffixes, BR children excluded: they never enter an OEP table) whose
+// state hashes like a query state.  Test 0's row filters first; only its
+// survivors hash all tests.
+__global__ __launch_bounds__(SCAN_WG) void match_kernel(DevTask task, MatchArgs a)
+{
+    __shared__ TestState s_st[SCAN_WAVES][MAXLEV][MAXT];
+    __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
+    __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int k = a.kk;
+    const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
+    for (;;) {
+        unsigned long long u = 0;
+        if (lane == 0) u = atomicAdd(a.work, 1ull);
+        u = __shfl(u, 0);
+        if (u >= a.n_units) break;
+        uint32_t pi, js = 0, js1 = 0;
+        if (k >= 2) {
+            uint32_t lo = 0, hi = a.n_groups;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (a.group_off[mid] <= u) lo = mid; else hi = mid;
+            }
+            const ScanGroup G = a.groups[lo];
+            pi = uni(G.prefix);
+            const uint32_t j = uni((uint32_t)(u - a.group_off[lo]));
+            if (k == 2) js = j;
+            else { js = uni(G.js); js1 = j; }
+        } else {
+            pi = uni((uint32_t)u);
+        }
+        const pca_prefix_t *P = &a.prefixes[pi];
+        const uint32_t live0 = uni(P->live);
+        replay_prefix(task, P, a.s, s_st[w][0], lane);
+        if (lane == 0) s_live[w][0] = live0;
+        wave_sync();
+        if (k >= 2) {
+            const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, (int)js));
+            step_level(task, s_st[w][0], s_st[w][1], c1, a.s, lane);
+            if (lane == 0) { s_live[w][1] = live_after(live0, c1); s_idx[w][1] = js; }
+            wave_sync();
+            if (k >= 3) {
+                const uint32_t lv1 = uni(s_live[w][1]);
+                const uint32_t c2 = uni(cand_at(a.cand, lv1, a.D - a.s - 2, (int)js1));
+                step_level(task, s_st[w][1], s_st[w][2], c2, a.s + 1, lane);
+                if (lane == 0) { s_live[w][2] = live_after(lv1, c2); s_idx[w][2] = js1; }
+                wave_sync();
+            }
+        }
+        if (lane == 0 && Lu + 1 < MAXLEV) s_idx[w][Lu + 1] = 0;
+        wave_sync();
+        int L = Lu;
+        for (;;) {
+            if (L == k - 1) {
+                // lanes = candidates at the last position p
+                const int pos = a.s + L;
+                const int mf = a.D - pos - 1;
+                const uint32_t lv = uni(s_live[w][L]);
+                const int n = cand_count(a.cand, lv, mf);
+                const TestState *PP = s_st[w][L];
+                for (int base = 0; base < n; base += 64) {
+                    const int j = base + lane;
+                    if (j < n) {
+                        const uint32_t c = cand_at(a.cand, lv, mf, j);
+                        if (insn_op(c) != OP_BR) {
+                            TestState c0 = PP[0];
+                            step_test(c0, c, (uint32_t)pos);
+                            const uint64_t h0 = row_words_hash(0x243F6A8885A308D3ull, c0) | 1ull;
+                            if (set_has(a.q0, a.q0_mask, h0)) {
+                                uint64_t h = 0x243F6A8885A308D3ull;
+                                for (int t = 0; t < task.T; t++) {
+                                    TestState ct = PP[t];
+                                    step_test(ct, c, (uint32_t)pos);
+                                    h = row_words_hash(h, ct);
+                                }
+                                const uint32_t q = map_get(a.qf, a.qf_mask, h | 1ull);
+                                if (q != 0xFFFFFFFFu) {
+                                    const uint32_t slot = atomicAdd(a.count, 1u);
+                                    if (slot < a.cap) {
+                                        MatchRec r;
+                                        r.prefix = pi;
+                                        r.query = q;
+                                        for (int qq = 0; qq < PCAGPU_MAX_KDEPTH; qq++) r.j[qq] = 0;
+                                        for (int qq = 1; qq <= L; qq++) r.j[qq - 1] = (uint16_t)s_idx[w][qq];
+                                        r.j[L] = (uint16_t)j;
+                                        a.out[slot] = r;
+                                    }
+                                }
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+                if (L == Lu) break;
+                L--;
+                if (lane == 0) s_idx[w][L + 1]++;
+                wave_sync();
+                continue;
+            }
+            const int pos = a.s + L;
+            const int mf = a.D - pos - 1;
+            const uint32_t lvL = uni(s_live[w][L]);
+            const int n = cand_count(a.cand, lvL, mf);
+            const int ix = (int)uni(s_idx[w][L + 1]);
+            if (ix >= n) {
+                if (L == Lu) break;
+                L--;
+                if (lane == 0) s_idx[w][L + 1]++;
+                wave_sync();
+                continue;
+            }
+            const uint32_t c = uni(cand_at(a.cand, lvL, mf, ix));
+            step_level(task, s_st[w][L], s_st[w][L + 1], c, pos, lane);
+            if (lane == 0) {
+                s_live[w][L + 1] = live_after(lvL, c);
+                if (L + 2 < MAXLEV) s_idx[w][L + 2] = 0;
+            }
+            wave_sync();
+            L++;
+        }
+    }
+}
+
+// Exhaustive ALU sweep: sum over a in [0, 2^16), b in [b_lo, b_hi) of a
+// 64-bit mix of (a, b, value, flags, fault) with the device ALU semantics.
+__device__ __forceinline__ uint64_t dg_mix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+__global__ void alu_digest_kernel(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out)
+{
+    const uint32_t b = b_lo + blockIdx.x;
+    if (b >= b_hi) return;
+    unsigned long long h = 0;
+    for (uint32_t a = threadIdx.x; a < 65536u; a += blockDim.x) {
+        uint32_t f = 0, v;
+        bool fault = false;
+        if (op == OP_CMP) v = 0, f = flags_sub(a, b);
+        else v = alu_eval(op, a, b, 0, 0, &f, &fault);
+        if (fault) v = 0, f = 0;
+        h += dg_mix((uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)v << 32) | ((uint64_t)f << 48) |
+                    ((uint64_t)fault << 56));
+    }
+    h = wave_sum(h);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
+}
+
 __global__ void table_init_kernel(unsigned long long *t, uint64_t cap)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -791,6 +968,17 @@ __global__ void table_init_kernel(unsigned long long *t, uint64_t cap)
 }  // namespace
 
 // ------------------------------------------------------------- launchers
+void launch_match(const DevTask &task, const MatchArgs &a, int grid, hipStream_t st)
+{
+    hipLaunchKernelGGL(match_kernel, dim3(grid), dim3(SCAN_WG), 0, st, task, a);
+}
+
+void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out, hipStream_t st)
+{
+    if (b_hi > b_lo)
+        hipLaunchKernelGGL(alu_digest_kernel, dim3(b_hi - b_lo), dim3(256), 0, st, op, b_lo, b_hi, out);
+}
+
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st)
 {
     hipLaunchKernelGGL(table_init_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table, cap);

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <set>
 #include <new>
 #include <string>
 #include <vector>
@@ -107,6 +108,12 @@ struct pcagpu {
     std::vector<uint64_t> ranks;
     std::vector<HostHit> raw;
     pcagpu_stats_t stats;
+    std::vector<int> order;          // device test order (index into task tests)
+
+    // state-match pass buffers
+    DevBuf<unsigned long long> d_q0, d_qf;
+    DevBuf<MatchRec> d_match;
+    DevBuf<uint32_t> d_match_count;
 };
 
 namespace {
@@ -155,6 +162,7 @@ int fill_devtask(pcagpu *h)
         h->dt.tgt[t] = h->task.tests[order[t]].out[0];
         h->dt.root[t] = roots[order[t]];
     }
+    h->order = order;
     return 0;
 }
 
@@ -335,6 +343,10 @@ void pcagpu_close(pcagpu_t *h)
     h->d_scan_tmp.release();
     h->d_table.release();
     h->d_keep_bits.release();
+    h->d_q0.release();
+    h->d_qf.release();
+    h->d_match.release();
+    h->d_match_count.release();
     if (h->h_batch) (void)hipHostFree(h->h_batch);
     if (h->h_groups) (void)hipHostFree(h->h_groups);
     if (h->h_goff) (void)hipHostFree(h->h_goff);
@@ -435,21 +447,217 @@ int pcagpu_submit(pcagpu_t *h, const pca_prefix_t *p, uint32_t n, uint64_t first
 
 namespace {
 
-struct Node {
-    uint64_t prefix;                 // depth-prefix index (rank order)
-    uint16_t j[PCAGPU_MAX_KDEPTH];
+// ---- node paths: a node below a depth prefix is (prefix index, candidate
+// indices at positions s, s+1, ...); lexicographic order = DFS rank order.
+struct Path {
+    uint32_t prefix = 0;
+    uint8_t len = 0;
+    uint16_t j[PCAGPU_MAX_KDEPTH] = {0, 0, 0, 0, 0, 0, 0, 0};
+    bool operator<(const Path &o) const
+    {
+        if (prefix != o.prefix) return prefix < o.prefix;
+        const int n = len < o.len ? len : o.len;
+        for (int q = 0; q < n; q++)
+            if (j[q] != o.j[q]) return j[q] < o.j[q];
+        return len < o.len;
+    }
+    bool operator==(const Path &o) const
+    {
+        if (prefix != o.prefix || len != o.len) return false;
+        for (int q = 0; q < len; q++)
+            if (j[q] != o.j[q]) return false;
+        return true;
+    }
+};
+
+Path parent_of(const Path &x)
+{
+    Path p = x;
+    p.len = (uint8_t)(x.len - 1);
+    p.j[p.len] = 0;
+    return p;
+}
+
+// kernel instructions of a node (positions 0 .. s+len-1) and its live mask
+int node_insns(const pcagpu *h, const Path &x, uint16_t *insns, uint32_t *live_out)
+{
+    const pca_prefix_t &p = h->prefixes[x.prefix];
+    for (int q = 0; q < h->s; q++) insns[q] = p.insns[q];
+    uint32_t live = p.live;
+    for (int q = 0; q < x.len; q++) {
+        const int pos = h->s + q;
+        const uint16_t c = h->ct.at(live, h->D - pos - 1, x.j[q]);
+        insns[pos] = c;
+        live = live_after(live, c);
+    }
+    if (live_out) *live_out = live;
+    return h->s + x.len;
+}
+
+// exact OEP key of a node: per test (device order) r0..r7 and flags
+void node_key(const pcagpu *h, const Path &x, uint8_t *key, TestState *st)
+{
     uint16_t insns[PCAGPU_MAX_KDEPTH];
+    const int n = node_insns(h, x, insns, nullptr);
+    for (int t = 0; t < h->T; t++) {
+        st[t] = h->dt.root[t];
+        for (int q = 0; q < n; q++) step_test(st[t], insns[q], (uint32_t)q);
+    }
+    oep_key_from_states(st, h->T, key);
+}
+
+uint64_t h_mix64(uint64_t x)
+{
+    x ^= x >> 30;
+    x *= 0xBF58476D1CE4E5B9ull;
+    x ^= x >> 27;
+    x *= 0x94D049BB133111EBull;
+    x ^= x >> 31;
+    return x;
+}
+
+// host twin of the device row hash (kernels.hip row_words_hash)
+uint64_t h_row_hash(uint64_t h, const TestState &s)
+{
+    const uint64_t w0 = (uint64_t)s.r[1] | ((uint64_t)s.r[2] << 16) | ((uint64_t)s.r[3] << 32) | ((uint64_t)s.r[4] << 48);
+    const uint64_t w1 = (uint64_t)s.r[5] | ((uint64_t)s.r[6] << 16) | ((uint64_t)s.r[7] << 32) | ((uint64_t)s.flags << 48);
+    h = h_mix64(h ^ w0);
+    return h_mix64(h ^ w1 ^ 0x5851F42D4C957F2Dull);
+}
+
+// Builds the scan groups of prefixes [first, first+n) for kk positions after
+// the split and calls launch(n_groups, n_units) for every filled chunk.
+template <typename F>
+int for_group_chunks(pcagpu *h, uint32_t first, uint32_t n, int kk, F &&launch)
+{
+    if (kk <= 1) return launch(0u, (uint64_t)n);
+    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
+    uint32_t g = 0;
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t live = h->prefixes[first + i].live;
+        const int ns = h->ct.count(live, mf_s);
+        if (kk == 2) {
+            h->h_groups[g] = ScanGroup{i, 0, 0};
+            h->h_goff[g] = acc;
+            acc += (uint64_t)ns;
+            g++;
+        } else {
+            for (int js = 0; js < ns; js++) {
+                const uint32_t c = h->ct.at(live, mf_s, js);
+                h->h_groups[g] = ScanGroup{i, (uint16_t)js, 0};
+                h->h_goff[g] = acc;
+                acc += (uint64_t)h->ct.count(live_after(live, c), mf_s1);
+                g++;
+            }
+        }
+        if (g + (uint32_t)CAND_LMAX + 64 > kGroupMax || i + 1 == n) {
+            h->h_goff[g] = acc;
+            int rc = launch(g, acc);
+            if (rc) return rc;
+            g = 0;
+            acc = 0;
+        }
+    }
+    return 0;
+}
+
+// Device pass: every node with kk positions after the split (all depth
+// prefixes) whose state hashes like one of the queries.  Returns candidate
+// matches (path, query id); the caller confirms them exactly.
+int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::vector<uint64_t> &qhf,
+              std::vector<std::pair<Path, uint32_t>> &out)
+{
+    int rc;
+    uint64_t cap0 = 64, capf = 64;
+    while (cap0 < 2 * qh0.size()) cap0 <<= 1;
+    while (capf < 2 * qhf.size()) capf <<= 1;
+    std::vector<unsigned long long> t0(cap0, 0ull), tf(2 * capf, 0ull);
+    for (uint64_t k : qh0) {
+        uint64_t sl = (k >> 1) & (cap0 - 1);
+        while (t0[sl] != 0ull && t0[sl] != k) sl = (sl + 1) & (cap0 - 1);
+        t0[sl] = k;
+    }
+    for (size_t q = 0; q < qhf.size(); q++) {
+        const uint64_t k = qhf[q];
+        uint64_t sl = (k >> 1) & (capf - 1);
+        while (tf[2 * sl] != 0ull && tf[2 * sl] != k) sl = (sl + 1) & (capf - 1);
+        if (tf[2 * sl] == k) continue;   // equal full hashes: one id answers for both (host confirms)
+        tf[2 * sl] = k;
+        tf[2 * sl + 1] = q;
+    }
+    if ((rc = h->d_q0.ensure(cap0)) || (rc = h->d_qf.ensure(2 * capf)) || (rc = h->d_match_count.ensure(1)))
+        return rc;
+    if (h->d_match.n == 0 && (rc = h->d_match.ensure(1 << 16))) return rc;
+    HIP_TRY(hipMemcpy(h->d_q0.p, t0.data(), cap0 * 8, hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(h->d_qf.p, tf.data(), 2 * capf * 8, hipMemcpyHostToDevice));
+    const uint32_t total = (uint32_t)h->prefixes.size();
+    for (uint32_t first = 0; first < total; first += h->batch_cap) {
+        const uint32_t n = std::min<uint32_t>(h->batch_cap, total - first);
+        HIP_TRY(hipMemcpy(h->d_batch.p, h->prefixes.data() + first, (size_t)n * sizeof(pca_prefix_t),
+                          hipMemcpyHostToDevice));
+        rc = for_group_chunks(h, first, n, kk, [&](uint32_t n_groups, uint64_t units) -> int {
+            if (n_groups) {
+                HIP_TRY(hipMemcpy(h->d_groups.p, h->h_groups, (size_t)n_groups * sizeof(ScanGroup),
+                                  hipMemcpyHostToDevice));
+                HIP_TRY(hipMemcpy(h->d_goff.p, h->h_goff, ((size_t)n_groups + 1) * 8, hipMemcpyHostToDevice));
+            }
+            for (;;) {
+                MatchArgs a;
+                a.prefixes = h->d_batch.p;
+                a.n_prefix = n;
+                a.groups = h->d_groups.p;
+                a.group_off = h->d_goff.p;
+                a.n_groups = n_groups;
+                a.n_units = units;
+                a.D = h->D;
+                a.s = h->s;
+                a.kk = kk;
+                a.cand = CandDev{h->d_lists.p, h->d_count.p, h->d_split.p};
+                a.q0 = h->d_q0.p;
+                a.q0_mask = cap0 - 1;
+                a.qf = h->d_qf.p;
+                a.qf_mask = capf - 1;
+                a.out = h->d_match.p;
+                a.cap = (uint32_t)h->d_match.n;
+                a.count = h->d_match_count.p;
+                a.work = h->d_work.p;
+                HIP_TRY(hipMemsetAsync(h->d_match_count.p, 0, 4, h->stream));
+                HIP_TRY(hipMemsetAsync(h->d_work.p, 0, 8, h->stream));
+                launch_match(h->dt, a, grid_for(h, units), h->stream);
+                HIP_TRY(hipGetLastError());
+                uint32_t cnt = 0;
+                HIP_TRY(hipMemcpyAsync(&cnt, h->d_match_count.p, 4, hipMemcpyDeviceToHost, h->stream));
+                HIP_TRY(hipStreamSynchronize(h->stream));
+                if (cnt > a.cap) {
+                    h->d_match.release();
+                    int r2 = h->d_match.ensure((size_t)cnt + cnt / 2 + 1024);
+                    if (r2) return r2;
+                    continue;
+                }
+                std::vector<MatchRec> buf(cnt);
+                if (cnt)
+                    HIP_TRY(hipMemcpy(buf.data(), h->d_match.p, (size_t)cnt * sizeof(MatchRec), hipMemcpyDeviceToHost));
+                for (const MatchRec &r : buf) {
+                    Path p;
+                    p.prefix = first + r.prefix;
+                    p.len = (uint8_t)kk;
+                    for (int q = 0; q < kk; q++) p.j[q] = r.j[q];
+                    out.push_back({p, r.query});
+                }
+                return 0;
+            }
+        });
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+struct Node {
+    Path path;
     uint32_t live;                   // live mask after the full kernel
     uint8_t mask, dead;
 };
-
-bool node_less(const Node &a, const Node &b, int k)
-{
-    if (a.prefix != b.prefix) return a.prefix < b.prefix;
-    for (int q = 0; q < k; q++)
-        if (a.j[q] != b.j[q]) return a.j[q] < b.j[q];
-    return false;
-}
 
 }  // namespace
 
@@ -459,46 +667,109 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
     int rc = pcagpu_flush(h);
     if (rc < 0) return rc;
     h->in_depth = false;
-    const int D = h->D, s = h->s, k = h->k, T = h->T;
-    // rebuild each hit's kernel from its prefix and candidate-list indices
+    const int D = h->D, k = h->k, T = h->T;
+    const int KL = T * 17 > 0 ? T * 17 : 1;
     std::vector<Node> nodes;
     nodes.reserve(h->raw.size());
     for (const HostHit &x : h->raw) {
         Node nd;
-        std::memset(&nd, 0, sizeof(nd));
-        nd.prefix = x.prefix;
-        const pca_prefix_t &p = h->prefixes[x.prefix];
-        for (int q = 0; q < s; q++) nd.insns[q] = p.insns[q];
-        uint32_t live = p.live;
-        for (int q = 0; q < k; q++) {
-            const int pos = s + q;
-            const uint16_t c = h->ct.at(live, D - pos - 1, x.j[q]);
-            nd.insns[pos] = c;
-            nd.j[q] = x.j[q];
-            live = live_after(live, c);
-        }
-        nd.live = live;
+        nd.path.prefix = (uint32_t)x.prefix;
+        nd.path.len = (uint8_t)k;
+        for (int q = 0; q < k; q++) nd.path.j[q] = x.j[q];
+        uint16_t insns[PCAGPU_MAX_KDEPTH];
+        node_insns(h, nd.path, insns, &nd.live);
         nd.mask = x.mask;
         nd.dead = x.flags & 1;
         nodes.push_back(nd);
     }
-    std::sort(nodes.begin(), nodes.end(), [k](const Node &a, const Node &b) { return node_less(a, b, k); });
-    // Exact OEP at the last level (tools/enumerate.c:435-456, exact keys):
-    // every node whose state has a target-matching register is a hit, so the
-    // first visitor of each such state is among the hits.  Keep it, drop the
-    // rest.  (k == 0: the prefixes were already deduplicated.)
-    std::vector<char> keep(nodes.size(), 1);
-    if (k >= 1) {
-        ExactKeySet set(T * 17 > 0 ? T * 17 : 1);
-        std::vector<TestState> root((size_t)(T > 0 ? T : 1)), st((size_t)(T > 0 ? T : 1));
-        root_states(&h->task, root.data());
-        std::vector<uint8_t> key((size_t)T * 17 + 1);
-        for (size_t i = 0; i < nodes.size(); i++) {
-            for (int t = 0; t < T; t++) {
-                st[t] = root[t];
-                for (int q = 0; q < D; q++) step_test(st[t], nodes[i].insns[q], (uint32_t)q);
+    std::sort(nodes.begin(), nodes.end(), [](const Node &a, const Node &b) { return a.path < b.path; });
+    std::vector<TestState> st((size_t)(T > 0 ? T : 1));
+    std::vector<uint8_t> key((size_t)KL);
+
+    // ---- exact OEP through the suffix levels (k >= 2).  A node at level p
+    // survives iff its parent survives and it is a BR or no lower-ranked
+    // node of the same state has a surviving parent (the reference's DFS
+    // first visitor, tools/enumerate.c:435-456, with exact keys).  Needed
+    // verdicts are collected top-down (parents of hits, then parents of the
+    // lower-ranked same-state nodes the device finds) and resolved bottom-up.
+    std::map<Path, char> survive;
+    if (k >= 2) {
+        std::vector<std::set<Path>> X((size_t)k);
+        for (const Node &nd : nodes) X[k - 1].insert(parent_of(nd.path));
+        std::map<Path, std::vector<Path>> earlier;
+        std::map<Path, char> is_br;
+        for (int L = k - 1; L >= 1; L--) {
+            std::vector<Path> xs(X[L].begin(), X[L].end());
+            std::map<std::vector<uint8_t>, uint32_t> qid;
+            std::vector<std::vector<uint8_t>> qkeys;
+            std::vector<uint64_t> qh0, qhf;
+            std::vector<uint32_t> xq(xs.size(), 0xFFFFFFFFu);
+            for (size_t i = 0; i < xs.size(); i++) {
+                uint16_t insns[PCAGPU_MAX_KDEPTH];
+                const int n = node_insns(h, xs[i], insns, nullptr);
+                const bool br = insn_op(insns[n - 1]) == OP_BR;
+                is_br[xs[i]] = br;
+                if (L > 1) X[L - 1].insert(parent_of(xs[i]));
+                if (br) continue;
+                node_key(h, xs[i], key.data(), st.data());
+                auto it = qid.find(key);
+                if (it == qid.end()) {
+                    const uint32_t id = (uint32_t)qkeys.size();
+                    qid.emplace(key, id);
+                    qkeys.push_back(key);
+                    qh0.push_back(h_row_hash(0x243F6A8885A308D3ull, st[0]) | 1ull);
+                    uint64_t hf = 0x243F6A8885A308D3ull;
+                    for (int t = 0; t < T; t++) hf = h_row_hash(hf, st[t]);
+                    qhf.push_back(hf | 1ull);
+                    xq[i] = id;
+                } else {
+                    xq[i] = it->second;
+                }
             }
-            oep_key_from_states(st.data(), T, key.data());
+            if (qkeys.empty()) continue;
+            std::vector<std::pair<Path, uint32_t>> m;
+            if ((rc = run_match(h, L, qh0, qhf, m))) return rc;
+            // confirm on the host, byte for byte, and bucket by query
+            std::vector<std::vector<Path>> byq(qkeys.size());
+            std::vector<uint8_t> k2((size_t)KL);
+            for (auto &pm : m) {
+                node_key(h, pm.first, k2.data(), st.data());
+                auto it = qid.find(k2);
+                if (it != qid.end()) byq[it->second].push_back(pm.first);
+                else h->stats.oep_collisions++;   // hash-equal, state-different: ignored
+            }
+            for (auto &v : byq) std::sort(v.begin(), v.end());
+            for (size_t i = 0; i < xs.size(); i++) {
+                if (xq[i] == 0xFFFFFFFFu) continue;
+                std::vector<Path> &e = earlier[xs[i]];
+                for (const Path &y : byq[xq[i]]) {
+                    if (!(y < xs[i])) break;
+                    e.push_back(y);
+                    if (L > 1) X[L - 1].insert(parent_of(y));
+                }
+            }
+        }
+        for (int L = 1; L <= k - 1; L++) {
+            for (const Path &x : X[L]) {
+                const bool ps = L == 1 ? true : survive[parent_of(x)] != 0;
+                if (!ps) { survive[x] = 0; continue; }
+                if (is_br[x]) { survive[x] = 1; continue; }
+                bool pruned = false;
+                for (const Path &y : earlier[x])
+                    if (L == 1 || survive[parent_of(y)]) { pruned = true; break; }
+                survive[x] = pruned ? 0 : 1;
+            }
+        }
+    }
+    // ---- last level: among hit nodes with a surviving parent, the first of
+    // each state (every node of a target-matching state is a hit)
+    std::vector<char> keep(nodes.size(), 0);
+    {
+        ExactKeySet set(KL);
+        for (size_t i = 0; i < nodes.size(); i++) {
+            if (k >= 2 && !survive[parent_of(nodes[i].path)]) continue;
+            if (k == 0) { keep[i] = 1; continue; }
+            node_key(h, nodes[i].path, key.data(), st.data());
             keep[i] = (char)set.insert(key.data());
         }
     }
@@ -506,32 +777,32 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
     for (size_t i = 0; i < nodes.size(); i++) {
         if (!keep[i] || nodes[i].dead) continue;
         const uint32_t solving = nodes[i].mask & nodes[i].live & 0xFEu;
+        uint16_t insns[PCAGPU_MAX_KDEPTH];
+        node_insns(h, nodes[i].path, insns, nullptr);
         for (int r = 1; r < 8; r++) {
             if (!(solving >> r & 1u)) continue;
             if (out && total < cap) {
                 pca_solution_t &so = out[total];
                 std::memset(&so, 0, sizeof(so));
-                so.prefix_rank = h->ranks[nodes[i].prefix];
+                so.prefix_rank = h->ranks[nodes[i].path.prefix];
                 int n = 0;
                 for (int q = 0; q < h->task.num_inputs; q++)
                     so.prog[n++] = enc_i(OP_IN, (uint32_t)q + 1, h->task.input_ports[q]);
-                for (int q = 0; q < D; q++) so.prog[n++] = nodes[i].insns[q];
+                for (int q = 0; q < D; q++) so.prog[n++] = insns[q];
                 so.prog[n++] = enc_i(OP_OUT, (uint32_t)r, h->task.output_ports[0]);
                 so.prog[n++] = enc_r(OP_HLT, 0, 0, 0);
                 so.len = (uint8_t)n;
                 so.out_reg = (uint8_t)r;
                 so.kernel_len = (uint8_t)D;
-                // candidate path: prefix positions are recovered from the
-                // grammar, suffix positions are the device indices
                 uint32_t live = 0;
                 for (int q = 0; q < h->task.num_inputs; q++) live |= 1u << (q + 1);
                 for (int q = 0; q < D; q++) {
                     uint16_t cands[CAND_LMAX + 64];
                     int nc = gen_candidates(cands, CAND_LMAX + 64, live, q, D);
                     int idx = 0;
-                    while (idx < nc && cands[idx] != nodes[i].insns[q]) idx++;
+                    while (idx < nc && cands[idx] != insns[q]) idx++;
                     so.cand[q] = (uint16_t)idx;
-                    live = live_after(live, nodes[i].insns[q]);
+                    live = live_after(live, insns[q]);
                 }
                 so.cpu_verified = task_evaluate(&h->task, so.prog, n) == h->task.num_tests ? 1 : 0;
             }
@@ -582,6 +853,45 @@ extern "C" long pcagpu_raw_hits(pcagpu_t *h, uint16_t *rows, long cap)
         for (int q = 0; q < k; q++) r[6 + q] = v[i].j[q];
     }
     return (long)v.size();
+}
+
+namespace {
+uint64_t host_dg_mix(uint64_t x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+}  // namespace
+
+extern "C" int pcagpu_alu_digest(pcagpu_t *h, int op, uint32_t b_lo, uint32_t b_hi, uint64_t *out)
+{
+    if (!out || b_lo > b_hi || b_hi > 65536 || op < (int)OP_ADD || op > (int)OP_CMP) return PCAGPU_EINVAL;
+    if (!h) {
+        uint64_t acc = 0;
+        for (uint32_t b = b_lo; b < b_hi; b++)
+            for (uint32_t a = 0; a < 65536; a++) {
+                uint32_t f = 0, v;
+                bool fault = false;
+                if ((uint32_t)op == OP_CMP) v = 0, f = flags_sub(a, b);
+                else v = alu_eval((uint32_t)op, a, b, 0, 0, &f, &fault);
+                if (fault) v = 0, f = 0;
+                acc += host_dg_mix((uint64_t)a | ((uint64_t)b << 16) | ((uint64_t)v << 32) | ((uint64_t)f << 48) |
+                                   ((uint64_t)fault << 56));
+            }
+        *out = acc;
+        return 0;
+    }
+    (void)hipSetDevice(h->device);
+    HIP_TRY(hipMemsetAsync(h->d_counters.p + 6, 0, sizeof(unsigned long long), h->stream));
+    launch_alu_digest((uint32_t)op, b_lo, b_hi, h->d_counters.p + 6, h->stream);
+    HIP_TRY(hipGetLastError());
+    unsigned long long v = 0;
+    HIP_TRY(hipMemcpyAsync(&v, h->d_counters.p + 6, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    *out = v;
+    return 0;
 }
 
 extern "C" int pcagpu_get_stats(pcagpu_t *h, pcagpu_stats_t *st)

@@ -91,7 +91,7 @@ def make_exact(names):
               flush=True)
 
 
-if __name__ == "__main__":
+if __name__ == "__main__" and (len(sys.argv) < 2 or sys.argv[1] != "alu"):
     what = sys.argv[1] if len(sys.argv) > 1 else "all"
     names = sys.argv[2:]
     write_generated_tasks()
@@ -99,3 +99,28 @@ if __name__ == "__main__":
         make_ref(names)
     if what in ("exact", "all"):
         make_exact(names)
+
+
+ALU_OPS = {"ADD": 2, "SUB": 3, "MUL": 4, "DIV": 5, "MOD": 6, "AND": 7, "OR": 8, "XOR": 9, "SHL": 10,
+           "SHR": 11, "ASR": 12, "CMP": 13}
+
+
+def make_alu_digests():
+    """alu_digests.json: the oracle's exhaustive digest of every ALU opcode
+    over all 2^32 operand pairs (see or_alu_digest)."""
+    import ctypes as C
+    L = O.lib()
+    L.or_alu_digest.restype = C.c_uint64
+    L.or_alu_digest.argtypes = [C.c_int, C.c_uint32, C.c_uint32]
+    out = {}
+    for name, op in ALU_OPS.items():
+        out[name] = {"op": op, "full": str(L.or_alu_digest(op, 0, 65536)),
+                     "b_0_256": str(L.or_alu_digest(op, 0, 256)),
+                     "b_65280_65536": str(L.or_alu_digest(op, 65280, 65536))}
+        print("alu", name, flush=True)
+    with open(os.path.join(HERE, "alu_digests.json"), "w") as f:
+        json.dump(out, f, indent=1)
+
+
+if __name__ == "__main__" and len(sys.argv) > 1 and sys.argv[1] == "alu":
+    make_alu_digests()

@@ -1,0 +1,251 @@
+"""CPU tests: the oracle pinned against the reference binary's outputs and
+known answers, the product's host logic against the oracle, and the C ABI
+surface.  No device calls here."""
+import ctypes as C
+import json
+import os
+import random
+import re
+import subprocess
+
+import pytest
+
+from conftest import GOLDEN, ROOT, golden_task_path
+
+REF_DIR = os.path.join(GOLDEN, "ref")
+EXACT_DIR = os.path.join(GOLDEN, "exact")
+
+
+def _mask(text):
+    text = re.sub(r"\([0-9.]+ M/sec\)", "(X M/sec)", text)
+    text = re.sub(r"elapsed: [0-9.]+ sec", "elapsed: X sec", text)
+    return re.sub(r"in [0-9.]+ sec", "in X sec", text)
+
+
+# ----------------------------------------------------------------- oracle
+def _ref_cases():
+    out = []
+    for f in sorted(os.listdir(REF_DIR)):
+        tag = f[:-4]
+        name, rest = tag.split("_d_", 1) if "_d_" in tag else (tag, "")
+        parts = rest.split("_")
+        args = ["-d", parts[0]] + ["--" + p for p in parts[1:]]
+        out.append((name, args, f))
+    return out
+
+
+@pytest.mark.parametrize("name,args,fname", _ref_cases())
+def test_oracle_ref32_is_byte_identical_to_reference_binary(oracle, name, args, fname):
+    """The oracle's 32-bit-table mode reproduces the reference enumerate's
+    stdout (tools/enumerate.c) byte for byte, wall-clock figures masked."""
+    cli = os.path.join(ROOT, "oracle", "oracle_enumerate")
+    out = subprocess.run([cli, golden_task_path(name), "--ref32", "--no-shadow"] + args,
+                         capture_output=True, text=True).stdout
+    with open(os.path.join(REF_DIR, fname)) as f:
+        want = f.read()
+    assert _mask(out) == want
+
+
+# the published optimal program lengths (README.md:41-51) as kernel depths
+KNOWN_LENGTHS = {"negate": 4, "double": 4, "square": 4, "add": 5, "abs": 6, "relu": 6}
+
+
+@pytest.mark.parametrize("fname", sorted(os.listdir(EXACT_DIR)))
+def test_exact_fixtures_agree_with_known_answers(oracle, fname):
+    with open(os.path.join(EXACT_DIR, fname)) as f:
+        rec = json.load(f)
+    t = oracle.load_task(golden_task_path(rec["task"]))
+    if rec["task"] in KNOWN_LENGTHS:
+        assert t.num_inputs + rec["optimal_depth"] + 2 == KNOWN_LENGTHS[rec["task"]]
+    # every fixture solution passes every test on the oracle VM
+    for prog in rec["solutions"]:
+        assert oracle.evaluate(t, prog) == t.num_tests
+    # runs with no 32-bit table events equal the reference semantics
+    if sum(rec["ref32_events"]) == 0:
+        assert rec["equals_ref32"]
+
+
+@pytest.mark.parametrize("name,d", [("negate", 3), ("add", 3), ("negate_16", 5), ("is_power2_32", 2),
+                                    ("clamp", 2)])
+def test_exact_fixture_recomputes(oracle, name, d):
+    with open(os.path.join(EXACT_DIR, f"{name}_d{d}.json")) as f:
+        rec = json.load(f)
+    t = oracle.load_task(golden_task_path(name))
+    r = oracle.solve(t, d, oracle.OEP_EXACT, max_solutions=0)
+    assert r["optimal_depth"] == rec["optimal_depth"]
+    assert r["solutions"] == rec["solutions"]
+    assert r["tried"][:d + 1] == rec["tried"]
+
+
+def test_oracle_grammar_counts(oracle):
+    """Candidate counts re-derived from tools/enumerate.c:206-267 and
+    src/search_state.c:280-330: 244 / 525 / 3,745 (branch mode, BR off) and
+    214 / 435 / 2,905 (branchless) for 1, 2 and 7 live registers."""
+    for live, n, nb in ((0b10, 244, 214), (0b110, 525, 435), (0xFE, 3745, 2905)):
+        assert len(oracle.gen_candidates(live, 0, 1)) == n
+        assert len(oracle.gen_candidates_branchless(live)) == nb
+    # BR block: 6 conditions x (D - pos - 1) offsets
+    assert len(oracle.gen_candidates(0b10, 0, 4)) == 244 + 6 * 3
+
+
+def test_oracle_fib_program(oracle):
+    """programs/fib.s computes 55 (Makefile:58-59): a loop with BNZ on the VM."""
+    # r1 = 0, r2 = 1, r3 = 10; loop: r4 = r1 + r2; r1 = r2; r2 = r4; r3 -= 1; BNZ loop
+    prog = [
+        (16 << 11) | (1 << 8) | 0, (16 << 11) | (2 << 8) | 1, (16 << 11) | (3 << 8) | 10,
+        (2 << 11) | (4 << 8) | (1 << 5) | (2 << 2), (2 << 11) | (1 << 8) | (2 << 5),
+        (2 << 11) | (2 << 8) | (4 << 5), (18 << 11) | (3 << 8) | 0xFF, (19 << 11) | (2 << 8) | 0xFB,
+        (24 << 11) | (1 << 8), 1 << 11,
+    ]
+    t = oracle.parse_task('{"name":"fib","max_cycles":1000,"input_ports":[],"output_ports":[0],'
+                          '"tests":[{"in":[],"out":[55]}]}')
+    assert oracle.evaluate(t, prog) == 1
+
+
+def test_oracle_div_mod_closed_forms(oracle):
+    """DIV/MOD are C's truncating signed ops after int promotion
+    (src/vm.c:94-103); /0 faults.  Closed forms on the edge cases."""
+    def sdiv(a, b):
+        a = a - 65536 if a & 0x8000 else a
+        b = b - 65536 if b & 0x8000 else b
+        q = abs(a) // abs(b)
+        return (q if (a < 0) == (b < 0) else -q) & 0xFFFF, (a - (q if (a < 0) == (b < 0) else -q) * b) & 0xFFFF
+    rng = random.Random(3)
+    vals = [0, 1, 2, 3, 0x7FFF, 0x8000, 0x8001, 0xFFFF, 0xFFFE] + [rng.randrange(65536) for _ in range(60)]
+    for a in vals:
+        for b in vals:
+            v, f, fault = oracle.alu(5, a, b)
+            w, f2, fault2 = oracle.alu(6, a, b)
+            if b == 0:
+                assert fault and fault2
+                continue
+            q, r = sdiv(a, b)
+            assert (v, w) == (q, r), (a, b)
+            assert f == ((1 if q == 0 else 0) | (2 if q & 0x8000 else 0))
+    assert oracle.alu(5, 0x8000, 0xFFFF)[0] == 0x8000      # -32768 / -1
+    assert oracle.alu(6, 0x8000, 0xFFFF)[0] == 0
+
+
+def test_oracle_flag_closed_forms(oracle):
+    """ADD/SUB/CMP flags (src/vm.c:35-54): Z N C V on corner operands."""
+    Z, N, Cf, V = 1, 2, 4, 8
+    assert oracle.alu(2, 0xFFFF, 1)[1] == Z | Cf
+    assert oracle.alu(2, 0x7FFF, 1)[1] == N | V
+    assert oracle.alu(3, 0, 1)[1] == N                   # borrow: C = (a >= b) is 0
+    assert oracle.alu(3, 5, 5)[1] == Z | Cf
+    assert oracle.alu(3, 0x8000, 1)[1] == Cf | V
+    assert oracle.alu(4, 0x100, 0x100)[1] == Z | Cf       # 0x10000 overflows 16 bits
+    assert oracle.alu(13, 3, 7)[1] == oracle.alu(3, 3, 7)[1]
+
+
+# ----------------------------------------------------------- product host
+def test_abi_exports_every_declared_symbol(pca):
+    with open(os.path.join(ROOT, "include", "pcagpu.h")) as f:
+        hdr = f.read()
+    names = set(re.findall(r"\b(pcagpu_[a-z_0-9]+)\s*\(", hdr))
+    assert len(names) >= 15
+    lib = C.CDLL(os.path.join(ROOT, "pca_amd", "libpcagpu.so"))
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    from pca_amd._lib import SIGNATURES
+    assert names == {s[0] for s in SIGNATURES}
+
+
+def test_struct_layouts(pca):
+    assert C.sizeof(pca.Task) == 1128        # task_spec_t (tasks/spec.h:20-34)
+    assert C.sizeof(pca.Prefix) == 18        # prefix_t (gpu/hybrid.cu:56-60)
+
+
+@pytest.mark.parametrize("name", sorted(n[:-5] for n in os.listdir(os.path.join(GOLDEN, "tasks"))))
+def test_task_loader_matches_oracle(pca, oracle, name):
+    t = pca.Task.load(golden_task_path(name))
+    assert bytes(t) == bytes(oracle.load_task(golden_task_path(name)))
+
+
+@pytest.mark.parametrize("text", [
+    '{"name":"x","input_ports":[0],"output_ports":[0],"tests":[{"in":[-1],"out":[70000]}]}',
+    '{"name":"no_cycles","tests":[{"in":[1,2],"out":[3]},{"out":[4],"in":[5]}]}',
+    '{"name":"many","max_cycles":7,"input_ports":[1,0],"output_ports":[2],"tests":[' +
+    ",".join('{"in":[%d,%d],"out":[%d]}' % (i, i * 3, i * 7) for i in range(40)) + "]}",
+    '{"name":"ports","input_ports":[0,1,2,3,4,5,6,7,8,9],"output_ports":[0],"tests":[]}',
+])
+def test_task_parser_quirks_match_oracle(pca, oracle, text):
+    """Defaults, 32-test cap, & 0xFFFF, key order, extra ports
+    (tasks/spec.c:80-184)."""
+    assert bytes(pca.Task.parse(text)) == bytes(oracle.parse_task(text))
+
+
+def test_candidate_lists_match_oracle_all_masks(pca, oracle):
+    for m in range(128):
+        live = m << 1
+        for pos, D in ((0, 1), (0, 4), (2, 4), (5, 8)):
+            assert pca.gen_candidates(live, pos, D) == oracle.gen_candidates(live, pos, D)
+
+
+def test_disassembler_matches_oracle_exhaustively(pca, oracle):
+    for w in range(0, 65536, 3):
+        assert pca.disasm(w) == oracle.disasm(w)
+
+
+def test_task_evaluate_matches_oracle_random_programs(pca, oracle):
+    """Random programs over the whole ISA (memory, calls, loops, faults) on
+    the product's re-verification VM vs the oracle VM."""
+    t = pca.Task.load(golden_task_path("max"))
+    ot = oracle.task_from_bytes(bytes(t))
+    rng = random.Random(9)
+    for _ in range(3000):
+        n = rng.randrange(1, 14)
+        prog = [rng.randrange(65536) for _ in range(n)] + [1 << 11]
+        assert t.evaluate(prog) == oracle.evaluate(ot, prog)
+    # programs the search grammar makes, with their true outputs as targets
+    for _ in range(500):
+        prog = [(23 << 11) | (1 << 8), (23 << 11) | (2 << 8) | 1]
+        for _ in range(rng.randrange(1, 6)):
+            prog.append(rng.choice(pca.gen_candidates(0b11111110, 0, 8)))
+        prog += [(24 << 11) | (rng.randrange(1, 8) << 8), 1 << 11]
+        assert t.evaluate(prog) == oracle.evaluate(ot, prog)
+
+
+@pytest.mark.parametrize("name,D,s", [("abs", 3, 2), ("max", 3, 2), ("is_power2", 4, 2), ("ident0", 3, 2),
+                                      ("negate_16", 3, 2), ("relu", 4, 1)])
+def test_host_prefix_producer_matches_oracle(pca, oracle, name, D, s):
+    """The product's exact-OEP DFS (incremental state) emits the same
+    prefixes, in the same order, as the oracle's replay-based DFS."""
+    t = pca.Task.load(golden_task_path(name))
+    got, nodes, pruned = pca.host_frontier(t, D, s)
+    want = oracle.frontier(oracle.task_from_bytes(bytes(t)), D, s)
+    assert got == want
+
+
+def _alu_golden():
+    with open(os.path.join(GOLDEN, "alu_digests.json")) as f:
+        return json.load(f)
+
+
+@pytest.mark.parametrize("opname", ["ADD", "SUB", "MUL", "DIV", "MOD", "AND", "OR", "XOR", "SHL", "SHR",
+                                    "ASR", "CMP"])
+def test_host_alu_digest_matches_oracle(pca, opname):
+    """Product ALU (isa.h, host build) vs the oracle on every a and the
+    b ranges [0,256) and [65280,65536)."""
+    from pca_amd._lib import alu_digest
+    g = _alu_golden()[opname]
+    assert str(alu_digest(g["op"], 0, 256)) == g["b_0_256"]
+    assert str(alu_digest(g["op"], 65280, 65536)) == g["b_65280_65536"]
+
+
+def test_taskgen_workloads(pca):
+    from pca_amd import taskgen
+    t = taskgen.negate_16(42)
+    assert t.num_tests == 16
+    assert [t.tests[i].inp[0] for i in range(5)] == [0, 1, 0x7FFF, 0x8000, 0xFFFF]
+    p = taskgen.is_power2_32(42)
+    xs = [p.tests[i].inp[0] for i in range(32)]
+    assert len(set(xs)) == 32 and all(p.tests[i].out[0] == taskgen.is_power2(xs[i]) for i in range(32))
+    # the committed copies are what the generator makes
+    assert bytes(pca.Task.load(golden_task_path("is_power2_32"))) == bytes(p)
+    assert bytes(pca.Task.load(golden_task_path("negate_16"))) == bytes(t)
+    # xoshiro256** known answer (seed 42, splitmix64 seeding): stable first outputs
+    r = taskgen.Xoshiro256(42)
+    first = [r.next() for _ in range(3)]
+    r2 = taskgen.Xoshiro256(42)
+    assert first == [r2.next() for _ in range(3)]

@@ -179,3 +179,53 @@ def test_scan_raw_hits_with_shadow_hits(pca, oracle, name, D):
         raw = sv.raw_depth(D, s, [prefixes[i] for i in idxs])
     got = [(p, js, mask, flags & 1) for p, flags, mask, js in raw]
     assert got == want
+
+
+@pytest.mark.parametrize("name,maxd,split", [("abs", 3, 1), ("abs", 3, 0), ("relu", 3, 1), ("relu", 3, 0),
+                                             ("sign", 3, 1), ("negate_16", 5, 0), ("is_power2_32", 2, 0),
+                                             ("max", 2, 0)])
+def test_solve_any_split_matches_exact_fixture(pca, name, maxd, split):
+    """Suffixes longer than one instruction are brute-forced on the device
+    and then cut down to the exact OEP set (state-match passes + host
+    resolution): the result must not depend on the split."""
+    want = _exact_fixture(name, maxd)
+    t = pca.Task.load(golden_task_path(name))
+    print(f"[solve-split] {name} d{maxd} split={split}", flush=True)
+    with pca.Solver(t) as sv:
+        res = sv.solve(maxd, split=split, cap=1 << 16)
+    assert res.optimal_depth == want["optimal_depth"]
+    assert res.num_solutions == want["num_solutions"]
+    assert [s[0] for s in res.solutions] == want["solutions"]
+
+
+@pytest.mark.parametrize("opname", ["ADD", "SUB", "MUL", "DIV", "MOD", "AND", "OR", "XOR", "SHL", "SHR",
+                                    "ASR", "CMP"])
+def test_device_alu_exhaustive_matches_oracle(pca, opname):
+    """Every (a, b) in 2^16 x 2^16 on the device vs the oracle's digest."""
+    import json
+    import os
+    from conftest import GOLDEN
+    from pca_amd._lib import alu_digest
+    with open(os.path.join(GOLDEN, "alu_digests.json")) as f:
+        g = json.load(f)[opname]
+    t = pca.Task.load(golden_task_path("negate"))
+    with pca.Solver(t) as sv:
+        assert str(alu_digest(g["op"], 0, 65536, solver=sv)) == g["full"]
+
+
+def test_cli_hybrid_drop_in(pca):
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "pca_amd", "gpu_hybrid")
+    out = subprocess.run([exe, golden_task_path("abs"), "4", "5"], capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    assert "Optimal solution length: 6 instructions (kernel depth 3)" in out.stdout
+    assert out.stdout.count("cpu_verify=8/8 OK") == 5
+    for exe_name in ("gpu_wavefront", "gpu_enumerate_oep"):
+        o2 = subprocess.run([os.path.join(ROOT, "pca_amd", exe_name), golden_task_path("abs"), "4", "5"],
+                            capture_output=True, text=True, timeout=120)
+        assert o2.returncode == 0, o2.stderr
+        # the same canonical programs, whichever producer made the prefixes
+        assert o2.stdout.split("Optimal solution length")[1].split("Total:")[0] == \
+            out.stdout.split("Optimal solution length")[1].split("Total:")[0]
