@@ -299,7 +299,34 @@ int gen_candidates(uint16_t *out, int max, uint32_t live, int pos, int kernel_de
 void CandTable::build()
 {
     list.assign((size_t)128 * CAND_LMAX, 0);
-    for (int m = 0; m < 128; m++) n[m] = build_list(&list[(size_t)m * CAND_LMAX], (uint32_t)m << 1, &a[m]);
+    vg_insn.assign((size_t)128 * VG_MAX, 0);
+    vg_rdmask.assign((size_t)128 * VG_MAX, 0);
+    vg_j.assign((size_t)128 * VG_MAX * 8, 0xFFFF);
+    for (int m = 0; m < 128; m++) {
+        const uint32_t live = (uint32_t)m << 1;
+        uint16_t *L = &list[(size_t)m * CAND_LMAX];
+        n[m] = build_list(L, live, &a[m]);
+        cand_total[m] = 0;
+        std::vector<uint16_t> keys;
+        for (int j = 0; j < n[m]; j++) {
+            const uint32_t c = L[j];
+            cand_total[m] += (uint32_t)__builtin_popcount(live_after(live, c) & 0xFEu);
+            const uint32_t op = insn_op(c), rd = insn_rd(c);
+            if (op == OP_CMP) continue;
+            // the value written depends on the opcode and sources only,
+            // except ADDI, whose source is rd itself
+            const uint16_t key = op == OP_ADDI ? (uint16_t)c : (uint16_t)(c & ~(7u << 8));
+            size_t g = 0;
+            while (g < keys.size() && keys[g] != key) g++;
+            if (g == keys.size()) {
+                keys.push_back(key);
+                vg_insn[(size_t)m * VG_MAX + g] = key;
+            }
+            vg_rdmask[(size_t)m * VG_MAX + g] |= (uint8_t)(1u << rd);
+            vg_j[((size_t)m * VG_MAX + g) * 8 + rd] = (uint16_t)j;
+        }
+        vg_n[m] = (int)keys.size();
+    }
 }
 
 void root_states(const pca_task_t *t, TestState *out)

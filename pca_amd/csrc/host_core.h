@@ -44,10 +44,20 @@ int gen_candidates(uint16_t *out, int max, uint32_t live, int pos, int kernel_de
 // the count.  Position p of a depth-D tree holds list(m)[0:a] ++
 // BR(cond 1..6 x off 1..mf) ++ list(m)[a:], mf = D - p - 1.
 constexpr int CAND_LMAX = 3776;   // >= 3,745 (all registers live)
+// Last-position "value groups": candidates that differ only in rd write the
+// same value vector (op applied to the same sources), so the device tests
+// each distinct (op, sources, immediate) once and keeps a mask of the rd
+// values still possible.  CMP is left out (it writes no register).
+constexpr int VG_MAX = 640;        // >= distinct value groups of any mask
 struct CandTable {
     std::vector<uint16_t> list;   // [128][CAND_LMAX]
     int n[128];
     int a[128];
+    std::vector<uint16_t> vg_insn;   // [128][VG_MAX] representative insn (rd field = 0 unless ADDI)
+    std::vector<uint8_t> vg_rdmask;  // [128][VG_MAX] rd values present in the list
+    std::vector<uint16_t> vg_j;      // [128][VG_MAX][8] list index of (group, rd)
+    int vg_n[128];
+    uint32_t cand_total[128];        // sum over the list of live out-registers
     void build();
     int count(uint32_t live, int mf) const { return n[(live >> 1) & 127] + (mf > 0 ? 6 * mf : 0); }
     uint16_t at(uint32_t live, int mf, int j) const

@@ -38,10 +38,19 @@ struct HitRec {
 };
 static_assert(sizeof(HitRec) == 24, "HitRec layout");
 
+constexpr int VGMAX = 640;                      // == VG_MAX
+
 struct CandDev {
     const uint16_t *lists;   // [128][LMAX]
     const int *count;        // [128]
     const int *split;        // [128]
+    // last-position value groups (see CandTable): representative insn, rd
+    // mask, list index per (group, rd), group count, total out-registers
+    const uint16_t *vg_insn;       // [128][VGMAX]
+    const uint8_t *vg_rdmask;      // [128][VGMAX]
+    const uint16_t *vg_j;          // [128][VGMAX][8]
+    const int *vg_n;               // [128]
+    const uint32_t *cand_total;    // [128]
 };
 
 // Work units of a scan launch.  k <= 1: unit = prefix.  k >= 2: units are
@@ -119,6 +128,7 @@ struct ExpandArgs {
 
 void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out, hipStream_t st);
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
+hipError_t scan_occupancy(int *blocks_per_cu);
 void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st);
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st);
 void launch_expand_insert(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);

@@ -166,26 +166,181 @@ __device__ __forceinline__ void pair_eval(const DevTask &task, const TestState *
 // still alive are pushed (ballot prefix count) to a per-wave LDS queue, and
 // stage B runs the full early-exit test loop 64 queued survivors at a time,
 // so lanes that died on test 0 do not idle through the slow lanes' tests.
-__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P,
-                                            uint32_t live, uint32_t prefix_idx, const uint16_t *jpath, int lane,
-                                            WaveCounters &wc, uint16_t *queue)
+struct ParentSummary {
+    uint32_t pm;       // registers r1..r7 equal to the target on every test
+    uint64_t skipm;    // tests on which the last instruction does not execute
+    uint32_t dead;     // some test already faulted or cannot fit max_cycles
+};
+
+// Lanes = tests.  Also leaves per test the mask of registers equal to the
+// target (regmatch) in LDS for the value-group scan.
+__device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, const TestState *P, uint32_t pos,
+                                                          int lane, uint8_t *regmatch)
 {
     const int T = task.T;
-    const uint32_t pos = (uint32_t)(a.D - 1);
-    // ---- parent summary, lanes = tests
     const bool act = lane < T;
     TestState st;
     if (act) st = P[lane];
     const uint32_t tgt = act ? task.tgt[lane] : 0u;
-    uint32_t pm = 0;
-    for (int r = 1; r < 8; r++) pm |= (__ballot(act && st.r[r] != tgt) == 0ull ? 1u : 0u) << r;
-    // tests on which the last instruction does not execute (skipped by a
-    // pending BR, or the VM already stopped)
+    ParentSummary s;
+    uint32_t pm = 0, rm = 0;
+    for (int r = 1; r < 8; r++) {
+        const bool eq = act && st.r[r] == tgt;
+        rm |= (eq ? 1u : 0u) << r;
+        pm |= (__ballot(act && !eq) == 0ull ? 1u : 0u) << r;
+    }
+    if (act) regmatch[lane] = (uint8_t)rm;
+    s.pm = pm;
+    // skipped by a pending BR, or the VM already stopped
     const bool noexec = act && (st.frozen || st.resume > pos);
-    const uint64_t skipm = __ballot(noexec);
+    s.skipm = __ballot(noexec);
     const bool cyc = act && ((uint32_t)st.exec + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
-    const uint32_t dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
-    wc.parents += lane == 0;
+    s.dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
+    return s;
+}
+
+__device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const uint16_t *jpath,
+                                         uint32_t jlast, uint32_t mask, uint32_t flags)
+{
+    const uint32_t slot = atomicAdd(a.hit_count, 1u);
+    if (slot < a.hit_cap) {
+        HitRec h;
+        h.prefix = prefix_idx;
+        h.mask = (uint8_t)mask;
+        h.flags = (uint8_t)flags;
+        for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
+        for (int q = 0; q + 1 < a.k; q++) h.j[q] = jpath[q];
+        h.j[a.k - 1] = (uint16_t)jlast;
+        h.pad = 0;
+        a.hits[slot] = h;
+    }
+}
+
+// Value a value-group representative writes on one test; DIV/MOD only when
+// the wave holds any (the caller guards), everything else branch-free.
+__device__ __forceinline__ uint32_t group_value(uint32_t op, uint32_t x, uint32_t y, uint32_t imm)
+{
+    const uint32_t sh = y & 15u;
+    uint32_t v = x + y;
+    v = op == OP_SUB ? x - y : v;
+    v = op == OP_MUL ? x * y : v;
+    v = op == OP_AND ? (x & y) : v;
+    v = op == OP_OR ? (x | y) : v;
+    v = op == OP_XOR ? (x ^ y) : v;
+    v = op == OP_SHL ? x << sh : v;
+    v = op == OP_SHR ? x >> sh : v;
+    v = op == OP_ASR ? (uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) : v;
+    v = op == OP_LDI ? imm : v;
+    v = op == OP_ADDI ? x + (uint32_t)(int32_t)(int8_t)(uint8_t)imm : v;
+    return v & 0xFFFFu;
+}
+
+// Parent with no target-matching register (the common case): scan the
+// last position by value groups.  A lane owns one (op, sources, imm) group
+// and a mask `ok` of the rd values that still match: a test on which the
+// instruction runs keeps all or none of them, a test on which it does not
+// (skipped, VM stopped, DIV/MOD fault) keeps those rd whose old value
+// already equals the target (regmatch).  Tests 0-1 run for every lane; the
+// survivors go through an LDS queue to the early-exit loop over the rest.
+__device__ __forceinline__ void scan_parent_groups(const DevTask &task, const ScanArgs &a, const TestState *P,
+                                                   uint32_t live, uint32_t prefix_idx, const uint16_t *jpath,
+                                                   int lane, WaveCounters &wc, uint16_t *queue,
+                                                   const uint8_t *regmatch, const ParentSummary &ps)
+{
+    const int T = task.T;
+    const int m = (live >> 1) & 127;
+    const int ng = a.cand.vg_n[m];
+    const uint16_t *G = a.cand.vg_insn + m * VGMAX;
+    const uint8_t *GM = a.cand.vg_rdmask + m * VGMAX;
+    const uint16_t *GJ = a.cand.vg_j + (size_t)m * VGMAX * 8;
+    if (lane == 0) {
+        wc.cand += a.cand.cand_total[m];
+        wc.suf += (unsigned)a.cand.count[m];
+    }
+    uint32_t n_tests = 0;
+    uint32_t qn = 0;
+    const int T0 = T < 2 ? T : 2;
+
+    // evaluates tests [t0, t1) for the lane's group, updating ok / fault
+    auto run_tests = [&](uint32_t g, int t0, int t1, uint32_t &ok, bool &fault, bool early) {
+        const uint32_t op = insn_op(g), rd = insn_rd(g), rs = insn_rs(g), rt = insn_rt(g), imm = insn_imm(g);
+        const uint32_t sa = op == OP_ADDI ? rd : rs;
+        const bool divmod = op == OP_DIV || op == OP_MOD;
+        const bool any_div = __any(divmod && ok != 0u);
+        for (int t = t0; t < t1; t++) {
+            if (early && !__any(ok != 0u)) break;
+            if (ok != 0u) {
+                const uint32_t x = P[t].r[sa], y = P[t].r[rt];
+                uint32_t v = group_value(op, x, y, imm);
+                if (any_div && divmod && y != 0u) v = op == OP_DIV ? sdiv16(x, y) : smod16(x, y);
+                const bool ran = !((ps.skipm >> t) & 1ull);
+                const bool f = ran && divmod && y == 0u;
+                fault = fault || f;
+                ok &= (ran && !f) ? (v == task.tgt[t] ? 0xFFu : 0u) : regmatch[t];
+                n_tests++;
+            }
+        }
+    };
+    auto drain = [&](uint32_t cnt) {
+        const bool valid = (uint32_t)lane < cnt;
+        const uint32_t gi = valid ? queue[lane] : 0u;
+        const uint32_t g = valid ? G[gi] : 0u;
+        uint32_t ok = valid ? GM[gi] : 0u;
+        bool fault = false;
+        run_tests(g, 0, T, ok, fault, true);
+        if (ok) {
+            const uint32_t fl = (ps.dead || fault) ? 1u : 0u;
+            for (uint32_t r = 1; r < 8; r++)
+                if (ok >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[gi * 8 + r], 1u << r, fl);
+        }
+    };
+    for (int base = 0; base < ng; base += 64) {
+        const int gi = base + lane;
+        const bool valid = gi < ng;
+        const uint32_t g = valid ? G[gi] : 0u;
+        uint32_t ok = valid ? GM[gi] : 0u;
+        bool fault = false;
+        run_tests(g, 0, T0, ok, fault, false);
+        const bool alive = ok != 0u;
+        if (T <= 2) {
+            if (alive) {
+                const uint32_t fl = (ps.dead || fault) ? 1u : 0u;
+                for (uint32_t r = 1; r < 8; r++)
+                    if (ok >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[gi * 8 + r], 1u << r, fl);
+            }
+            continue;
+        }
+        const uint64_t sb = __ballot(alive);
+        if (alive) queue[qn + (uint32_t)__popcll(sb & ((1ull << lane) - 1ull))] = (uint16_t)gi;
+        qn += (uint32_t)__popcll(sb);
+        if (qn >= 64) {
+            wave_sync();
+            drain(64);
+            wave_sync();
+            const uint32_t rest = qn - 64;
+            if ((uint32_t)lane < rest) queue[lane] = queue[64 + lane];
+            qn = rest;
+            wave_sync();
+        }
+    }
+    if (qn) {
+        wave_sync();
+        drain(qn);
+        wave_sync();
+    }
+    wc.tests += n_tests;
+}
+
+// Parent where some register already equals the target: every candidate
+// that leaves it alone is a hit, so go candidate by candidate.
+__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P,
+                                            uint32_t live, uint32_t prefix_idx, const uint16_t *jpath, int lane,
+                                            WaveCounters &wc, uint16_t *queue, const ParentSummary &ps)
+{
+    const int T = task.T;
+    const uint32_t pm = ps.pm;
+    const uint64_t skipm = ps.skipm;
+    const uint32_t dead = ps.dead;
 
     const int m = (live >> 1) & 127;
     const int n = a.cand.count[m];
@@ -379,6 +534,7 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
     __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
     __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
     __shared__ uint16_t s_queue[SCAN_WAVES][128];
+    __shared__ uint8_t s_rm[SCAN_WAVES][MAXT];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = a.k;
     // level of the parent a unit designates: 0 (prefix), 1 (+ position s),
@@ -453,7 +609,17 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
             for (;;) {
                 if (L == k - 1) {
                     for (int q = 1; q <= L; q++) jpath[q - 1] = (uint16_t)uni(s_idx[w][q]);
-                    scan_parent(task, a, s_st[w][L], uni(s_live[w][L]), pi, jpath, lane, wc, s_queue[w]);
+                    {
+                        const TestState *PP = s_st[w][L];
+                        const uint32_t lvL = uni(s_live[w][L]);
+                        const ParentSummary ps = summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, s_rm[w]);
+                        wc.parents += lane == 0;
+                        wave_sync();
+                        if (ps.pm == 0u)
+                            scan_parent_groups(task, a, PP, lvL, pi, jpath, lane, wc, s_queue[w], s_rm[w], ps);
+                        else
+                            scan_parent(task, a, PP, lvL, pi, jpath, lane, wc, s_queue[w], ps);
+                    }
                     wave_sync();
                     if (L == Lu) break;
                     L--;
@@ -982,6 +1148,11 @@ void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long 
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st)
 {
     hipLaunchKernelGGL(table_init_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table, cap);
+}
+
+hipError_t scan_occupancy(int *blocks_per_cu)
+{
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel, SCAN_WG, 0);
 }
 
 void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st)

@@ -81,6 +81,10 @@ struct pcagpu {
 
     DevBuf<uint16_t> d_lists;
     DevBuf<int> d_count, d_split;
+    DevBuf<uint16_t> d_vg_insn, d_vg_j;
+    DevBuf<uint8_t> d_vg_rdmask;
+    DevBuf<int> d_vg_n;
+    DevBuf<uint32_t> d_cand_total;
     DevBuf<pca_prefix_t> d_batch;
     DevBuf<ScanGroup> d_groups;
     DevBuf<uint64_t> d_goff;
@@ -166,6 +170,12 @@ int fill_devtask(pcagpu *h)
     return 0;
 }
 
+CandDev cand_dev(const pcagpu *h)
+{
+    return CandDev{h->d_lists.p, h->d_count.p, h->d_split.p, h->d_vg_insn.p, h->d_vg_rdmask.p,
+                   h->d_vg_j.p, h->d_vg_n.p, h->d_cand_total.p};
+}
+
 int grid_for(pcagpu *h, uint64_t units)
 {
     uint64_t waves = (uint64_t)h->cu_count * h->waves_per_cu;
@@ -185,7 +195,7 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
     a.D = h->D;
     a.s = h->s;
     a.k = h->k;
-    a.cand = CandDev{h->d_lists.p, h->d_count.p, h->d_split.p};
+    a.cand = cand_dev(h);
     a.hits = h->d_hits.p;
     a.hit_cap = h->hit_cap;
     a.hit_count = h->d_hit_count.p;
@@ -300,6 +310,12 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PCAGPU_EHIP);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(PCAGPU_EHIP);
+    if (!h->opts.waves_per_cu) {
+        // persistent grid: as many resident workgroups as the scan kernel's
+        // register and LDS use allow on one CU
+        int blocks = 0;
+        if (scan_occupancy(&blocks) == hipSuccess && blocks > 0) h->waves_per_cu = (uint32_t)blocks * SCAN_WAVES;
+    }
     h->ct.build();
     fill_devtask(h);
     if ((rc = h->d_lists.ensure(h->ct.list.size())) || (rc = h->d_count.ensure(128)) ||
@@ -317,6 +333,16 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
         hipMemcpy(h->d_split.p, h->ct.a, sizeof(h->ct.a), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
         return fail(PCAGPU_EHIP);
+    if ((rc = h->d_vg_insn.ensure(h->ct.vg_insn.size())) || (rc = h->d_vg_j.ensure(h->ct.vg_j.size())) ||
+        (rc = h->d_vg_rdmask.ensure(h->ct.vg_rdmask.size())) || (rc = h->d_vg_n.ensure(128)) ||
+        (rc = h->d_cand_total.ensure(128)))
+        return fail(rc);
+    if (hipMemcpy(h->d_vg_insn.p, h->ct.vg_insn.data(), h->ct.vg_insn.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_vg_j.p, h->ct.vg_j.data(), h->ct.vg_j.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_vg_rdmask.p, h->ct.vg_rdmask.data(), h->ct.vg_rdmask.size(), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_vg_n.p, h->ct.vg_n, sizeof(h->ct.vg_n), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(h->d_cand_total.p, h->ct.cand_total, sizeof(h->ct.cand_total), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PCAGPU_EHIP);
     *out = h;
     return 0;
 }
@@ -329,6 +355,11 @@ void pcagpu_close(pcagpu_t *h)
     h->d_lists.release();
     h->d_count.release();
     h->d_split.release();
+    h->d_vg_insn.release();
+    h->d_vg_j.release();
+    h->d_vg_rdmask.release();
+    h->d_vg_n.release();
+    h->d_cand_total.release();
     h->d_batch.release();
     h->d_groups.release();
     h->d_goff.release();
@@ -613,7 +644,7 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
                 a.D = h->D;
                 a.s = h->s;
                 a.kk = kk;
-                a.cand = CandDev{h->d_lists.p, h->d_count.p, h->d_split.p};
+                a.cand = cand_dev(h);
                 a.q0 = h->d_q0.p;
                 a.q0_mask = cap0 - 1;
                 a.qf = h->d_qf.p;
@@ -928,7 +959,7 @@ extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, lo
         a.n_parents = n;
         a.D = D;
         a.pos = pos;
-        a.cand = CandDev{h->d_lists.p, h->d_count.p, h->d_split.p};
+        a.cand = cand_dev(h);
         if ((rc = h->d_child_off.ensure(n + 1)) || (rc = h->d_keep_cnt.ensure(n + 1)) ||
             (rc = h->d_scan_tmp.ensure(exclusive_scan_tmp_words(n) + 16)))
             return rc;
