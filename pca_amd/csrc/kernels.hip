@@ -199,20 +199,20 @@ __device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, c
     return s;
 }
 
-__device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const uint16_t *jpath,
+// jpath: the wave's LDS row of candidate indices at positions s..D-2.
+// The record is written field by field (no private copy).
+__device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const uint32_t *jpath,
                                          uint32_t jlast, uint32_t mask, uint32_t flags)
 {
     const uint32_t slot = atomicAdd(a.hit_count, 1u);
     if (slot < a.hit_cap) {
-        HitRec h;
-        h.prefix = prefix_idx;
-        h.mask = (uint8_t)mask;
-        h.flags = (uint8_t)flags;
-        for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
-        for (int q = 0; q + 1 < a.k; q++) h.j[q] = jpath[q];
-        h.j[a.k - 1] = (uint16_t)jlast;
-        h.pad = 0;
-        a.hits[slot] = h;
+        HitRec *h = a.hits + slot;
+        h->prefix = prefix_idx;
+        h->mask = (uint8_t)mask;
+        h->flags = (uint8_t)flags;
+        h->pad = 0;
+        for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++)
+            h->j[q] = (uint16_t)(q + 1 < a.k ? jpath[q] : (q + 1 == a.k ? jlast : 0u));
     }
 }
 
@@ -243,7 +243,7 @@ __device__ __forceinline__ uint32_t group_value(uint32_t op, uint32_t x, uint32_
 // already equals the target (regmatch).  Tests 0-1 run for every lane; the
 // survivors go through an LDS queue to the early-exit loop over the rest.
 __device__ __forceinline__ void scan_parent_groups(const DevTask &task, const ScanArgs &a, const TestState *P,
-                                                   uint32_t live, uint32_t prefix_idx, const uint16_t *jpath,
+                                                   uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
                                                    int lane, WaveCounters &wc, uint16_t *queue,
                                                    const uint8_t *regmatch, const ParentSummary &ps)
 {
@@ -294,11 +294,15 @@ __device__ __forceinline__ void scan_parent_groups(const DevTask &task, const Sc
                 if (ok >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[gi * 8 + r], 1u << r, fl);
         }
     };
+    uint32_t g_next = lane < ng ? G[lane] : 0u;
+    uint32_t m_next = lane < ng ? GM[lane] : 0u;
     for (int base = 0; base < ng; base += 64) {
         const int gi = base + lane;
-        const bool valid = gi < ng;
-        const uint32_t g = valid ? G[gi] : 0u;
-        uint32_t ok = valid ? GM[gi] : 0u;
+        const uint32_t g = g_next;
+        uint32_t ok = m_next;
+        // the next chunk's group record, in flight while this one computes
+        g_next = gi + 64 < ng ? G[gi + 64] : 0u;
+        m_next = gi + 64 < ng ? GM[gi + 64] : 0u;
         bool fault = false;
         run_tests(g, 0, T0, ok, fault, false);
         const bool alive = ok != 0u;
@@ -334,7 +338,7 @@ __device__ __forceinline__ void scan_parent_groups(const DevTask &task, const Sc
 // Parent where some register already equals the target: every candidate
 // that leaves it alone is a hit, so go candidate by candidate.
 __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P,
-                                            uint32_t live, uint32_t prefix_idx, const uint16_t *jpath, int lane,
+                                            uint32_t live, uint32_t prefix_idx, const uint32_t *jpath, int lane,
                                             WaveCounters &wc, uint16_t *queue, const ParentSummary &ps)
 {
     const int T = task.T;
@@ -388,20 +392,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
             pending = pending && !mine;
         }
         const uint32_t match = valid ? (other | (rd_ok ? (1u << rd) : 0u)) & 0xFEu : 0u;
-        if (match) {
-            const uint32_t slot = atomicAdd(a.hit_count, 1u);
-            if (slot < a.hit_cap) {
-                HitRec h;
-                h.prefix = prefix_idx;
-                h.mask = (uint8_t)match;
-                h.flags = (uint8_t)((dead || fault) ? 1u : 0u);
-                for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
-                for (int q = 0; q + 1 < a.k; q++) h.j[q] = jpath[q];
-                h.j[a.k - 1] = (uint16_t)j;
-                h.pad = 0;
-                a.hits[slot] = h;
-            }
-        }
+        if (match) emit_hit(a, prefix_idx, jpath, j, match, (dead || fault) ? 1u : 0u);
     };
 
     uint32_t insn_next = lane < n ? L[lane] : 0u;
@@ -420,20 +411,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
         bool alive;
         if (is_cmp || !valid) {
             // CMP writes no register: its hits are the parent's matches
-            if (valid && other) {
-                const uint32_t slot = atomicAdd(a.hit_count, 1u);
-                if (slot < a.hit_cap) {
-                    HitRec h;
-                    h.prefix = prefix_idx;
-                    h.mask = (uint8_t)(other & 0xFEu);
-                    h.flags = (uint8_t)dead;
-                    for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++) h.j[q] = 0;
-                    for (int q = 0; q + 1 < a.k; q++) h.j[q] = jpath[q];
-                    h.j[a.k - 1] = (uint16_t)j;
-                    h.pad = 0;
-                    a.hits[slot] = h;
-                }
-            }
+            if (valid && other) emit_hit(a, prefix_idx, jpath, j, other & 0xFEu, dead);
             alive = false;
         } else if (two && other == 0u) {
             const uint32_t sa = op == OP_ADDI ? rd : rs;
@@ -526,7 +504,7 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
     }
 }
 
-constexpr int SCAN_CHUNK = 8;   // units taken per atomic by one wave
+constexpr int SCAN_CHUNK = 32;   // units taken per atomic by one wave
 
 __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
 {
@@ -535,13 +513,26 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
     __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
     __shared__ uint16_t s_queue[SCAN_WAVES][128];
     __shared__ uint8_t s_rm[SCAN_WAVES][MAXT];
+    __shared__ int s_cnt[128], s_split[128], s_vgn[128];
+    __shared__ uint32_t s_ctot[128];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int k = a.k;
+    // per-mask scalars live in LDS: no global round trip on a unit's setup
+    for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
+        s_cnt[i] = a.cand.count[i];
+        s_split[i] = a.cand.split[i];
+        s_vgn[i] = a.cand.vg_n[i];
+        s_ctot[i] = a.cand.cand_total[i];
+    }
+    __syncthreads();
+    a.cand.count = s_cnt;
+    a.cand.split = s_split;
+    a.cand.vg_n = s_vgn;
+    a.cand.cand_total = s_ctot;
     // level of the parent a unit designates: 0 (prefix), 1 (+ position s),
     // 2 (+ position s+1)
     const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
     WaveCounters wc;
-    uint16_t jpath[PCAGPU_MAX_KDEPTH];
     uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
     for (;;) {
         unsigned long long u0 = 0;
@@ -608,7 +599,7 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
             int L = Lu;
             for (;;) {
                 if (L == k - 1) {
-                    for (int q = 1; q <= L; q++) jpath[q - 1] = (uint16_t)uni(s_idx[w][q]);
+                    const uint32_t *jpath = &s_idx[w][1];
                     {
                         const TestState *PP = s_st[w][L];
                         const uint32_t lvL = uni(s_live[w][L]);
