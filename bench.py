@@ -141,6 +141,12 @@ def main():
                    "candidates": cand, "suffixes": st["suffixes"], "tests_evaluated": st["tests_evaluated"],
                    "parents": st["parents"], "raw_hits": st["raw_hits"], "kernel_candidates_per_s": cand / kernel_s if kernel_s else None},
     }
+    # HBM bytes per launch from the committed rocprofv3 PMC passes of this
+    # kernel and workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)
+    prof = os.path.join(ROOT, "profiles", "scan_kernel_is_power2_32_d8s5.json")
+    if os.path.exists(prof) and (D, s, B) == (8, 5, 16):
+        with open(prof) as f:
+            line["roofline"]["traffic"] = json.load(f)["traffic_bytes_per_launch"]
     if not args.no_cpu:
         cb = cpu_baseline(task_path, args.cpu_seconds)
         line["cpu_baseline"] = cb
