@@ -210,6 +210,58 @@ typedef struct {
 int  pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *opts, pcagpu_result_t *res,
                   pca_solution_t *out, long cap);
 
+/* ---- continuous-batching serving (the request loop above the path) ----
+ * No reference counterpart: the reference solves one task per process.
+ * Requests (task + depth budget) are queued; each pcagpu_server_step()
+ * admits queued requests into resident slots while max_slots and the HBM
+ * budget allow, stages the next split-depth prefix batch of every active
+ * task, scans all of them in ONE persistent kernel launch, and resolves the
+ * tasks whose depth is complete.  Results are identical to pcagpu_solve on
+ * the task alone (split D-1).  Single-threaded: one caller drives step(). */
+typedef struct pcagpu_server pcagpu_server_t;
+
+#define PCAGPU_REQ_QUEUED  0
+#define PCAGPU_REQ_ACTIVE  1
+#define PCAGPU_REQ_DONE    2   /* optimum proven, or none up to the budget  */
+#define PCAGPU_REQ_FAILED  3
+
+typedef struct {
+    int      device;
+    uint32_t max_slots;          /* tasks resident at once (default 64)        */
+    uint64_t hbm_budget;         /* device bytes the slots may hold (default 200 GiB) */
+    uint32_t prefixes_per_item;  /* prefixes a task adds to one launch (default 16384) */
+    uint32_t reserved[4];
+} pcagpu_server_opts_t;
+
+typedef struct {
+    uint64_t       id;
+    int            status;          /* PCAGPU_REQ_*                               */
+    int            optimal_depth;   /* -1: no program up to max_depth             */
+    long           num_solutions;
+    int            max_depth;
+    double         t_submit, t_admit, t_done;   /* seconds, server clock          */
+    pca_solution_t first;           /* first program in canonical order           */
+} pcagpu_request_t;
+
+typedef struct {
+    uint64_t submitted, admitted, completed, failed;
+    uint64_t launches, items;       /* scan launches, task batches in them        */
+    uint64_t kernel_ns, candidates, suffixes;
+    uint64_t slots, slots_opened, device_bytes;
+} pcagpu_server_stats_t;
+
+int    pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_server_t **out);
+void   pcagpu_server_close(pcagpu_server_t *s);
+int    pcagpu_server_submit(pcagpu_server_t *s, const pca_task_t *task, int max_depth, uint64_t *id);
+/* One scheduling iteration; returns requests still queued or active. */
+int    pcagpu_server_step(pcagpu_server_t *s);
+/* Completed requests since the last poll (each reported once). */
+int    pcagpu_server_poll(pcagpu_server_t *s, pcagpu_request_t *out, int cap);
+/* Canonical solutions of a completed request (up to 4096 kept). */
+long   pcagpu_server_solutions(pcagpu_server_t *s, uint64_t id, pca_solution_t *out, long cap);
+int    pcagpu_server_get_stats(pcagpu_server_t *s, pcagpu_server_stats_t *st);
+double pcagpu_server_clock(pcagpu_server_t *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -149,6 +149,26 @@ class SolveRes(C.Structure):
                 ("prefixes", C.c_uint64 * (MAX_KDEPTH + 1))]
 
 
+class ServerOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("max_slots", C.c_uint32), ("hbm_budget", C.c_uint64),
+                ("prefixes_per_item", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+
+
+class Request(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("status", C.c_int), ("optimal_depth", C.c_int),
+                ("num_solutions", C.c_long), ("max_depth", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("first", Solution)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "submitted", "admitted", "completed", "failed", "launches", "items", "kernel_ns", "candidates",
+        "suffixes", "slots", "slots_opened", "device_bytes")]
+
+    def as_dict(self):
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
+
+
 _LIB = None
 
 # (name, restype, argtypes) for every symbol include/pcagpu.h declares
@@ -174,6 +194,14 @@ SIGNATURES = [
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("pcagpu_solve", C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveRes),
                                C.POINTER(Solution), C.c_long]),
+    ("pcagpu_server_open", C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
+    ("pcagpu_server_close", None, [C.c_void_p]),
+    ("pcagpu_server_submit", C.c_int, [C.c_void_p, C.POINTER(Task), C.c_int, C.POINTER(C.c_uint64)]),
+    ("pcagpu_server_step", C.c_int, [C.c_void_p]),
+    ("pcagpu_server_poll", C.c_int, [C.c_void_p, C.POINTER(Request), C.c_int]),
+    ("pcagpu_server_solutions", C.c_long, [C.c_void_p, C.c_uint64, C.POINTER(Solution), C.c_long]),
+    ("pcagpu_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
+    ("pcagpu_server_clock", C.c_double, [C.c_void_p]),
 ]
 
 

@@ -127,9 +127,37 @@ struct ExpandArgs {
 };
 
 void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out, hipStream_t st);
+// One work item of a scan launch: a task (its resident record), a depth and
+// split, a batch of its prefixes with their group table, and its own hit
+// buffer.  Items of different tasks share one persistent launch.
+struct ScanItem {
+    const DevTask *task;
+    const pca_prefix_t *prefixes;
+    const ScanGroup *groups;
+    const uint64_t *group_off;
+    uint32_t n_prefix;
+    uint32_t n_groups;
+    uint64_t unit_begin;             // first global unit of this item
+    uint64_t n_units;
+    int D, s, k, pad;
+    HitRec *hits;
+    uint32_t *hit_count;
+    uint32_t hit_cap;
+    uint32_t pad2;
+};
+
+struct ScanLaunch {
+    const ScanItem *items;
+    uint32_t n_items;
+    uint64_t n_units;                // sum over the items
+    CandDev cand;
+    unsigned long long *counters;    // [0] candidates [1] suffixes [2] tests [3] parents
+    unsigned long long *work;
+};
+
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
 hipError_t scan_occupancy(int *blocks_per_cu);
-void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st);
+void launch_scan(const ScanLaunch &l, int grid, hipStream_t st);
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st);
 void launch_expand_insert(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);
 void launch_expand_survive(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);

@@ -506,7 +506,7 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
 
 constexpr int SCAN_CHUNK = 32;   // units taken per atomic by one wave
 
-__global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
+__global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch)
 {
     __shared__ TestState s_st[SCAN_WAVES][MAXLEV][MAXT];
     __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
@@ -516,42 +516,84 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
     __shared__ int s_cnt[128], s_split[128], s_vgn[128];
     __shared__ uint32_t s_ctot[128];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int k = a.k;
     // per-mask scalars live in LDS: no global round trip on a unit's setup
     for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
-        s_cnt[i] = a.cand.count[i];
-        s_split[i] = a.cand.split[i];
-        s_vgn[i] = a.cand.vg_n[i];
-        s_ctot[i] = a.cand.cand_total[i];
+        s_cnt[i] = lch.cand.count[i];
+        s_split[i] = lch.cand.split[i];
+        s_vgn[i] = lch.cand.vg_n[i];
+        s_ctot[i] = lch.cand.cand_total[i];
     }
     __syncthreads();
-    a.cand.count = s_cnt;
-    a.cand.split = s_split;
-    a.cand.vg_n = s_vgn;
-    a.cand.cand_total = s_ctot;
-    // level of the parent a unit designates: 0 (prefix), 1 (+ position s),
-    // 2 (+ position s+1)
-    const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
+    CandDev cand = lch.cand;
+    cand.count = s_cnt;
+    cand.split = s_split;
+    cand.vg_n = s_vgn;
+    cand.cand_total = s_ctot;
+
     WaveCounters wc;
+    // current item (wave-uniform) and its context
+    uint32_t cur_item = 0xFFFFFFFFu;
+    uint64_t it_begin = 0, it_end = 0;
+    const DevTask *taskp = nullptr;
+    ScanArgs a;
+    a.cand = cand;
+    a.counters = lch.counters;
+    a.work = lch.work;
     uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
     for (;;) {
         unsigned long long u0 = 0;
-        if (lane == 0) u0 = atomicAdd(a.work, (unsigned long long)SCAN_CHUNK);
+        if (lane == 0) u0 = atomicAdd(lch.work, (unsigned long long)SCAN_CHUNK);
         u0 = __shfl(u0, 0);
-        if (u0 >= a.n_units) break;
-        const unsigned long long u1 = u0 + SCAN_CHUNK < a.n_units ? u0 + SCAN_CHUNK : a.n_units;
+        if (u0 >= lch.n_units) break;
+        const unsigned long long u1 = u0 + SCAN_CHUNK < lch.n_units ? u0 + SCAN_CHUNK : lch.n_units;
         uint32_t g = 0;
-        if (k >= 2) {
-            uint32_t lo = 0, hi = a.n_groups;   // largest g with group_off[g] <= u0
-            while (hi - lo > 1) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (a.group_off[mid] <= u0) lo = mid; else hi = mid;
+        bool g_valid = false;
+        for (unsigned long long ug = u0; ug < u1; ug++) {
+            if (ug < it_begin || ug >= it_end) {
+                // the item holding this unit: largest ii with unit_begin <= ug
+                uint32_t lo = 0, hi = lch.n_items;
+                while (hi - lo > 1) {
+                    const uint32_t mid = (lo + hi) >> 1;
+                    if (lch.items[mid].unit_begin <= ug) lo = mid; else hi = mid;
+                }
+                cur_item = uni(lo);
+                const ScanItem &it = lch.items[cur_item];
+                taskp = it.task;
+                a.prefixes = it.prefixes;
+                a.n_prefix = it.n_prefix;
+                a.groups = it.groups;
+                a.group_off = it.group_off;
+                a.n_groups = it.n_groups;
+                a.n_units = it.n_units;
+                a.D = it.D;
+                a.s = it.s;
+                a.k = it.k;
+                a.hits = it.hits;
+                a.hit_cap = it.hit_cap;
+                a.hit_count = it.hit_count;
+                it_begin = it.unit_begin;
+                it_end = it.unit_begin + it.n_units;
+                cur_i = 0xFFFFFFFFu;
+                cur_js = 0xFFFFFFFFu;
+                g_valid = false;
             }
-            g = uni(lo);
-        }
-        for (unsigned long long u = u0; u < u1; u++) {
+            const DevTask &task = *taskp;
+            const int k = a.k;
+            // level of the parent a unit designates: 0 (prefix), 1 (+ position
+            // s), 2 (+ position s+1)
+            const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
+            const uint64_t u = ug - it_begin;
             uint32_t pi, js = 0, js1 = 0;
             if (k >= 2) {
+                if (!g_valid) {
+                    uint32_t lo = 0, hi = a.n_groups;   // largest g with group_off[g] <= u
+                    while (hi - lo > 1) {
+                        const uint32_t mid = (lo + hi) >> 1;
+                        if (a.group_off[mid] <= u) lo = mid; else hi = mid;
+                    }
+                    g = uni(lo);
+                    g_valid = true;
+                }
                 while (a.group_off[g + 1] <= u) g++;
                 const ScanGroup G = a.groups[g];
                 pi = uni(G.prefix);
@@ -639,17 +681,15 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(DevTask task, ScanArgs a)
                 wave_sync();
                 L++;
             }
-            // levels above Lu were overwritten only at L > Lu; level 1 stays
-            // valid for the cached js
         }
     }
     const unsigned long long c0 = wave_sum(wc.cand), c1 = wave_sum(wc.suf), c2 = wave_sum(wc.tests),
                              c3 = wave_sum(wc.parents);
     if (lane == 0) {
-        atomicAdd(&a.counters[0], c0);
-        atomicAdd(&a.counters[1], c1);
-        atomicAdd(&a.counters[2], c2);
-        atomicAdd(&a.counters[3], c3);
+        atomicAdd(&lch.counters[0], c0);
+        atomicAdd(&lch.counters[1], c1);
+        atomicAdd(&lch.counters[2], c2);
+        atomicAdd(&lch.counters[3], c3);
     }
 }
 
@@ -1146,9 +1186,9 @@ hipError_t scan_occupancy(int *blocks_per_cu)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel, SCAN_WG, 0);
 }
 
-void launch_scan(const DevTask &task, const ScanArgs &a, int grid, hipStream_t st)
+void launch_scan(const ScanLaunch &l, int grid, hipStream_t st)
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(grid), dim3(SCAN_WG), 0, st, task, a);
+    hipLaunchKernelGGL(scan_kernel, dim3(grid), dim3(SCAN_WG), 0, st, l);
 }
 
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st)

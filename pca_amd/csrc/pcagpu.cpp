@@ -18,111 +18,12 @@
 #include <string>
 #include <vector>
 
-#include "host_core.h"
 #include "host_dfs.h"
-#include "kernels.h"
+#include "pcagpu_impl.h"
 
 using namespace pca;
 
-namespace {
-
-#define HIP_TRY(expr)                                  \
-    do {                                               \
-        if ((expr) != hipSuccess) return PCAGPU_EHIP;  \
-    } while (0)
-
-struct HostHit {
-    uint64_t prefix;     // index into the depth's prefix list
-    uint8_t mask, flags;
-    uint16_t j[PCAGPU_MAX_KDEPTH];
-};
-
-template <typename T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    int ensure(size_t want)
-    {
-        if (want <= n) return 0;
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-        if (hipMalloc((void **)&p, want * sizeof(T)) != hipSuccess) { p = nullptr; return PCAGPU_ENOMEM; }
-        n = want;
-        return 0;
-    }
-    void release()
-    {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-};
-
-double now_s()
-{
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-}  // namespace
-
-struct pcagpu {
-    pca_task_t task;
-    int T = 0;
-    DevTask dt;
-    CandTable ct;
-    int device = 0;
-    int cu_count = 256;
-    std::string devname;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    pcagpu_opts_t opts;
-    uint32_t waves_per_cu = 20;
-
-    DevBuf<uint16_t> d_lists;
-    DevBuf<int> d_count, d_split;
-    DevBuf<uint16_t> d_vg_insn, d_vg_j;
-    DevBuf<uint8_t> d_vg_rdmask;
-    DevBuf<int> d_vg_n;
-    DevBuf<uint32_t> d_cand_total;
-    DevBuf<pca_prefix_t> d_batch;
-    DevBuf<ScanGroup> d_groups;
-    DevBuf<uint64_t> d_goff;
-    DevBuf<HitRec> d_hits;
-    DevBuf<uint32_t> d_hit_count;
-    DevBuf<unsigned long long> d_counters, d_work;
-    pca_prefix_t *h_batch = nullptr;
-    ScanGroup *h_groups = nullptr;
-    uint64_t *h_goff = nullptr;
-    uint32_t batch_cap = 0;
-    uint32_t hit_cap = 0;
-
-    // wavefront buffers
-    DevBuf<pca_prefix_t> d_front[2];
-    DevBuf<uint64_t> d_child_off, d_keep_cnt, d_scan_tmp;
-    DevBuf<unsigned long long> d_table;
-    DevBuf<uint32_t> d_keep_bits;
-
-    // depth state
-    bool in_depth = false;
-    int D = 0, s = 0, k = 0;
-    uint32_t batch_n = 0;
-    uint64_t batch_first = 0;   // depth-prefix index of h_batch[0]
-    std::vector<pca_prefix_t> prefixes;
-    std::vector<uint64_t> ranks;
-    std::vector<HostHit> raw;
-    pcagpu_stats_t stats;
-    std::vector<int> order;          // device test order (index into task tests)
-
-    // state-match pass buffers
-    DevBuf<unsigned long long> d_q0, d_qf;
-    DevBuf<MatchRec> d_match;
-    DevBuf<uint32_t> d_match_count;
-};
-
-namespace {
-
-constexpr uint32_t kGroupMax = 1u << 20;   // scan groups per launch
+namespace pca {
 
 // Test order on the device.  Tests whose inputs look random (about half the
 // bits set, far from 0 and 0xFFFF) come first: few candidates map such an
@@ -185,31 +86,39 @@ int grid_for(pcagpu *h, uint64_t units)
 
 int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
 {
-    ScanArgs a;
-    a.prefixes = h->d_batch.p;
-    a.n_prefix = n;
-    a.groups = h->d_groups.p;
-    a.group_off = h->d_goff.p;
-    a.n_groups = n_groups;
-    a.n_units = units;
-    a.D = h->D;
-    a.s = h->s;
-    a.k = h->k;
-    a.cand = cand_dev(h);
-    a.hits = h->d_hits.p;
-    a.hit_cap = h->hit_cap;
-    a.hit_count = h->d_hit_count.p;
-    a.counters = h->d_counters.p;
-    a.work = h->d_work.p;
+    ScanItem it;
+    std::memset(&it, 0, sizeof(it));
+    it.task = h->d_task.p;
+    it.prefixes = h->d_batch.p;
+    it.groups = h->d_groups.p;
+    it.group_off = h->d_goff.p;
+    it.n_prefix = n;
+    it.n_groups = n_groups;
+    it.unit_begin = 0;
+    it.n_units = units;
+    it.D = h->D;
+    it.s = h->s;
+    it.k = h->k;
+    it.hits = h->d_hits.p;
+    it.hit_count = h->d_hit_count.p;
+    it.hit_cap = h->hit_cap;
     if (n_groups) {
         HIP_TRY(hipMemcpyAsync(h->d_groups.p, h->h_groups, (size_t)n_groups * sizeof(ScanGroup),
                                hipMemcpyHostToDevice, h->stream));
         HIP_TRY(hipMemcpyAsync(h->d_goff.p, h->h_goff, ((size_t)n_groups + 1) * sizeof(uint64_t),
                                hipMemcpyHostToDevice, h->stream));
     }
+    HIP_TRY(hipMemcpyAsync(h->d_items.p, &it, sizeof(it), hipMemcpyHostToDevice, h->stream));
+    ScanLaunch l;
+    l.items = h->d_items.p;
+    l.n_items = 1;
+    l.n_units = units;
+    l.cand = cand_dev(h);
+    l.counters = h->d_counters.p;
+    l.work = h->d_work.p;
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    launch_scan(h->dt, a, grid_for(h, units), h->stream);
+    launch_scan(l, grid_for(h, units), h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -253,6 +162,66 @@ int scan_batch(pcagpu *h, uint32_t n)
             acc = 0;
         }
     }
+    return 0;
+}
+
+size_t handle_device_bytes(const pcagpu *h)
+{
+    return h->d_lists.bytes() + h->d_count.bytes() + h->d_split.bytes() + h->d_vg_insn.bytes() +
+           h->d_vg_j.bytes() + h->d_vg_rdmask.bytes() + h->d_vg_n.bytes() + h->d_cand_total.bytes() +
+           h->d_batch.bytes() + h->d_groups.bytes() + h->d_goff.bytes() + h->d_task.bytes() + h->d_items.bytes() +
+           h->d_hits.bytes() + h->d_hit_count.bytes() + h->d_counters.bytes() + h->d_work.bytes() +
+           h->d_front[0].bytes() + h->d_front[1].bytes() + h->d_child_off.bytes() + h->d_keep_cnt.bytes() +
+           h->d_scan_tmp.bytes() + h->d_table.bytes() + h->d_keep_bits.bytes() + h->d_q0.bytes() + h->d_qf.bytes() +
+           h->d_match.bytes() + h->d_match_count.bytes();
+}
+
+int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank)
+{
+    if (!h->in_depth) return PCAGPU_ESTATE;
+    if (h->batch_n + n > h->batch_cap) return PCAGPU_EOVERFLOW;
+    for (uint32_t i = 0; i < n; i++) {
+        if (p[i].num_insns != h->s) return PCAGPU_EINVAL;
+        if (!h->ranks.empty() && first_rank + i <= h->ranks.back()) return PCAGPU_EINVAL;
+        h->h_batch[h->batch_n++] = p[i];
+        h->prefixes.push_back(p[i]);
+        h->ranks.push_back(first_rank + i);
+        h->stats.prefixes++;
+    }
+    return 0;
+}
+
+int build_groups(pcagpu *h, uint32_t *n_groups, uint64_t *n_units)
+{
+    const uint32_t n = h->batch_n;
+    if (h->k <= 1) {
+        *n_groups = 0;
+        *n_units = n;
+        return 0;
+    }
+    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
+    uint32_t g = 0;
+    uint64_t acc = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t live = h->h_batch[i].live;
+        const int ns = h->ct.count(live, mf_s);
+        if (g + (uint32_t)ns + 1 > kGroupMax) return PCAGPU_EOVERFLOW;
+        if (h->k == 2) {
+            h->h_groups[g] = ScanGroup{i, 0, 0};
+            h->h_goff[g++] = acc;
+            acc += (uint64_t)ns;
+        } else {
+            for (int js = 0; js < ns; js++) {
+                const uint32_t c = h->ct.at(live, mf_s, js);
+                h->h_groups[g] = ScanGroup{i, (uint16_t)js, 0};
+                h->h_goff[g++] = acc;
+                acc += (uint64_t)h->ct.count(live_after(live, c), mf_s1);
+            }
+        }
+    }
+    h->h_goff[g] = acc;
+    *n_groups = g;
+    *n_units = acc;
     return 0;
 }
 
@@ -333,6 +302,9 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
         hipMemcpy(h->d_split.p, h->ct.a, sizeof(h->ct.a), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
         return fail(PCAGPU_EHIP);
+    if ((rc = h->d_task.ensure(1)) || (rc = h->d_items.ensure(64))) return fail(rc);
+    if (hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice) != hipSuccess)
+        return fail(PCAGPU_EHIP);
     if ((rc = h->d_vg_insn.ensure(h->ct.vg_insn.size())) || (rc = h->d_vg_j.ensure(h->ct.vg_j.size())) ||
         (rc = h->d_vg_rdmask.ensure(h->ct.vg_rdmask.size())) || (rc = h->d_vg_n.ensure(128)) ||
         (rc = h->d_cand_total.ensure(128)))
@@ -363,6 +335,8 @@ void pcagpu_close(pcagpu_t *h)
     h->d_batch.release();
     h->d_groups.release();
     h->d_goff.release();
+    h->d_task.release();
+    h->d_items.release();
     h->d_hits.release();
     h->d_hit_count.release();
     h->d_counters.release();

@@ -1,0 +1,460 @@
+// server.cpp -- continuous-batching serving of solve requests.
+//
+// A request is a task plus a depth budget; the answer is its proven optimal
+// length and canonical program set (or "none up to the budget").  The
+// server keeps up to max_slots tasks resident on the device (task record,
+// prefix queue, hit buffer, wavefront/OEP buffers per slot, all counted
+// against an HBM budget) and runs a scheduling loop:
+//
+//   admit queued requests into free slots while slots and budget allow
+//   every active task contributes its next batch of split-depth prefixes
+//   ONE persistent scan launch processes the batches of all tasks
+//   tasks whose depth is complete are resolved on the host (exact OEP,
+//   canonical order, CPU re-verification); solved ones free their slot,
+//   the others move to the next depth
+//
+// Requests arriving while others are on the GPU join at the next step, so
+// uneven tasks overlap in the same launches.  Each task's result is exactly
+// what pcagpu_solve gives for it alone: the same prefixes, kernel and
+// resolution, only scheduled together.
+#include <algorithm>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <new>
+#include <vector>
+
+#include "host_dfs.h"
+#include "pcagpu_impl.h"
+
+using namespace pca;
+
+namespace {
+
+struct Slot {
+    pcagpu *h = nullptr;
+    bool busy = false;
+    uint64_t req = 0;
+    int max_depth = 0;
+    int D = 0;
+    std::vector<pca_prefix_t> frontier;
+    size_t next = 0;
+    uint32_t staged = 0;     // prefixes of this step's launch
+};
+
+struct Request {
+    uint64_t id = 0;
+    pca_task_t task;
+    int max_depth = 0;
+    int status = PCAGPU_REQ_QUEUED;
+    int optimal_depth = -1;
+    long nsol = 0;
+    double t_submit = 0, t_admit = 0, t_done = 0;
+    uint64_t prefixes = 0;
+    std::vector<pca_solution_t> sols;
+};
+
+}  // namespace
+
+struct pcagpu_server {
+    pcagpu_server_opts_t opts;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<Slot> slots;
+    std::deque<uint64_t> queue;
+    std::map<uint64_t, Request> reqs;
+    std::vector<uint64_t> done_unpolled;
+    uint64_t next_id = 1;
+    double t0 = 0;
+    DevBuf<ScanItem> d_items;
+    DevBuf<unsigned long long> d_counters, d_work;
+    std::vector<ScanItem> items;
+    pcagpu_server_stats_t st;
+};
+
+namespace {
+
+double clock_s(const pcagpu_server *s) { return now_s() - s->t0; }
+
+size_t slots_bytes(const pcagpu_server *s)
+{
+    size_t b = 0;
+    for (const Slot &sl : s->slots)
+        if (sl.h) b += handle_device_bytes(sl.h);
+    return b;
+}
+
+// Re-points a resident handle at another task: only the task record changes
+// (the grammar tables are task independent).
+int retask(pcagpu *h, const pca_task_t *task)
+{
+    if (task->num_outputs != 1 || task->num_inputs < 0 || task->num_inputs > 7 || task->num_tests < 0 ||
+        task->num_tests > PCAGPU_MAX_TESTS || task->output_ports[0] > 7)
+        return PCAGPU_EINVAL;
+    for (int i = 0; i < task->num_inputs; i++)
+        if (task->input_ports[i] > 7) return PCAGPU_EINVAL;
+    h->task = *task;
+    h->T = task->num_tests;
+    fill_devtask(h);
+    HIP_TRY(hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice));
+    h->in_depth = false;
+    return 0;
+}
+
+// The split-depth prefixes of depth D (split D-1): a CPU DFS for splits of
+// at most one level (at most a few hundred prefixes), the device wavefront
+// beyond (the host thread also schedules every other task).
+int start_depth(Slot &sl)
+{
+    pcagpu *h = sl.h;
+    const int D = sl.D, s = D > 0 ? D - 1 : 0;
+    sl.frontier.clear();
+    sl.next = 0;
+    if (s <= 1) {
+        long cap = 1 << 12, n;
+        for (;;) {
+            sl.frontier.resize((size_t)cap);
+            n = pcagpu_host_frontier(&h->task, D, s, sl.frontier.data(), cap, nullptr, nullptr);
+            if (n < 0) return (int)n;
+            if (n < cap) break;
+            cap *= 4;
+        }
+        sl.frontier.resize((size_t)n);
+    } else {
+        long n = pcagpu_frontier(h, D, s, nullptr, 0);
+        if (n < 0) return (int)n;
+        sl.frontier.resize((size_t)n);
+        if (n > 0) {
+            long m = pcagpu_frontier(h, D, s, sl.frontier.data(), n);
+            if (m < 0) return (int)m;
+        }
+    }
+    return pcagpu_begin_depth(h, D, s);
+}
+
+void finish(pcagpu_server *s, Slot &sl, int optimal, long nsol, const std::vector<pca_solution_t> &sols)
+{
+    Request &r = s->reqs[sl.req];
+    r.status = PCAGPU_REQ_DONE;
+    r.optimal_depth = optimal;
+    r.nsol = nsol;
+    r.sols = sols;
+    r.t_done = clock_s(s);
+    s->done_unpolled.push_back(r.id);
+    s->st.completed++;
+    sl.busy = false;
+    sl.frontier.clear();
+}
+
+void fail_request(pcagpu_server *s, Slot &sl)
+{
+    Request &r = s->reqs[sl.req];
+    r.status = PCAGPU_REQ_FAILED;
+    r.t_done = clock_s(s);
+    s->done_unpolled.push_back(r.id);
+    s->st.failed++;
+    sl.busy = false;
+    sl.frontier.clear();
+}
+
+// Depth finished: resolve; either complete the request or go one deeper.
+int advance(pcagpu_server *s, Slot &sl)
+{
+    std::vector<pca_solution_t> buf(4096);
+    for (;;) {
+        const long n = pcagpu_end_depth(sl.h, buf.data(), (long)buf.size());
+        if (n < 0) return (int)n;
+        if (n > 0) {
+            buf.resize((size_t)std::min<long>(n, (long)buf.size()));
+            finish(s, sl, sl.D, n, buf);
+            return 0;
+        }
+        if (sl.D >= sl.max_depth) {
+            finish(s, sl, -1, 0, {});
+            return 0;
+        }
+        sl.D++;
+        int rc = start_depth(sl);
+        if (rc < 0) return rc;
+        if (!sl.frontier.empty()) return 0;
+        // an empty frontier closes the depth at once
+    }
+}
+
+int admit(pcagpu_server *s)
+{
+    while (!s->queue.empty()) {
+        Slot *free_slot = nullptr;
+        for (Slot &sl : s->slots)
+            if (!sl.busy && sl.h) { free_slot = &sl; break; }
+        if (!free_slot) {
+            if (s->slots.size() >= s->opts.max_slots) return 0;
+            // a new resident slot must fit the HBM budget (the first one is
+            // always allowed, so a small budget serialises instead of stalling)
+            const size_t per = s->slots.empty() ? (size_t)64 << 20 : slots_bytes(s) / s->slots.size();
+            if (!s->slots.empty() && slots_bytes(s) + per > s->opts.hbm_budget) return 0;
+            Request &r = s->reqs[s->queue.front()];
+            pcagpu_opts_t o;
+            std::memset(&o, 0, sizeof(o));
+            o.device = s->opts.device;
+            o.batch_capacity = s->opts.prefixes_per_item;
+            o.hit_capacity = 1u << 16;
+            pcagpu *h = nullptr;
+            int rc = pcagpu_open(&r.task, &o, &h);
+            if (rc == PCAGPU_EINVAL) {
+                // the task itself is unsupported: fail this request only
+                const uint64_t bad = s->queue.front();
+                s->queue.pop_front();
+                Request &b = s->reqs[bad];
+                b.status = PCAGPU_REQ_FAILED;
+                b.t_done = clock_s(s);
+                s->done_unpolled.push_back(bad);
+                s->st.failed++;
+                continue;
+            }
+            if (rc < 0) return rc;
+            s->slots.push_back(Slot());
+            s->slots.back().h = h;
+            free_slot = &s->slots.back();
+            s->st.slots_opened++;
+        }
+        const uint64_t id = s->queue.front();
+        s->queue.pop_front();
+        Request &r = s->reqs[id];
+        Slot &sl = *free_slot;
+        if (retask(sl.h, &r.task) < 0) {
+            sl.req = id;
+            fail_request(s, sl);
+            continue;
+        }
+        sl.busy = true;
+        sl.req = id;
+        sl.max_depth = r.max_depth;
+        sl.D = 0;
+        r.status = PCAGPU_REQ_ACTIVE;
+        r.t_admit = clock_s(s);
+        s->st.admitted++;
+        int rc = start_depth(sl);
+        if (rc < 0) { fail_request(s, sl); continue; }
+        if (sl.frontier.empty() && (rc = advance(s, sl)) < 0) fail_request(s, sl);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_server_t **out)
+{
+    if (!out) return PCAGPU_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return PCAGPU_ENODEV;
+    pcagpu_server *s = new (std::nothrow) pcagpu_server();
+    if (!s) return PCAGPU_ENOMEM;
+    std::memset(&s->opts, 0, sizeof(s->opts));
+    if (opts) s->opts = *opts;
+    if (!s->opts.max_slots) s->opts.max_slots = 64;
+    if (!s->opts.hbm_budget) s->opts.hbm_budget = (uint64_t)200 << 30;
+    if (!s->opts.prefixes_per_item) s->opts.prefixes_per_item = 1u << 14;
+    std::memset(&s->st, 0, sizeof(s->st));
+    s->t0 = now_s();
+    if (hipSetDevice(s->opts.device) != hipSuccess ||
+        hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&s->ev0) != hipSuccess || hipEventCreate(&s->ev1) != hipSuccess) {
+        pcagpu_server_close(s);
+        return PCAGPU_EHIP;
+    }
+    if (s->d_items.ensure(s->opts.max_slots) || s->d_counters.ensure(8) || s->d_work.ensure(1)) {
+        pcagpu_server_close(s);
+        return PCAGPU_ENOMEM;
+    }
+    *out = s;
+    return 0;
+}
+
+extern "C" void pcagpu_server_close(pcagpu_server_t *s)
+{
+    if (!s) return;
+    for (Slot &sl : s->slots) pcagpu_close(sl.h);
+    s->d_items.release();
+    s->d_counters.release();
+    s->d_work.release();
+    if (s->ev0) (void)hipEventDestroy(s->ev0);
+    if (s->ev1) (void)hipEventDestroy(s->ev1);
+    if (s->stream) (void)hipStreamDestroy(s->stream);
+    delete s;
+}
+
+extern "C" double pcagpu_server_clock(pcagpu_server_t *s) { return s ? clock_s(s) : 0.0; }
+
+extern "C" int pcagpu_server_submit(pcagpu_server_t *s, const pca_task_t *task, int max_depth, uint64_t *id)
+{
+    if (!s || !task || max_depth < 0 || max_depth > PCAGPU_MAX_KDEPTH) return PCAGPU_EINVAL;
+    if (task->num_inputs + max_depth + 2 > PCAGPU_MAX_PROG) return PCAGPU_EINVAL;
+    Request r;
+    r.id = s->next_id++;
+    r.task = *task;
+    r.max_depth = max_depth;
+    r.t_submit = clock_s(s);
+    s->reqs[r.id] = r;
+    s->queue.push_back(r.id);
+    s->st.submitted++;
+    if (id) *id = r.id;
+    return 0;
+}
+
+extern "C" int pcagpu_server_step(pcagpu_server_t *s)
+{
+    if (!s) return PCAGPU_EINVAL;
+    (void)hipSetDevice(s->opts.device);
+    int rc = admit(s);
+    if (rc < 0) return rc;
+    // ---- every active task stages its next batch
+    s->items.clear();
+    uint64_t units = 0;
+    std::vector<Slot *> in_launch;
+    for (Slot &sl : s->slots) {
+        if (!sl.busy || sl.next >= sl.frontier.size()) continue;
+        pcagpu *h = sl.h;
+        const uint32_t n = (uint32_t)std::min<size_t>(sl.frontier.size() - sl.next, h->batch_cap);
+        if ((rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return rc;
+        uint32_t ng = 0;
+        uint64_t nu = 0;
+        if ((rc = build_groups(h, &ng, &nu)) < 0) return rc;
+        HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
+                               s->stream));
+        if (ng) {
+            HIP_TRY(hipMemcpyAsync(h->d_groups.p, h->h_groups, (size_t)ng * sizeof(ScanGroup),
+                                   hipMemcpyHostToDevice, s->stream));
+            HIP_TRY(hipMemcpyAsync(h->d_goff.p, h->h_goff, ((size_t)ng + 1) * 8, hipMemcpyHostToDevice, s->stream));
+        }
+        HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), s->stream));
+        ScanItem it;
+        std::memset(&it, 0, sizeof(it));
+        it.task = h->d_task.p;
+        it.prefixes = h->d_batch.p;
+        it.groups = h->d_groups.p;
+        it.group_off = h->d_goff.p;
+        it.n_prefix = n;
+        it.n_groups = ng;
+        it.unit_begin = units;
+        it.n_units = nu;
+        it.D = h->D;
+        it.s = h->s;
+        it.k = h->k;
+        it.hits = h->d_hits.p;
+        it.hit_count = h->d_hit_count.p;
+        it.hit_cap = h->hit_cap;
+        s->items.push_back(it);
+        units += nu;
+        sl.staged = n;
+        sl.next += n;
+        in_launch.push_back(&sl);
+    }
+    // ---- one launch for every task's batch
+    if (!s->items.empty()) {
+        if ((rc = s->d_items.ensure(s->items.size()))) return rc;
+        HIP_TRY(hipMemcpyAsync(s->d_items.p, s->items.data(), s->items.size() * sizeof(ScanItem),
+                               hipMemcpyHostToDevice, s->stream));
+        HIP_TRY(hipMemsetAsync(s->d_counters.p, 0, 4 * sizeof(unsigned long long), s->stream));
+        HIP_TRY(hipMemsetAsync(s->d_work.p, 0, sizeof(unsigned long long), s->stream));
+        ScanLaunch l;
+        l.items = s->d_items.p;
+        l.n_items = (uint32_t)s->items.size();
+        l.n_units = units;
+        l.cand = cand_dev(in_launch[0]->h);
+        l.counters = s->d_counters.p;
+        l.work = s->d_work.p;
+        HIP_TRY(hipEventRecord(s->ev0, s->stream));
+        launch_scan(l, grid_for(in_launch[0]->h, units), s->stream);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(s->ev1, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        unsigned long long c[4];
+        HIP_TRY(hipMemcpy(c, s->d_counters.p, sizeof(c), hipMemcpyDeviceToHost));
+        s->st.launches++;
+        s->st.items += s->items.size();
+        s->st.kernel_ns += (uint64_t)((double)ms * 1e6);
+        s->st.candidates += c[0];
+        s->st.suffixes += c[1];
+        // ---- collect each task's hits
+        for (Slot *sp : in_launch) {
+            pcagpu *h = sp->h;
+            uint32_t hits = 0;
+            HIP_TRY(hipMemcpy(&hits, h->d_hit_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            if (hits > h->hit_cap) {
+                // overflow: rescan this batch alone with a larger buffer
+                rc = pcagpu_flush(h);
+                if (rc < 0) return rc;
+                continue;
+            }
+            if (hits) {
+                std::vector<HitRec> buf(hits);
+                HIP_TRY(hipMemcpy(buf.data(), h->d_hits.p, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost));
+                for (const HitRec &r : buf) {
+                    HostHit x;
+                    x.prefix = h->batch_first + r.prefix;
+                    x.mask = r.mask;
+                    x.flags = r.flags;
+                    std::memcpy(x.j, r.j, sizeof(x.j));
+                    h->raw.push_back(x);
+                }
+            }
+            h->batch_first += h->batch_n;
+            h->batch_n = 0;
+        }
+    }
+    // ---- tasks whose depth is complete
+    for (Slot &sl : s->slots) {
+        if (!sl.busy || sl.next < sl.frontier.size()) continue;
+        if ((rc = advance(s, sl)) < 0) fail_request(s, sl);
+    }
+    int pending = (int)s->queue.size();
+    for (const Slot &sl : s->slots) pending += sl.busy;
+    return pending;
+}
+
+extern "C" int pcagpu_server_poll(pcagpu_server_t *s, pcagpu_request_t *out, int cap)
+{
+    if (!s || (!out && cap > 0)) return PCAGPU_EINVAL;
+    int n = 0;
+    while (n < cap && !s->done_unpolled.empty()) {
+        const Request &r = s->reqs[s->done_unpolled.front()];
+        s->done_unpolled.erase(s->done_unpolled.begin());
+        pcagpu_request_t &o = out[n++];
+        std::memset(&o, 0, sizeof(o));
+        o.id = r.id;
+        o.status = r.status;
+        o.optimal_depth = r.optimal_depth;
+        o.num_solutions = r.nsol;
+        o.max_depth = r.max_depth;
+        o.t_submit = r.t_submit;
+        o.t_admit = r.t_admit;
+        o.t_done = r.t_done;
+        if (!r.sols.empty()) o.first = r.sols[0];
+    }
+    return n;
+}
+
+extern "C" long pcagpu_server_solutions(pcagpu_server_t *s, uint64_t id, pca_solution_t *out, long cap)
+{
+    if (!s) return PCAGPU_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return PCAGPU_EINVAL;
+    const Request &r = it->second;
+    if (r.status != PCAGPU_REQ_DONE) return PCAGPU_ESTATE;
+    for (long i = 0; i < cap && i < (long)r.sols.size(); i++) out[i] = r.sols[(size_t)i];
+    return r.nsol;
+}
+
+extern "C" int pcagpu_server_get_stats(pcagpu_server_t *s, pcagpu_server_stats_t *st)
+{
+    if (!s || !st) return PCAGPU_EINVAL;
+    *st = s->st;
+    size_t b = slots_bytes(s);
+    st->device_bytes = b;
+    st->slots = s->slots.size();
+    return 0;
+}

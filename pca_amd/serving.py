@@ -1,0 +1,147 @@
+"""Continuous-batching serving over the C ABI (pcagpu_server_*).
+
+``Server`` wraps the library's request loop: requests (task + depth budget)
+are admitted into resident device slots as they free up, every active task
+contributes its next prefix batch to one shared persistent scan launch, and
+each result is the task's exact canonical answer.
+
+``poisson_run`` replays a Poisson arrival schedule against the server, and
+``sequential_run`` replays the same schedule through the plain
+one-task-at-a-time GPU path (``Solver.solve`` per request, in arrival order),
+so the two latency distributions are comparable.
+"""
+import ctypes as C
+import random
+import time
+
+from ._lib import (PcaGpuError, Task, Solution, lib, ServerOpts, Request, ServerStats)
+from .solver import Solver, _sol_tuple
+
+QUEUED, ACTIVE, DONE, FAILED = 0, 1, 2, 3
+
+
+class Server:
+    def __init__(self, device=0, max_slots=64, hbm_budget=200 << 30, prefixes_per_item=1 << 14):
+        o = ServerOpts(device=device, max_slots=max_slots, hbm_budget=hbm_budget,
+                       prefixes_per_item=prefixes_per_item)
+        h = C.c_void_p()
+        rc = lib().pcagpu_server_open(C.byref(o), C.byref(h))
+        if rc < 0:
+            raise PcaGpuError(rc, "pcagpu_server_open")
+        self._h = h
+
+    def close(self):
+        if self._h:
+            lib().pcagpu_server_close(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def clock(self):
+        return lib().pcagpu_server_clock(self._h)
+
+    def submit(self, task, max_depth):
+        rid = C.c_uint64()
+        rc = lib().pcagpu_server_submit(self._h, C.byref(task), max_depth, C.byref(rid))
+        if rc < 0:
+            raise PcaGpuError(rc, "server_submit")
+        return rid.value
+
+    def step(self):
+        rc = lib().pcagpu_server_step(self._h)
+        if rc < 0:
+            raise PcaGpuError(rc, "server_step")
+        return rc
+
+    def poll(self, cap=256):
+        buf = (Request * cap)()
+        n = lib().pcagpu_server_poll(self._h, buf, cap)
+        if n < 0:
+            raise PcaGpuError(n, "server_poll")
+        return [{"id": r.id, "status": r.status, "optimal_depth": r.optimal_depth,
+                 "num_solutions": r.num_solutions, "max_depth": r.max_depth, "t_submit": r.t_submit,
+                 "t_admit": r.t_admit, "t_done": r.t_done} for r in buf[:n]]
+
+    def solutions(self, rid, cap=4096):
+        buf = (Solution * cap)()
+        n = lib().pcagpu_server_solutions(self._h, rid, buf, cap)
+        if n < 0:
+            raise PcaGpuError(n, "server_solutions")
+        return [_sol_tuple(buf[i]) for i in range(min(n, cap))], n
+
+    def stats(self):
+        st = ServerStats()
+        lib().pcagpu_server_get_stats(self._h, C.byref(st))
+        return st.as_dict()
+
+    def drain(self, timeout=600.0):
+        """Steps until every submitted request is done; returns polled records."""
+        out = []
+        t0 = time.perf_counter()
+        while True:
+            pending = self.step()
+            out += self.poll()
+            if pending == 0:
+                out += self.poll()
+                return out
+            if time.perf_counter() - t0 > timeout:
+                raise TimeoutError("server drain")
+
+
+def poisson_schedule(n, rate, seed=0):
+    rng = random.Random(seed)
+    t, out = 0.0, []
+    for _ in range(n):
+        t += rng.expovariate(rate)
+        out.append(t)
+    return out
+
+
+def poisson_run(tasks, budgets, arrivals, **server_kw):
+    """Replays arrivals (seconds) against a Server; returns per-request
+    (arrival, done, optimal_depth, num_solutions) and the server stats."""
+    with Server(**server_kw) as sv:
+        ids = {}
+        recs = {}
+        i = 0
+        t0 = time.perf_counter()
+        while len(recs) < len(tasks):
+            now = time.perf_counter() - t0
+            while i < len(tasks) and arrivals[i] <= now:
+                ids[sv.submit(tasks[i], budgets[i])] = i
+                i += 1
+            pending = sv.step()
+            now = time.perf_counter() - t0
+            for r in sv.poll():
+                recs[ids[r["id"]]] = (arrivals[ids[r["id"]]], now, r["optimal_depth"], r["num_solutions"])
+            if pending == 0 and i < len(tasks):
+                time.sleep(max(0.0, min(arrivals[i] - (time.perf_counter() - t0), 0.001)))
+        return [recs[k] for k in range(len(tasks))], sv.stats()
+
+
+def sequential_run(tasks, budgets, arrivals):
+    """The same arrivals through the plain one-task-at-a-time GPU path."""
+    recs = []
+    t0 = time.perf_counter()
+    for task, budget, arr in zip(tasks, budgets, arrivals):
+        now = time.perf_counter() - t0
+        if now < arr:
+            time.sleep(arr - now)
+        with Solver(task) as s:
+            r = s.solve(budget, cap=1)
+        recs.append((arr, time.perf_counter() - t0, r.optimal_depth, r.num_solutions))
+    return recs
+
+
+def latency_stats(recs):
+    lat = sorted(done - arr for arr, done, _, _ in recs)
+    if not lat:
+        return {}
+
+    def pct(p):
+        return lat[min(len(lat) - 1, int(round(p / 100.0 * (len(lat) - 1))))]
+    return {"p50_s": pct(50), "p99_s": pct(99), "mean_s": sum(lat) / len(lat), "n": len(lat)}

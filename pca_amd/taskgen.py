@@ -174,3 +174,35 @@ def random_task(seed, index):
     if not ins:
         ins, outs = [tuple(0 for _ in range(n_in))], [0]
     return Task.from_io(f"syn_{seed}_{index}", ins, outs, max_cycles=100)
+
+
+def shallow_task(seed, index, max_len=3):
+    """A serving request: the outputs of a seeded random program of 1..max_len
+    search-grammar instructions (ALU, LDI, ADDI over live registers) on 1 or 2
+    inputs and 8-32 seeded uniform tests, so the optimum is at most max_len
+    kernel instructions deep."""
+    from ._lib import gen_candidates
+    rng = Xoshiro256((seed * 7919 + index * 104729 + 17) & M64)
+    n_in = rng.randint(1, 2)
+    live = sum(1 << (i + 1) for i in range(n_in))
+    length = rng.randint(1, max_len)
+    prog = [(23 << 11) | ((i + 1) << 8) | i for i in range(n_in)]
+    for pos in range(length):
+        cands = [c for c in gen_candidates(live, pos, length)
+                 if ((c >> 11) & 31) not in (13, 19)]          # no CMP / BR
+        c = cands[rng.next() % len(cands)]
+        prog.append(c)
+        live |= 1 << ((c >> 8) & 7)
+    regs = [r for r in range(1, 8) if live >> r & 1]
+    out_reg = regs[rng.next() % len(regs)]
+    prog += [(24 << 11) | (out_reg << 8), 1 << 11]
+    n_tests = rng.randint(8, 32)
+    ins, outs = [], []
+    while len(ins) < n_tests:
+        x = tuple(rng.u16() for _ in range(n_in))
+        y = _interp(prog, x)
+        if y is None:          # DIV/MOD by zero on this input: draw another
+            continue
+        ins.append(x)
+        outs.append(y)
+    return Task.from_io(f"srv_{seed}_{index}", ins, outs, max_cycles=100)

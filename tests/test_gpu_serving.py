@@ -1,0 +1,68 @@
+"""Serving: continuous batching over one persistent multi-task scan."""
+import pytest
+
+from conftest import golden_task_path
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = [("negate", 3), ("double", 3), ("add", 3), ("abs", 3), ("relu", 3), ("sign", 2), ("clamp", 2),
+          ("is_power2", 2), ("ident0", 3), ("negate_16", 3), ("max", 2)]
+
+
+def _tasks(pca):
+    from pca_amd import taskgen
+    tasks = [(pca.Task.load(golden_task_path(n)), d) for n, d in GOLDEN]
+    tasks += [(taskgen.shallow_task(3, i), 4 + i % 5) for i in range(16)]
+    return tasks
+
+
+def _solo(pca, task, budget):
+    with pca.Solver(task) as s:
+        r = s.solve(budget, cap=4096)
+    return r.optimal_depth, r.num_solutions, [x[0] for x in r.solutions]
+
+
+def test_server_mix_equals_solo(pca):
+    from pca_amd.serving import Server, DONE
+    tasks = _tasks(pca)
+    print(f"[serving] {len(tasks)} tasks", flush=True)
+    want = [_solo(pca, t, d) for t, d in tasks]
+    with Server(max_slots=8) as sv:
+        ids = [sv.submit(t, d) for t, d in tasks[:10]]
+        for _ in range(3):
+            sv.step()                      # the rest arrive while these run
+        ids += [sv.submit(t, d) for t, d in tasks[10:]]
+        recs = {r["id"]: r for r in sv.drain()}
+        st = sv.stats()
+        for i, rid in enumerate(ids):
+            r = recs[rid]
+            assert r["status"] == DONE
+            sols, n = sv.solutions(rid)
+            got = (r["optimal_depth"], n, [x[0] for x in sols])
+            assert got == want[i], (i, got[:2], want[i][:2])
+    assert st["items"] > st["launches"], "tasks did not share launches"
+    assert st["slots"] <= 8
+
+
+@pytest.mark.parametrize("max_slots,budget", [(2, 200 << 30), (8, 1)])
+def test_server_admission_limits(pca, max_slots, budget):
+    from pca_amd.serving import Server, DONE
+    tasks = _tasks(pca)[:12]
+    with Server(max_slots=max_slots, hbm_budget=budget) as sv:
+        ids = [sv.submit(t, d) for t, d in tasks]
+        recs = {r["id"]: r for r in sv.drain()}
+        st = sv.stats()
+    assert all(recs[i]["status"] == DONE for i in ids)
+    assert st["slots"] <= (max_slots if budget > 1 else 1)
+
+
+def test_server_rejects_bad_task_without_stalling(pca):
+    from pca_amd.serving import Server, DONE, FAILED
+    bad = pca.Task.load(golden_task_path("negate"))
+    bad.num_outputs = 2                     # multi-output: unsupported
+    good = pca.Task.load(golden_task_path("double"))
+    with Server(max_slots=2) as sv:
+        b = sv.submit(bad, 2)
+        g = sv.submit(good, 2)
+        recs = {r["id"]: r for r in sv.drain()}
+    assert recs[b]["status"] == FAILED and recs[g]["status"] == DONE
