@@ -52,6 +52,31 @@ def cpu_baseline(task_path, seconds):
             "tried_per_s": tried / secs}
 
 
+def serving_bench(rates=(10, 50, 200), n_req=200, seed=7):
+    """Poisson arrivals of generated requests (depth budgets 4-8) at each
+    rate, through the continuous-batching server and through the plain
+    one-task-at-a-time GPU path on the same arrival schedule."""
+    from pca_amd import taskgen, serving
+    out = {"requests_per_rate": n_req, "task": "shallow_task (1-3 insn source, 1-2 inputs, 8-32 tests)",
+           "budgets": "uniform 4-8"}
+    for rate in rates:
+        tasks = [taskgen.shallow_task(seed + rate, i) for i in range(n_req)]
+        budgets = [4 + (i * 7 + rate) % 5 for i in range(n_req)]
+        arr = serving.poisson_schedule(n_req, rate, seed=rate)
+        recs, st = serving.poisson_run(tasks, budgets, arr, max_slots=64)
+        print(f"serving rate {rate}: server done", file=sys.stderr, flush=True)
+        seq = serving.sequential_run(tasks, budgets, arr)
+        same = all(a[2:] == b[2:] for a, b in zip(recs, seq))
+        span = max(r[1] for r in recs) - min(r[0] for r in recs)
+        out[f"{rate}_req_s"] = {
+            "server": serving.latency_stats(recs), "one_at_a_time": serving.latency_stats(seq),
+            "server_candidates_per_s": st["candidates"] / max(span, 1e-9),
+            "launches": st["launches"], "items_per_launch": st["items"] / max(st["launches"], 1),
+            "same_results": same}
+        print(f"serving rate {rate}: one-at-a-time done", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -62,6 +87,8 @@ def main():
     ap.add_argument("--split", type=int, default=5)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -147,6 +174,8 @@ def main():
     if os.path.exists(prof) and (D, s, B) == (8, 5, 16):
         with open(prof) as f:
             line["roofline"]["traffic"] = json.load(f)["traffic_bytes_per_launch"]
+    if not args.no_serving:
+        line["serving"] = serving_bench(n_req=args.serving_requests)
     if not args.no_cpu:
         cb = cpu_baseline(task_path, args.cpu_seconds)
         line["cpu_baseline"] = cb

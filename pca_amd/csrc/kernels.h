@@ -156,8 +156,11 @@ struct ScanLaunch {
 };
 
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
-hipError_t scan_occupancy(int *blocks_per_cu);
-void launch_scan(const ScanLaunch &l, int grid, hipStream_t st);
+hipError_t scan_occupancy(bool multi, int *blocks_per_cu);
+// host_items: the same items as l.items, in host memory; task0: the task
+// record of item 0.  A one-item launch passes both in the kernel arguments.
+void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask &task0, int grid,
+                 hipStream_t st);
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st);
 void launch_expand_insert(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);
 void launch_expand_survive(const DevTask &task, const ExpandArgs &a, int grid, hipStream_t st);

@@ -506,7 +506,12 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
 
 constexpr int SCAN_CHUNK = 32;   // units taken per atomic by one wave
 
-__global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch)
+// MULTI = false: one item and its task record, passed by value in the kernel
+// arguments (the single-task path; no per-unit item lookup, the context is
+// read from the kernel-argument segment).  MULTI = true: lch.items[0..n_items) in device
+// memory, the item of each unit found by binary search over unit_begin.
+template <bool MULTI>
+__global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem one, DevTask task1)
 {
     __shared__ TestState s_st[SCAN_WAVES][MAXLEV][MAXT];
     __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
@@ -540,6 +545,24 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch)
     a.counters = lch.counters;
     a.work = lch.work;
     uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
+    if constexpr (!MULTI) {
+        cur_item = 0;
+        taskp = &task1;
+        a.prefixes = one.prefixes;
+        a.n_prefix = one.n_prefix;
+        a.groups = one.groups;
+        a.group_off = one.group_off;
+        a.n_groups = one.n_groups;
+        a.n_units = one.n_units;
+        a.D = one.D;
+        a.s = one.s;
+        a.k = one.k;
+        a.hits = one.hits;
+        a.hit_cap = one.hit_cap;
+        a.hit_count = one.hit_count;
+        it_begin = 0;
+        it_end = one.n_units;
+    }
     for (;;) {
         unsigned long long u0 = 0;
         if (lane == 0) u0 = atomicAdd(lch.work, (unsigned long long)SCAN_CHUNK);
@@ -549,7 +572,7 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch)
         uint32_t g = 0;
         bool g_valid = false;
         for (unsigned long long ug = u0; ug < u1; ug++) {
-            if (ug < it_begin || ug >= it_end) {
+            if (MULTI && (ug < it_begin || ug >= it_end)) {
                 // the item holding this unit: largest ii with unit_begin <= ug
                 uint32_t lo = 0, hi = lch.n_items;
                 while (hi - lo > 1) {
@@ -1181,14 +1204,19 @@ void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st)
     hipLaunchKernelGGL(table_init_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table, cap);
 }
 
-hipError_t scan_occupancy(int *blocks_per_cu)
+hipError_t scan_occupancy(bool multi, int *blocks_per_cu)
 {
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel, SCAN_WG, 0);
+    if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel<true>, SCAN_WG, 0);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel<false>, SCAN_WG, 0);
 }
 
-void launch_scan(const ScanLaunch &l, int grid, hipStream_t st)
+void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask &task0, int grid,
+                 hipStream_t st)
 {
-    hipLaunchKernelGGL(scan_kernel, dim3(grid), dim3(SCAN_WG), 0, st, l);
+    if (l.n_items == 1)
+        hipLaunchKernelGGL(scan_kernel<false>, dim3(grid), dim3(SCAN_WG), 0, st, l, host_items[0], task0);
+    else
+        hipLaunchKernelGGL(scan_kernel<true>, dim3(grid), dim3(SCAN_WG), 0, st, l, host_items[0], task0);
 }
 
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st)

@@ -77,9 +77,9 @@ CandDev cand_dev(const pcagpu *h)
                    h->d_vg_j.p, h->d_vg_n.p, h->d_cand_total.p};
 }
 
-int grid_for(pcagpu *h, uint64_t units)
+int grid_for(pcagpu *h, uint64_t units, bool multi)
 {
-    uint64_t waves = (uint64_t)h->cu_count * h->waves_per_cu;
+    uint64_t waves = (uint64_t)h->cu_count * (multi ? h->waves_per_cu_multi : h->waves_per_cu);
     uint64_t wgs = (std::min<uint64_t>(units, waves) + SCAN_WAVES - 1) / SCAN_WAVES;
     return (int)std::max<uint64_t>(wgs, 1);
 }
@@ -118,7 +118,7 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
     l.work = h->d_work.p;
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    launch_scan(l, grid_for(h, units), h->stream);
+    launch_scan(l, &it, h->dt, grid_for(h, units), h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -268,7 +268,7 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     if (h->device < 0 || h->device >= ndev) { delete h; return PCAGPU_EINVAL; }
     h->batch_cap = h->opts.batch_capacity ? h->opts.batch_capacity : (1u << 20);
     h->hit_cap = h->opts.hit_capacity ? h->opts.hit_capacity : (1u << 20);
-    if (h->opts.waves_per_cu) h->waves_per_cu = h->opts.waves_per_cu;
+    if (h->opts.waves_per_cu) h->waves_per_cu = h->waves_per_cu_multi = h->opts.waves_per_cu;
     std::memset(&h->stats, 0, sizeof(h->stats));
     int rc = 0;
     auto fail = [&](int code) { pcagpu_close(h); return code; };
@@ -283,7 +283,10 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
         // persistent grid: as many resident workgroups as the scan kernel's
         // register and LDS use allow on one CU
         int blocks = 0;
-        if (scan_occupancy(&blocks) == hipSuccess && blocks > 0) h->waves_per_cu = (uint32_t)blocks * SCAN_WAVES;
+        if (scan_occupancy(false, &blocks) == hipSuccess && blocks > 0) h->waves_per_cu = (uint32_t)blocks * SCAN_WAVES;
+        blocks = 0;
+        if (scan_occupancy(true, &blocks) == hipSuccess && blocks > 0)
+            h->waves_per_cu_multi = (uint32_t)blocks * SCAN_WAVES;
     }
     h->ct.build();
     fill_devtask(h);

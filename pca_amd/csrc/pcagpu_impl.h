@@ -75,7 +75,8 @@ struct pcagpu {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pcagpu_opts_t opts;
-    uint32_t waves_per_cu = 20;
+    uint32_t waves_per_cu = 20;          // resident waves per CU, one-item scan launches
+    uint32_t waves_per_cu_multi = 16;    // the same for multi-item launches
 
     DevBuf<uint16_t> d_lists;
     DevBuf<int> d_count, d_split;
@@ -124,7 +125,8 @@ struct pcagpu {
 namespace pca {
 int fill_devtask(pcagpu *h);
 CandDev cand_dev(const pcagpu *h);
-int grid_for(pcagpu *h, uint64_t units);
+// persistent grid for `units` work units (multi: a launch of several items)
+int grid_for(pcagpu *h, uint64_t units, bool multi = false);
 size_t handle_device_bytes(const pcagpu *h);
 // queue prefixes for the depth without flushing (the caller scans them)
 int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank);

@@ -366,7 +366,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         l.counters = s->d_counters.p;
         l.work = s->d_work.p;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
-        launch_scan(l, grid_for(in_launch[0]->h, units), s->stream);
+        launch_scan(l, s->items.data(), in_launch[0]->h->dt, grid_for(in_launch[0]->h, units, s->items.size() > 1), s->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));

@@ -176,13 +176,10 @@ def random_task(seed, index):
     return Task.from_io(f"syn_{seed}_{index}", ins, outs, max_cycles=100)
 
 
-def shallow_task(seed, index, max_len=3):
-    """A serving request: the outputs of a seeded random program of 1..max_len
-    search-grammar instructions (ALU, LDI, ADDI over live registers) on 1 or 2
-    inputs and 8-32 seeded uniform tests, so the optimum is at most max_len
-    kernel instructions deep."""
-    from ._lib import gen_candidates
-    rng = Xoshiro256((seed * 7919 + index * 104729 + 17) & M64)
+def _shallow_draw(rng, max_len, gen_candidates):
+    """One program draw of shallow_task and its tests; None when 256 inputs
+    in a row divide by zero (a divisor that is constant zero), in which case
+    the caller draws another program from the same stream."""
     n_in = rng.randint(1, 2)
     live = sum(1 << (i + 1) for i in range(n_in))
     length = rng.randint(1, max_len)
@@ -198,11 +195,31 @@ def shallow_task(seed, index, max_len=3):
     prog += [(24 << 11) | (out_reg << 8), 1 << 11]
     n_tests = rng.randint(8, 32)
     ins, outs = [], []
+    misses = 0
     while len(ins) < n_tests:
         x = tuple(rng.u16() for _ in range(n_in))
         y = _interp(prog, x)
         if y is None:          # DIV/MOD by zero on this input: draw another
+            misses += 1
+            if misses == 256:
+                return None
             continue
+        misses = 0
         ins.append(x)
         outs.append(y)
+    return ins, outs
+
+
+def shallow_task(seed, index, max_len=3):
+    """A serving request: the outputs of a seeded random program of 1..max_len
+    search-grammar instructions (ALU, LDI, ADDI over live registers) on 1 or 2
+    inputs and 8-32 seeded uniform tests, so the optimum is at most max_len
+    kernel instructions deep."""
+    from ._lib import gen_candidates
+    rng = Xoshiro256((seed * 7919 + index * 104729 + 17) & M64)
+    while True:
+        drawn = _shallow_draw(rng, max_len, gen_candidates)
+        if drawn is not None:
+            ins, outs = drawn
+            break
     return Task.from_io(f"srv_{seed}_{index}", ins, outs, max_cycles=100)

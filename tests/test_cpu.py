@@ -249,3 +249,14 @@ def test_taskgen_workloads(pca):
     first = [r.next() for _ in range(3)]
     r2 = taskgen.Xoshiro256(42)
     assert first == [r2.next() for _ in range(3)]
+
+
+def test_shallow_task_redraws_constant_zero_divisor(pca):
+    """Index 12 of seed 17 first draws a program whose divisor is constant
+    zero; the generator must move on to another program, not loop."""
+    from pca_amd import taskgen
+    for i in range(40):
+        t = taskgen.shallow_task(17, i)
+        assert 8 <= t.num_tests <= 32
+    a, b = taskgen.shallow_task(17, 12), taskgen.shallow_task(17, 12)
+    assert bytes(a) == bytes(b)
