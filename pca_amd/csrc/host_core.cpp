@@ -5,6 +5,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
+#include <map>
 #include <string>
 
 namespace pca {
@@ -233,7 +235,7 @@ void disasm(uint16_t insn, char *buf, size_t len)
 
 // ------------------------------------------------------------- grammar
 namespace {
-const uint8_t kLdi[12] = {0, 1, 2, 3, 4, 5, 8, 16, 32, 64, 128, 255};
+const uint8_t (&kLdi)[12] = kLdiImms;
 const int8_t kAddi[6] = {-1, 1, -2, 2, -3, 3};
 const uint32_t kAlu[11] = {OP_ADD, OP_SUB, OP_MUL, OP_DIV, OP_MOD, OP_AND,
                            OP_OR,  OP_XOR, OP_SHL, OP_SHR, OP_ASR};
@@ -327,6 +329,75 @@ void CandTable::build()
         }
         vg_n[m] = (int)keys.size();
     }
+    bits_init.assign((size_t)128 * NBK, 0);
+    bits_gidx.assign((size_t)128 * NBK * 64, 0xFFFF);
+
+    for (int m = 0; m < 128; m++)
+        for (int g = 0; g < vg_n[m]; g++) {
+            const uint32_t c = vg_insn[(size_t)m * VG_MAX + g];
+            const uint32_t op = insn_op(c);
+            int kind, bit;
+            if (op >= OP_ADD && op <= OP_ASR) {
+                kind = (int)(op - OP_ADD);
+                bit = (int)(insn_rs(c) * 8 + insn_rt(c));
+            } else if (op == OP_ADDI) {
+                const int v = insn_simm(c);
+                kind = 11;
+                bit = (int)insn_rd(c) * 8 + (v < 0 ? 2 * (-v - 1) : 2 * (v - 1) + 1);
+            } else if (op == OP_LDI) {
+                kind = 12;
+                bit = 0;
+                while (bit < 12 && kLdi[bit] != insn_imm(c)) bit++;
+            } else {
+                std::abort();   // the grammar has no other value-writing kind
+            }
+            uint64_t &init = bits_init[(size_t)m * NBK + kind];
+            if (bit >= 64 || (init >> bit & 1ull)) std::abort();   // one group per bit
+            init |= 1ull << bit;
+            bits_gidx[((size_t)m * NBK + kind) * 64 + bit] = (uint16_t)g;
+            // the device takes r1..r7 as the rd values of every DIV / MOD group
+            if ((op == OP_DIV || op == OP_MOD) && vg_rdmask[(size_t)m * VG_MAX + g] != 0xFE) std::abort();
+        }
+}
+
+std::vector<int> device_test_order(const pca_task_t *t)
+{
+    const int T = t->num_tests;
+    auto score = [&](int k) {
+        int sc = 0;
+        for (int i = 0; i < t->num_inputs; i++) {
+            const uint32_t x = t->tests[k].in[i];
+            const int pc = __builtin_popcount(x);
+            sc += 64 - 8 * std::abs(pc - 8) + (int)(std::min<uint32_t>(x, 65536u - x) >> 10);
+        }
+        return sc;
+    };
+    std::map<uint16_t, std::vector<int>> by_val;
+    for (int k = 0; k < T; k++) by_val[t->tests[k].out[0]].push_back(k);
+    std::vector<std::vector<int>> groups;
+    for (auto &kv : by_val) {
+        std::vector<int> g = kv.second;
+        std::stable_sort(g.begin(), g.end(), [&](int x, int y) { return score(x) > score(y); });
+        groups.push_back(g);
+    }
+    std::stable_sort(groups.begin(), groups.end(), [&](const std::vector<int> &x, const std::vector<int> &y) {
+        return score(x[0]) > score(y[0]);
+    });
+    std::vector<int> order;
+    for (size_t r = 0; order.size() < (size_t)T; r++)
+        for (auto &g : groups)
+            if (r < g.size()) order.push_back(g[r]);
+    return order;
+}
+
+const CandTable &shared_grammar()
+{
+    static const CandTable table = [] {
+        CandTable t;
+        t.build();
+        return t;
+    }();
+    return table;
 }
 
 void root_states(const pca_task_t *t, TestState *out)

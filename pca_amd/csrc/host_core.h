@@ -49,6 +49,13 @@ constexpr int CAND_LMAX = 3776;   // >= 3,745 (all registers live)
 // each distinct (op, sources, immediate) once and keeps a mask of the rd
 // values still possible.  CMP is left out (it writes no register).
 constexpr int VG_MAX = 640;        // >= distinct value groups of any mask
+// Bit layout of the value groups for the wave-mask scan: kind 0..10 = the ALU
+// ops ADD..ASR with bit 8*rs + rt, kind 11 = ADDI with bit 8*rd + index of
+// the immediate in {-1, 1, -2, 2, -3, 3}, kind 12 = LDI with bit = index of
+// the immediate in {0, 1, 2, 3, 4, 5, 8, 16, 32, 64, 128, 255}.
+constexpr int NBK = 13;
+// LDI immediates of the grammar, in list order (tools/enumerate.c:198)
+inline constexpr uint8_t kLdiImms[12] = {0, 1, 2, 3, 4, 5, 8, 16, 32, 64, 128, 255};
 struct CandTable {
     std::vector<uint16_t> list;   // [128][CAND_LMAX]
     int n[128];
@@ -58,6 +65,8 @@ struct CandTable {
     std::vector<uint16_t> vg_j;      // [128][VG_MAX][8] list index of (group, rd)
     int vg_n[128];
     uint32_t cand_total[128];        // sum over the list of live out-registers
+    std::vector<uint64_t> bits_init;    // [128][NBK] groups present, one bit each
+    std::vector<uint16_t> bits_gidx;    // [128][NBK][64] group index of a bit (0xFFFF: none)
     void build();
     int count(uint32_t live, int mf) const { return n[(live >> 1) & 127] + (mf > 0 ? 6 * mf : 0); }
     uint16_t at(uint32_t live, int mf, int j) const
@@ -72,6 +81,18 @@ struct CandTable {
         return list[(size_t)m * CAND_LMAX + j - nbr];
     }
 };
+
+// Test order on the device.  Tests whose inputs look random (about half the
+// bits set, far from 0 and 0xFFFF) come first: few candidates map such an
+// input to the target by accident, while inputs like 0 or 1 are matched by
+// many constant-ish candidates.  Equal-target tests are interleaved
+// round-robin so a near-constant candidate fails within two tests.  The
+// order only decides when a lane stops testing, never which programs pass.
+std::vector<int> device_test_order(const pca_task_t *t);
+
+// The grammar tables, built once per process on first use (thread-safe) and
+// never modified afterwards.
+const CandTable &shared_grammar();
 
 // Per-test root state: the registers after the IN scaffold
 // (tools/enumerate.c:598-604) computed by running it on the VM.

@@ -23,6 +23,9 @@ struct DevTask {
     int max_cycles;          // clamped to 255
     int pad;
     uint16_t tgt[MAXT];      // expected output port value per test
+    // per test, tgt | ldi_pass << 16 (bit i of ldi_pass: the i-th LDI
+    // immediate of the grammar equals tgt), one dword for a scalar load
+    uint32_t tgtw[MAXT];
     TestState root[MAXT];    // per-test state after the IN scaffold
 };
 
@@ -39,18 +42,31 @@ struct HitRec {
 static_assert(sizeof(HitRec) == 24, "HitRec layout");
 
 constexpr int VGMAX = 640;                      // == VG_MAX
+constexpr int NBKINDS = 13;                     // == NBK
 
+// The branch-mode grammar tables (CandTable) in one device block, shared by
+// every handle on a device: per live mask m over r1..r7 the non-BR list in
+// reference order, its count and BR split point, the total of live
+// out-registers, and the last-position value groups as wave bit masks.
+struct GrammarDev {
+    uint16_t lists[128][LMAX];
+    int count[128];
+    int split[128];
+    uint32_t cand_total[128];
+    uint8_t vg_rdmask[128][VGMAX];            // rd values of a value group
+    uint16_t vg_j[128][VGMAX][8];             // list index of (group, rd)
+    uint64_t bits_init[128][NBKINDS];         // groups present per kind (ADD..ASR, ADDI, LDI)
+    uint16_t bits_gidx[128][NBKINDS][64];     // group of a bit (0xFFFF: none)
+};
+
+// What the kernels see: the shared tables, and the per-mask scalars, which a
+// kernel may point at its own LDS copies.
 struct CandDev {
-    const uint16_t *lists;   // [128][LMAX]
-    const int *count;        // [128]
-    const int *split;        // [128]
-    // last-position value groups (see CandTable): representative insn, rd
-    // mask, list index per (group, rd), group count, total out-registers
-    const uint16_t *vg_insn;       // [128][VGMAX]
-    const uint8_t *vg_rdmask;      // [128][VGMAX]
-    const uint16_t *vg_j;          // [128][VGMAX][8]
-    const int *vg_n;               // [128]
-    const uint32_t *cand_total;    // [128]
+    const GrammarDev *g;
+    const int *count;           // [128]
+    const int *split;           // [128]
+    const uint32_t *cand_total; // [128]
+    const uint64_t *bits_init;  // [128][NBKINDS]
 };
 
 // Work units of a scan launch.  k <= 1: unit = prefix.  k >= 2: units are
@@ -63,12 +79,22 @@ struct ScanGroup {
     uint16_t pad;
 };
 
+// Work unit of a scan launch for k >= 2 suffix positions: up to 64
+// consecutive children of one node a level above the scanned parents'
+// first level -- children at position s of prefix `prefix` (k == 2), or at
+// position s+1 of the prefix followed by its position-s candidate js
+// (k >= 3).  The block holds children j0 .. min(j0 + 64, count) - 1.
+struct ScanBlock {
+    uint32_t prefix;
+    uint16_t js;
+    uint16_t j0;
+};
+constexpr int BLOCK_KIDS = 64;
+
 struct ScanArgs {
     const pca_prefix_t *prefixes;
     uint32_t n_prefix;
-    const ScanGroup *groups;
-    const uint64_t *group_off;   // [n_groups + 1] exclusive prefix sums of unit counts
-    uint32_t n_groups;
+    const ScanBlock *blocks;     // k >= 2: one unit per block; k <= 1: one unit per prefix
     uint64_t n_units;
     int D, s, k;
     CandDev cand;
@@ -133,10 +159,9 @@ void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long 
 struct ScanItem {
     const DevTask *task;
     const pca_prefix_t *prefixes;
-    const ScanGroup *groups;
-    const uint64_t *group_off;
+    const ScanBlock *blocks;
     uint32_t n_prefix;
-    uint32_t n_groups;
+    uint32_t pad0;
     uint64_t unit_begin;             // first global unit of this item
     uint64_t n_units;
     int D, s, k, pad;
@@ -149,6 +174,7 @@ struct ScanItem {
 struct ScanLaunch {
     const ScanItem *items;
     uint32_t n_items;
+    uint32_t levels;                 // state levels per wave in LDS: max over items of max(k, 1)
     uint64_t n_units;                // sum over the items
     CandDev cand;
     unsigned long long *counters;    // [0] candidates [1] suffixes [2] tests [3] parents
@@ -156,7 +182,9 @@ struct ScanLaunch {
 };
 
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
-hipError_t scan_occupancy(bool multi, int *blocks_per_cu);
+// LDS bytes of a scan launch with `levels` state levels per wave
+size_t scan_dyn_lds(uint32_t levels);
+hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu);
 // host_items: the same items as l.items, in host memory; task0: the task
 // record of item 0.  A one-item launch passes both in the kernel arguments.
 void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask &task0, int grid,

@@ -24,6 +24,13 @@ __device__ __forceinline__ void wave_sync()
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// lane 0's value, known uniform to the compiler (SGPRs, uniform branches)
+__device__ __forceinline__ unsigned long long uni64(unsigned long long v)
+{
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
 
 __device__ __forceinline__ int cand_count(const CandDev &c, uint32_t live, int mf)
 {
@@ -36,13 +43,13 @@ __device__ __forceinline__ uint32_t cand_at(const CandDev &c, uint32_t live, int
 {
     const int m = (live >> 1) & 127;
     const int a = c.split[m];
-    if (j < a) return c.lists[m * LMAX + j];
+    if (j < a) return c.g->lists[m][j];
     const int nbr = mf > 0 ? 6 * mf : 0;
     if (j < a + nbr) {
         const int q = j - a;
         return enc_i(OP_BR, 1 + q / mf, 1 + q % mf);
     }
-    return c.lists[m * LMAX + j - nbr];
+    return c.g->lists[m][j - nbr];
 }
 
 __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
@@ -218,105 +225,178 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 
 // Value a value-group representative writes on one test; DIV/MOD only when
 // the wave holds any (the caller guards), everything else branch-free.
-__device__ __forceinline__ uint32_t group_value(uint32_t op, uint32_t x, uint32_t y, uint32_t imm)
-{
-    const uint32_t sh = y & 15u;
-    uint32_t v = x + y;
-    v = op == OP_SUB ? x - y : v;
-    v = op == OP_MUL ? x * y : v;
-    v = op == OP_AND ? (x & y) : v;
-    v = op == OP_OR ? (x | y) : v;
-    v = op == OP_XOR ? (x ^ y) : v;
-    v = op == OP_SHL ? x << sh : v;
-    v = op == OP_SHR ? x >> sh : v;
-    v = op == OP_ASR ? (uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) : v;
-    v = op == OP_LDI ? imm : v;
-    v = op == OP_ADDI ? x + (uint32_t)(int32_t)(int8_t)(uint8_t)imm : v;
-    return v & 0xFFFFu;
-}
-
-// Parent with no target-matching register (the common case): scan the
-// last position by value groups.  A lane owns one (op, sources, imm) group
-// and a mask `ok` of the rd values that still match: a test on which the
-// instruction runs keeps all or none of them, a test on which it does not
-// (skipped, VM stopped, DIV/MOD fault) keeps those rd whose old value
-// already equals the target (regmatch).  Tests 0-1 run for every lane; the
-// survivors go through an LDS queue to the early-exit loop over the rest.
-__device__ __forceinline__ void scan_parent_groups(const DevTask &task, const ScanArgs &a, const TestState *P,
-                                                   uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
-                                                   int lane, WaveCounters &wc, uint16_t *queue,
-                                                   const uint8_t *regmatch, const ParentSummary &ps)
+// Parent with no target-matching register (the common case): scan the last
+// position by value groups held as wave bit masks.  Lane 8*a + b holds the
+// source pair (a, b); each ALU op keeps a 64-bit mask of its (op, a, b)
+// groups still matching, updated by one compare ballot per test, so one
+// test of every group of the live mask costs one evaluation per op, and an
+// op with no group left is skipped.  ADDI groups use lane 8*rd + i (source
+// rd, i-th immediate of -1, 1, -2, 2, -3, 3); LDI groups are per-test
+// constants (DevTask::tgtw).  A test on which the last instruction does
+// not run (skipped by a pending BR, VM stopped) keeps the rd values whose old
+// register already equals the target: that regmatch mask is ANDed into the
+// rd masks of all groups (rm_and) and prunes the groups left without an rd.
+// DIV and MOD keep a per-lane rd mask, since a zero divisor (a fault) makes
+// a test behave like that for that lane only.  `a.cand.bits_init` points at
+// the kernel's LDS copy of the per-mask group bits.
+__device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const TestState *P,
+                                                 uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
+                                                 int lane, WaveCounters &wc, const uint8_t *regmatch,
+                                                 uint32_t *queue, const ParentSummary &ps)
 {
     const int T = task.T;
     const int m = (live >> 1) & 127;
-    const int ng = a.cand.vg_n[m];
-    const uint16_t *G = a.cand.vg_insn + m * VGMAX;
-    const uint8_t *GM = a.cand.vg_rdmask + m * VGMAX;
-    const uint16_t *GJ = a.cand.vg_j + (size_t)m * VGMAX * 8;
     if (lane == 0) {
         wc.cand += a.cand.cand_total[m];
         wc.suf += (unsigned)a.cand.count[m];
     }
-    uint32_t n_tests = 0;
-    uint32_t qn = 0;
-    const int T0 = T < 2 ? T : 2;
-
-    // evaluates tests [t0, t1) for the lane's group, updating ok / fault
-    auto run_tests = [&](uint32_t g, int t0, int t1, uint32_t &ok, bool &fault, bool early) {
-        const uint32_t op = insn_op(g), rd = insn_rd(g), rs = insn_rs(g), rt = insn_rt(g), imm = insn_imm(g);
-        const uint32_t sa = op == OP_ADDI ? rd : rs;
-        const bool divmod = op == OP_DIV || op == OP_MOD;
-        const bool any_div = __any(divmod && ok != 0u);
-        for (int t = t0; t < t1; t++) {
-            if (early && !__any(ok != 0u)) break;
-            if (ok != 0u) {
-                const uint32_t x = P[t].r[sa], y = P[t].r[rt];
-                uint32_t v = group_value(op, x, y, imm);
-                if (any_div && divmod && y != 0u) v = op == OP_DIV ? sdiv16(x, y) : smod16(x, y);
-                const bool ran = !((ps.skipm >> t) & 1ull);
-                const bool f = ran && divmod && y == 0u;
-                fault = fault || f;
-                ok &= (ran && !f) ? (v == task.tgt[t] ? 0xFFu : 0u) : regmatch[t];
-                n_tests++;
-            }
+    const uint64_t *B = a.cand.bits_init + m * NBKINDS;
+    uint64_t mk[NBKINDS];
+#pragma unroll
+    for (int q = 0; q < NBKINDS; q++) mk[q] = B[q];
+    // every DIV / MOD group has all of r1..r7 as rd (the host checks this)
+    uint32_t okd = (mk[OP_DIV - OP_ADD] >> lane & 1ull) ? 0xFEu : 0u;
+    uint32_t okm = (mk[OP_MOD - OP_ADD] >> lane & 1ull) ? 0xFEu : 0u;
+    mk[OP_DIV - OP_ADD] = mk[OP_MOD - OP_ADD] = 0;   // tracked per lane in okd / okm
+    bool fault = false;
+    uint32_t rm_and = 0xFFu;
+    const uint32_t ra = (uint32_t)lane >> 3, rb = (uint32_t)lane & 7u;
+    const uint32_t addi_imm = (rb & 1u) ? (rb >> 1) + 1u : 0x10000u - ((rb >> 1) + 1u);
+    uint32_t evals = 0;
+    // phase A: the first tests by masks, one evaluation per op for all groups
+    const int TA = T < 2 ? T : 2;
+    int t = 0;
+    for (; t < TA; t++) {
+        const uint64_t dvm = __ballot(okd != 0u), mdm = __ballot(okm != 0u);
+        uint64_t any = dvm | mdm;
+#pragma unroll
+        for (int q = 0; q < NBKINDS; q++) any |= mk[q];
+        if (any == 0ull) break;
+        // group evaluations of this test; once a skipped test has narrowed
+        // the rd values (rm_and), groups left without one are not told
+        // apart here, so such parents are not counted further
+        if (rm_and == 0xFFu) {
+            uint32_t alive = (uint32_t)__popcll(dvm) + (uint32_t)__popcll(mdm);
+#pragma unroll
+            for (int q = 0; q < NBKINDS; q++) alive += (uint32_t)__popcll(mk[q]);
+            evals += alive;
         }
-    };
-    auto drain = [&](uint32_t cnt) {
-        const bool valid = (uint32_t)lane < cnt;
-        const uint32_t gi = valid ? queue[lane] : 0u;
-        const uint32_t g = valid ? G[gi] : 0u;
-        uint32_t ok = valid ? GM[gi] : 0u;
-        bool fault = false;
-        run_tests(g, 0, T, ok, fault, true);
-        if (ok) {
-            const uint32_t fl = (ps.dead || fault) ? 1u : 0u;
-            for (uint32_t r = 1; r < 8; r++)
-                if (ok >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[gi * 8 + r], 1u << r, fl);
-        }
-    };
-    uint32_t g_next = lane < ng ? G[lane] : 0u;
-    uint32_t m_next = lane < ng ? GM[lane] : 0u;
-    for (int base = 0; base < ng; base += 64) {
-        const int gi = base + lane;
-        const uint32_t g = g_next;
-        uint32_t ok = m_next;
-        // the next chunk's group record, in flight while this one computes
-        g_next = gi + 64 < ng ? G[gi + 64] : 0u;
-        m_next = gi + 64 < ng ? GM[gi + 64] : 0u;
-        bool fault = false;
-        run_tests(g, 0, T0, ok, fault, false);
-        const bool alive = ok != 0u;
-        if (T <= 2) {
-            if (alive) {
-                const uint32_t fl = (ps.dead || fault) ? 1u : 0u;
-                for (uint32_t r = 1; r < 8; r++)
-                    if (ok >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[gi * 8 + r], 1u << r, fl);
-            }
+        if ((ps.skipm >> t) & 1ull) {
+            const uint32_t rmt = uni(regmatch[t]);
+            okd &= rmt;
+            okm &= rmt;
+            rm_and &= rmt;
             continue;
         }
-        const uint64_t sb = __ballot(alive);
-        if (alive) queue[qn + (uint32_t)__popcll(sb & ((1ull << lane) - 1ull))] = (uint16_t)gi;
-        qn += (uint32_t)__popcll(sb);
+        const uint32_t x = P[t].r[ra], y = P[t].r[rb];
+        const uint32_t tw = task.tgtw[t];
+        const uint32_t tg = tw & 0xFFFFu;
+        const uint32_t sh = y & 15u;
+        if (mk[0]) mk[0] &= __ballot(((x + y) & 0xFFFFu) == tg);
+        if (mk[1]) mk[1] &= __ballot(((x - y) & 0xFFFFu) == tg);
+        if (mk[2]) mk[2] &= __ballot(((x * y) & 0xFFFFu) == tg);
+        if (mk[5]) mk[5] &= __ballot((x & y) == tg);
+        if (mk[6]) mk[6] &= __ballot((x | y) == tg);
+        if (mk[7]) mk[7] &= __ballot((x ^ y) == tg);
+        if (mk[8]) mk[8] &= __ballot(((x << sh) & 0xFFFFu) == tg);
+        if (mk[9]) mk[9] &= __ballot((x >> sh) == tg);
+        if (mk[10]) mk[10] &= __ballot(((uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) & 0xFFFFu) == tg);
+        if (mk[11]) mk[11] &= __ballot(((x + addi_imm) & 0xFFFFu) == tg);
+        mk[12] &= tw >> 16;
+        if (dvm | mdm) {
+            const bool z = y == 0u;
+            if (z) {
+                const uint32_t rmt = regmatch[t];
+                okd &= rmt;
+                okm &= rmt;
+                fault = fault || okd != 0u || okm != 0u;
+            } else {
+                const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
+                const int32_t qt = sx / sy;
+                if (((uint32_t)qt & 0xFFFFu) != tg) okd = 0;
+                if (((uint32_t)(sx - qt * sy) & 0xFFFFu) != tg) okm = 0;
+            }
+        }
+    }
+    if (lane == 0) wc.tests += evals;
+    if (rm_and == 0u) return;   // no rd value is possible for any group
+    mk[OP_DIV - OP_ADD] = __ballot(okd != 0u);
+    mk[OP_MOD - OP_ADD] = __ballot(okm != 0u);
+    // phase B: the survivors, one group per lane (queue entry: kind | bit << 4
+    // | rd mask << 10 | fault << 18), tested to the end with an early exit
+    const uint32_t fl_base = ps.dead ? 1u : 0u;
+    const uint16_t *GX = &a.cand.g->bits_gidx[m][0][0];
+    const uint8_t *GM = a.cand.g->vg_rdmask[m];
+    const uint16_t *GJ = &a.cand.g->vg_j[m][0][0];
+    const int t0 = t;
+    auto drain = [&](uint32_t cnt) {
+        const bool valid = (uint32_t)lane < cnt;
+        const uint32_t e = valid ? queue[lane] : 0u;
+        const uint32_t q = e & 15u, bit = (e >> 4) & 63u;
+        uint32_t ok = valid ? (e >> 10) & 0xFFu : 0u;
+        bool flt = (e >> 18) & 1u;
+        const uint32_t xa = bit >> 3, yb = bit & 7u, i = bit & 7u;
+        const uint32_t aimm = (i & 1u) ? (i >> 1) + 1u : 0x10000u - ((i >> 1) + 1u);
+        const bool divmod = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
+        const bool any_div = __any(divmod && ok != 0u);
+        uint32_t n_ev = 0;
+        for (int tt = t0; tt < T; tt++) {
+            const uint64_t al = __ballot(ok != 0u);
+            if (al == 0ull) break;
+            n_ev += (uint32_t)__popcll(al);
+            if ((ps.skipm >> tt) & 1ull) {
+                ok &= uni(regmatch[tt]);
+                continue;
+            }
+            const uint32_t tw = task.tgtw[tt];
+            const uint32_t tg = tw & 0xFFFFu;
+            const uint32_t x = P[tt].r[xa], y = P[tt].r[yb];
+            const uint32_t sh = y & 15u;
+            uint32_t v = x + y;
+            v = q == 1u ? x - y : v;
+            v = q == 2u ? x * y : v;
+            v = q == 5u ? (x & y) : v;
+            v = q == 6u ? (x | y) : v;
+            v = q == 7u ? (x ^ y) : v;
+            v = q == 8u ? x << sh : v;
+            v = q == 9u ? x >> sh : v;
+            v = q == 10u ? (uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) : v;
+            v = q == 11u ? x + aimm : v;
+            bool match = (v & 0xFFFFu) == tg;
+            if (q == 12u) match = (tw >> (16 + bit)) & 1u;
+            if (any_div && divmod) {
+                if (y == 0u) {
+                    ok &= regmatch[tt];
+                    flt = true;
+                    match = true;
+                } else {
+                    const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
+                    const int32_t qt = sx / sy;
+                    match = ((uint32_t)(q == OP_DIV - OP_ADD ? qt : sx - qt * sy) & 0xFFFFu) == tg;
+                }
+            }
+            if (!match) ok = 0;
+        }
+        if (lane == 0) wc.tests += n_ev;
+        if (ok != 0u) {
+            const uint32_t g = GX[q * 64 + bit];
+            const uint32_t okf = GM[g] & ok;
+            const uint32_t fl = fl_base | ((divmod && flt) ? 1u : 0u);
+            for (uint32_t r = 1; r < 8; r++)
+                if (okf >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[g * 8 + r], 1u << r, fl);
+        }
+    };
+    uint32_t qn = 0;
+    for (int q = 0; q < NBKINDS; q++) {
+        const uint64_t bits = mk[q];
+        if (bits == 0ull) continue;
+        if (bits >> lane & 1ull) {
+            const uint32_t okq = q == OP_DIV - OP_ADD ? okd : (q == OP_MOD - OP_ADD ? okm : rm_and);
+            const bool fq = (q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD) && fault;
+            queue[qn + (uint32_t)__popcll(bits & ((1ull << lane) - 1ull))] =
+                (uint32_t)q | ((uint32_t)lane << 4) | (okq << 10) | (fq ? 1u << 18 : 0u);
+        }
+        qn += (uint32_t)__popcll(bits);
         if (qn >= 64) {
             wave_sync();
             drain(64);
@@ -332,14 +412,13 @@ __device__ __forceinline__ void scan_parent_groups(const DevTask &task, const Sc
         drain(qn);
         wave_sync();
     }
-    wc.tests += n_tests;
 }
 
 // Parent where some register already equals the target: every candidate
 // that leaves it alone is a hit, so go candidate by candidate.
 __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P,
                                             uint32_t live, uint32_t prefix_idx, const uint32_t *jpath, int lane,
-                                            WaveCounters &wc, uint16_t *queue, const ParentSummary &ps)
+                                            WaveCounters &wc, uint32_t *queue, const ParentSummary &ps)
 {
     const int T = task.T;
     const uint32_t pm = ps.pm;
@@ -348,7 +427,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
 
     const int m = (live >> 1) & 127;
     const int n = a.cand.count[m];
-    const uint16_t *L = a.cand.lists + m * LMAX;
+    const uint16_t *L = a.cand.g->lists[m];
     uint32_t n_cand = 0, n_suf = 0, n_tests = 0;
     uint32_t qn = 0;   // queued survivors (wave-uniform)
     const bool two = T >= 2;
@@ -504,40 +583,42 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
     }
 }
 
-constexpr int SCAN_CHUNK = 32;   // units taken per atomic by one wave
-
 // MULTI = false: one item and its task record, passed by value in the kernel
 // arguments (the single-task path; no per-unit item lookup, the context is
-// read from the kernel-argument segment).  MULTI = true: lch.items[0..n_items) in device
-// memory, the item of each unit found by binary search over unit_begin.
+// read from the kernel-argument segment).  MULTI = true: lch.items[0..n_items)
+// in device memory, the item of each unit found by binary search over
+// unit_begin.  The per-wave state levels live in dynamic LDS, lch.levels of
+// them per wave.
 template <bool MULTI>
-__global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem one, DevTask task1)
+__device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem &one, const DevTask &task1)
 {
-    __shared__ TestState s_st[SCAN_WAVES][MAXLEV][MAXT];
+    extern __shared__ TestState s_dyn[];
+    __shared__ uint64_t s_bits[128][NBKINDS];
     __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
     __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
-    __shared__ uint16_t s_queue[SCAN_WAVES][128];
+    __shared__ uint32_t s_queue[SCAN_WAVES][128];
     __shared__ uint8_t s_rm[SCAN_WAVES][MAXT];
-    __shared__ int s_cnt[128], s_split[128], s_vgn[128];
+    __shared__ int s_cnt[128], s_split[128];
     __shared__ uint32_t s_ctot[128];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    // per-mask scalars live in LDS: no global round trip on a unit's setup
+    // per-mask tables live in LDS: no global round trip per parent
     for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
         s_cnt[i] = lch.cand.count[i];
         s_split[i] = lch.cand.split[i];
-        s_vgn[i] = lch.cand.vg_n[i];
         s_ctot[i] = lch.cand.cand_total[i];
     }
+    for (int i = threadIdx.x; i < 128 * NBKINDS; i += SCAN_WG) (&s_bits[0][0])[i] = (&lch.cand.g->bits_init[0][0])[i];
     __syncthreads();
     CandDev cand = lch.cand;
     cand.count = s_cnt;
     cand.split = s_split;
-    cand.vg_n = s_vgn;
     cand.cand_total = s_ctot;
+    cand.bits_init = &s_bits[0][0];
+    const uint32_t nlev = lch.levels;
+    auto lev = [&](int L) -> TestState * { return s_dyn + ((uint32_t)w * nlev + (uint32_t)L) * MAXT; };
 
     WaveCounters wc;
     // current item (wave-uniform) and its context
-    uint32_t cur_item = 0xFFFFFFFFu;
     uint64_t it_begin = 0, it_end = 0;
     const DevTask *taskp = nullptr;
     ScanArgs a;
@@ -546,13 +627,10 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem 
     a.work = lch.work;
     uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
     if constexpr (!MULTI) {
-        cur_item = 0;
         taskp = &task1;
         a.prefixes = one.prefixes;
         a.n_prefix = one.n_prefix;
-        a.groups = one.groups;
-        a.group_off = one.group_off;
-        a.n_groups = one.n_groups;
+        a.blocks = one.blocks;
         a.n_units = one.n_units;
         a.D = one.D;
         a.s = one.s;
@@ -564,98 +642,88 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem 
         it_end = one.n_units;
     }
     for (;;) {
-        unsigned long long u0 = 0;
-        if (lane == 0) u0 = atomicAdd(lch.work, (unsigned long long)SCAN_CHUNK);
-        u0 = __shfl(u0, 0);
-        if (u0 >= lch.n_units) break;
-        const unsigned long long u1 = u0 + SCAN_CHUNK < lch.n_units ? u0 + SCAN_CHUNK : lch.n_units;
-        uint32_t g = 0;
-        bool g_valid = false;
-        for (unsigned long long ug = u0; ug < u1; ug++) {
-            if (MULTI && (ug < it_begin || ug >= it_end)) {
-                // the item holding this unit: largest ii with unit_begin <= ug
-                uint32_t lo = 0, hi = lch.n_items;
-                while (hi - lo > 1) {
-                    const uint32_t mid = (lo + hi) >> 1;
-                    if (lch.items[mid].unit_begin <= ug) lo = mid; else hi = mid;
-                }
-                cur_item = uni(lo);
-                const ScanItem &it = lch.items[cur_item];
-                taskp = it.task;
-                a.prefixes = it.prefixes;
-                a.n_prefix = it.n_prefix;
-                a.groups = it.groups;
-                a.group_off = it.group_off;
-                a.n_groups = it.n_groups;
-                a.n_units = it.n_units;
-                a.D = it.D;
-                a.s = it.s;
-                a.k = it.k;
-                a.hits = it.hits;
-                a.hit_cap = it.hit_cap;
-                a.hit_count = it.hit_count;
-                it_begin = it.unit_begin;
-                it_end = it.unit_begin + it.n_units;
-                cur_i = 0xFFFFFFFFu;
-                cur_js = 0xFFFFFFFFu;
-                g_valid = false;
+        unsigned long long ug = 0;
+        if (lane == 0) ug = atomicAdd(lch.work, 1ull);
+        ug = uni64(ug);
+        if (ug >= lch.n_units) break;
+        if (MULTI && (ug < it_begin || ug >= it_end)) {
+            // the item holding this unit: largest ii with unit_begin <= ug
+            uint32_t lo = 0, hi = lch.n_items;
+            while (hi - lo > 1) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (lch.items[mid].unit_begin <= ug) lo = mid; else hi = mid;
             }
-            const DevTask &task = *taskp;
-            const int k = a.k;
-            // level of the parent a unit designates: 0 (prefix), 1 (+ position
-            // s), 2 (+ position s+1)
-            const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
-            const uint64_t u = ug - it_begin;
-            uint32_t pi, js = 0, js1 = 0;
-            if (k >= 2) {
-                if (!g_valid) {
-                    uint32_t lo = 0, hi = a.n_groups;   // largest g with group_off[g] <= u
-                    while (hi - lo > 1) {
-                        const uint32_t mid = (lo + hi) >> 1;
-                        if (a.group_off[mid] <= u) lo = mid; else hi = mid;
-                    }
-                    g = uni(lo);
-                    g_valid = true;
-                }
-                while (a.group_off[g + 1] <= u) g++;
-                const ScanGroup G = a.groups[g];
-                pi = uni(G.prefix);
-                const uint32_t j = uni((uint32_t)(u - a.group_off[g]));
-                if (k == 2) js = j;
-                else { js = uni(G.js); js1 = j; }
-            } else {
-                pi = uni((uint32_t)u);
-            }
-            const pca_prefix_t *P = &a.prefixes[pi];
-            const uint32_t live0 = uni(P->live);
-            if (pi != cur_i) {
-                replay_prefix(task, P, a.s, s_st[w][0], lane);
-                cur_i = pi;
-                cur_js = 0xFFFFFFFFu;
+            const ScanItem &it = lch.items[uni(lo)];
+            taskp = it.task;
+            a.prefixes = it.prefixes;
+            a.n_prefix = it.n_prefix;
+            a.blocks = it.blocks;
+            a.n_units = it.n_units;
+            a.D = it.D;
+            a.s = it.s;
+            a.k = it.k;
+            a.hits = it.hits;
+            a.hit_cap = it.hit_cap;
+            a.hit_count = it.hit_count;
+            it_begin = it.unit_begin;
+            it_end = it.unit_begin + it.n_units;
+            cur_i = 0xFFFFFFFFu;
+            cur_js = 0xFFFFFFFFu;
+        }
+        const DevTask &task = *taskp;
+        const int k = a.k;
+        const uint64_t u = ug - it_begin;
+        uint32_t pi, js = 0, j0 = 0;
+        if (k >= 2) {
+            const ScanBlock B = a.blocks[u];
+            pi = uni(B.prefix);
+            js = uni(B.js);
+            j0 = uni(B.j0);
+        } else {
+            pi = uni((uint32_t)u);
+        }
+        const pca_prefix_t *P = &a.prefixes[pi];
+        const uint32_t live0 = uni(P->live);
+        if (pi != cur_i) {
+            replay_prefix(task, P, a.s, lev(0), lane);
+            cur_i = pi;
+            cur_js = 0xFFFFFFFFu;
+            wave_sync();
+        }
+        if (k == 0) {
+            leaf_parent(task, a, lev(0), live0, pi, lane, wc);
+            continue;
+        }
+        if (lane == 0) s_live[w][0] = live0;
+        // level of the scanned parents' first level: 0 (the prefix itself,
+        // k == 1), 1 (k == 2), 2 (k >= 3); the block's children sit there
+        const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
+        uint32_t nb = 1, lvp = live0, cw = 0;
+        if (k >= 3 && js != cur_js) {
+            const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, (int)js));
+            step_level(task, lev(0), lev(1), c1, a.s, lane);
+            if (lane == 0) s_live[w][1] = live_after(live0, c1);
+            cur_js = js;
+        }
+        if (k >= 2) {
+            if (k >= 3) {
                 wave_sync();
-            }
-            if (k == 0) {
-                leaf_parent(task, a, s_st[w][0], live0, pi, lane, wc);
-                continue;
-            }
-            if (lane == 0) s_live[w][0] = live0;
-            if (k >= 2) {
-                if (js != cur_js) {
-                    const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, (int)js));
-                    step_level(task, s_st[w][0], s_st[w][1], c1, a.s, lane);
-                    if (lane == 0) s_live[w][1] = live_after(live0, c1);
-                    cur_js = js;
-                }
+                lvp = uni(s_live[w][1]);
                 if (lane == 0) s_idx[w][1] = js;
-                if (k >= 3) {
-                    wave_sync();
-                    const uint32_t lv1 = uni(s_live[w][1]);
-                    const uint32_t c2 = uni(cand_at(a.cand, lv1, a.D - a.s - 2, (int)js1));
-                    step_level(task, s_st[w][1], s_st[w][2], c2, a.s + 1, lane);
-                    if (lane == 0) {
-                        s_live[w][2] = live_after(lv1, c2);
-                        s_idx[w][2] = js1;
-                    }
+            }
+            const int mfp = a.D - a.s - Lu;   // children at position s + Lu - 1
+            const int nc = cand_count(a.cand, lvp, mfp);
+            nb = (int)j0 < nc ? (uint32_t)min(nc - (int)j0, BLOCK_KIDS) : 0u;
+            // the block's candidate words, one per lane, in one coalesced read
+            cw = (uint32_t)lane < nb ? cand_at(a.cand, lvp, mfp, (int)j0 + lane) : 0u;
+        }
+        for (uint32_t ci = 0; ci < nb; ci++) {
+            if (k >= 2) {
+                const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
+                step_level(task, lev(Lu - 1), lev(Lu), c, a.s + Lu - 1, lane);
+                if (lane == 0) {
+                    s_live[w][Lu] = live_after(lvp, c);
+                    s_idx[w][Lu] = j0 + ci;
                 }
             }
             if (lane == 0 && Lu + 1 < MAXLEV) s_idx[w][Lu + 1] = 0;
@@ -666,13 +734,13 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem 
                 if (L == k - 1) {
                     const uint32_t *jpath = &s_idx[w][1];
                     {
-                        const TestState *PP = s_st[w][L];
+                        const TestState *PP = lev(L);
                         const uint32_t lvL = uni(s_live[w][L]);
                         const ParentSummary ps = summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, s_rm[w]);
                         wc.parents += lane == 0;
                         wave_sync();
                         if (ps.pm == 0u)
-                            scan_parent_groups(task, a, PP, lvL, pi, jpath, lane, wc, s_queue[w], s_rm[w], ps);
+                            scan_parent_bits(task, a, PP, lvL, pi, jpath, lane, wc, s_rm[w], s_queue[w], ps);
                         else
                             scan_parent(task, a, PP, lvL, pi, jpath, lane, wc, s_queue[w], ps);
                     }
@@ -696,7 +764,7 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem 
                     continue;
                 }
                 const uint32_t c = uni(cand_at(a.cand, lvL, mf, ix));
-                step_level(task, s_st[w][L], s_st[w][L + 1], c, pos, lane);
+                step_level(task, lev(L), lev(L + 1), c, pos, lane);
                 if (lane == 0) {
                     s_live[w][L + 1] = live_after(lvL, c);
                     if (L + 2 < MAXLEV) s_idx[w][L + 2] = 0;
@@ -714,6 +782,19 @@ __global__ __launch_bounds__(SCAN_WG) void scan_kernel(ScanLaunch lch, ScanItem 
         atomicAdd(&lch.counters[2], c2);
         atomicAdd(&lch.counters[3], c3);
     }
+}
+
+// One-item launches: register budget of 5 waves per SIMD (at most 96
+// VGPRs), measured faster than the 4 waves of the unconstrained allocation
+// despite a few spills.  Multi-item launches keep the unconstrained budget.
+__global__ __launch_bounds__(SCAN_WG, 5) void scan_kernel_one(ScanLaunch lch, ScanItem one, DevTask task1)
+{
+    scan_body<false>(lch, one, task1);
+}
+
+__global__ __launch_bounds__(SCAN_WG) void scan_kernel_multi(ScanLaunch lch, ScanItem one, DevTask task1)
+{
+    scan_body<true>(lch, one, task1);
 }
 
 // ------------------------------------------------------------ wavefront
@@ -1204,19 +1285,23 @@ void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st)
     hipLaunchKernelGGL(table_init_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table, cap);
 }
 
-hipError_t scan_occupancy(bool multi, int *blocks_per_cu)
+size_t scan_dyn_lds(uint32_t levels) { return (size_t)SCAN_WAVES * levels * MAXT * sizeof(TestState); }
+
+hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
 {
-    if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel<true>, SCAN_WG, 0);
-    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel<false>, SCAN_WG, 0);
+    const size_t lds = scan_dyn_lds(levels);
+    if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_multi, SCAN_WG, lds);
+    return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_one, SCAN_WG, lds);
 }
 
 void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask &task0, int grid,
                  hipStream_t st)
 {
+    const size_t lds = scan_dyn_lds(l.levels);
     if (l.n_items == 1)
-        hipLaunchKernelGGL(scan_kernel<false>, dim3(grid), dim3(SCAN_WG), 0, st, l, host_items[0], task0);
+        hipLaunchKernelGGL(scan_kernel_one, dim3(grid), dim3(SCAN_WG), lds, st, l, host_items[0], task0);
     else
-        hipLaunchKernelGGL(scan_kernel<true>, dim3(grid), dim3(SCAN_WG), 0, st, l, host_items[0], task0);
+        hipLaunchKernelGGL(scan_kernel_multi, dim3(grid), dim3(SCAN_WG), lds, st, l, host_items[0], task0);
 }
 
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st)

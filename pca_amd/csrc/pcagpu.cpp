@@ -13,6 +13,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <set>
 #include <new>
 #include <string>
@@ -25,12 +27,8 @@ using namespace pca;
 
 namespace pca {
 
-// Test order on the device.  Tests whose inputs look random (about half the
-// bits set, far from 0 and 0xFFFF) come first: few candidates map such an
-// input to the target by accident, while inputs like 0 or 1 are matched by
-// many constant-ish candidates.  Equal-target tests are interleaved
-// round-robin so a near-constant candidate fails within two tests.  The
-// order only decides when a lane stops testing, never which programs pass.
+static_assert(NBKINDS == NBK && VGMAX == VG_MAX && LMAX == CAND_LMAX, "host and device grammar tables differ");
+
 int fill_devtask(pcagpu *h)
 {
     std::memset(&h->dt, 0, sizeof(h->dt));
@@ -39,61 +37,82 @@ int fill_devtask(pcagpu *h)
     h->dt.max_cycles = (int)std::min<uint64_t>(h->task.max_cycles, 255);
     TestState roots[PCAGPU_MAX_TESTS];
     root_states(&h->task, roots);
-    auto score = [&](int t) {
-        int sc = 0;
-        for (int i = 0; i < h->task.num_inputs; i++) {
-            const uint32_t x = h->task.tests[t].in[i];
-            const int pc = __builtin_popcount(x);
-            sc += 64 - 8 * std::abs(pc - 8) + (int)(std::min<uint32_t>(x, 65536u - x) >> 10);
-        }
-        return sc;
-    };
-    std::map<uint16_t, std::vector<int>> by_val;
-    for (int t = 0; t < h->T; t++) by_val[h->task.tests[t].out[0]].push_back(t);
-    std::vector<std::vector<int>> groups;
-    for (auto &kv : by_val) {
-        std::vector<int> g = kv.second;
-        std::stable_sort(g.begin(), g.end(), [&](int x, int y) { return score(x) > score(y); });
-        groups.push_back(g);
-    }
-    std::stable_sort(groups.begin(), groups.end(), [&](const std::vector<int> &x, const std::vector<int> &y) {
-        return score(x[0]) > score(y[0]);
-    });
-    std::vector<int> order;
-    for (size_t r = 0; order.size() < (size_t)h->T; r++)
-        for (auto &g : groups)
-            if (r < g.size()) order.push_back(g[r]);
+    const std::vector<int> order = device_test_order(&h->task);
     for (int t = 0; t < h->T; t++) {
         h->dt.tgt[t] = h->task.tests[order[t]].out[0];
+        h->dt.tgtw[t] = h->dt.tgt[t];
+        for (int i = 0; i < 12; i++)
+            if (kLdiImms[i] == h->dt.tgt[t]) h->dt.tgtw[t] |= 1u << (16 + i);
         h->dt.root[t] = roots[order[t]];
     }
     h->order = order;
     return 0;
 }
 
-CandDev cand_dev(const pcagpu *h)
+const GrammarDev *shared_grammar_dev(int device)
 {
-    return CandDev{h->d_lists.p, h->d_count.p, h->d_split.p, h->d_vg_insn.p, h->d_vg_rdmask.p,
-                   h->d_vg_j.p, h->d_vg_n.p, h->d_cand_total.p};
+    static std::mutex mu;
+    static std::map<int, const GrammarDev *> tables;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = tables.find(device);
+    if (it != tables.end()) return it->second;
+    const CandTable &ct = shared_grammar();
+    std::unique_ptr<GrammarDev> img(new (std::nothrow) GrammarDev);
+    if (!img) return nullptr;
+    std::memset(img.get(), 0, sizeof(GrammarDev));
+    for (int m = 0; m < 128; m++) {
+        std::memcpy(img->lists[m], &ct.list[(size_t)m * CAND_LMAX], sizeof(img->lists[m]));
+        img->count[m] = ct.n[m];
+        img->split[m] = ct.a[m];
+        img->cand_total[m] = ct.cand_total[m];
+        std::memcpy(img->vg_rdmask[m], &ct.vg_rdmask[(size_t)m * VG_MAX], sizeof(img->vg_rdmask[m]));
+        std::memcpy(img->vg_j[m], &ct.vg_j[(size_t)m * VG_MAX * 8], sizeof(img->vg_j[m]));
+        std::memcpy(img->bits_init[m], &ct.bits_init[(size_t)m * NBK], sizeof(img->bits_init[m]));
+        std::memcpy(img->bits_gidx[m], &ct.bits_gidx[(size_t)m * NBK * 64], sizeof(img->bits_gidx[m]));
+    }
+    GrammarDev *d = nullptr;
+    if (hipMalloc((void **)&d, sizeof(GrammarDev)) != hipSuccess) return nullptr;
+    if (hipMemcpy(d, img.get(), sizeof(GrammarDev), hipMemcpyHostToDevice) != hipSuccess) {
+        (void)hipFree(d);
+        return nullptr;
+    }
+    tables[device] = d;
+    return d;
 }
 
-int grid_for(pcagpu *h, uint64_t units, bool multi)
+CandDev cand_dev(const pcagpu *h)
 {
-    uint64_t waves = (uint64_t)h->cu_count * (multi ? h->waves_per_cu_multi : h->waves_per_cu);
+    const GrammarDev *g = h->d_grammar;
+    return CandDev{g, g->count, g->split, g->cand_total, &g->bits_init[0][0]};
+}
+
+int grid_for(pcagpu *h, uint64_t units, bool multi, uint32_t levels)
+{
+    uint32_t wpc = h->waves_per_cu;
+    if (!wpc) {
+        // persistent grid: as many resident workgroups as the scan kernel's
+        // registers and LDS (which grows with the state levels) allow
+        const uint32_t lv = levels ? std::min<uint32_t>(levels, (uint32_t)MAXLEV) : 3u;
+        uint32_t &c = h->scan_wpc[multi ? 1 : 0][lv];
+        if (!c) {
+            int b = 0;
+            c = (scan_occupancy(multi, lv, &b) == hipSuccess && b > 0) ? (uint32_t)b * SCAN_WAVES : 16u;
+        }
+        wpc = c;
+    }
+    uint64_t waves = (uint64_t)h->cu_count * wpc;
     uint64_t wgs = (std::min<uint64_t>(units, waves) + SCAN_WAVES - 1) / SCAN_WAVES;
     return (int)std::max<uint64_t>(wgs, 1);
 }
 
-int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
+int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
 {
     ScanItem it;
     std::memset(&it, 0, sizeof(it));
     it.task = h->d_task.p;
     it.prefixes = h->d_batch.p;
-    it.groups = h->d_groups.p;
-    it.group_off = h->d_goff.p;
+    it.blocks = h->d_blocks.p;
     it.n_prefix = n;
-    it.n_groups = n_groups;
     it.unit_begin = 0;
     it.n_units = units;
     it.D = h->D;
@@ -102,23 +121,21 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
     it.hits = h->d_hits.p;
     it.hit_count = h->d_hit_count.p;
     it.hit_cap = h->hit_cap;
-    if (n_groups) {
-        HIP_TRY(hipMemcpyAsync(h->d_groups.p, h->h_groups, (size_t)n_groups * sizeof(ScanGroup),
+    if (n_blocks)
+        HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
                                hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipMemcpyAsync(h->d_goff.p, h->h_goff, ((size_t)n_groups + 1) * sizeof(uint64_t),
-                               hipMemcpyHostToDevice, h->stream));
-    }
     HIP_TRY(hipMemcpyAsync(h->d_items.p, &it, sizeof(it), hipMemcpyHostToDevice, h->stream));
     ScanLaunch l;
     l.items = h->d_items.p;
     l.n_items = 1;
+    l.levels = scan_levels(h->k);
     l.n_units = units;
     l.cand = cand_dev(h);
     l.counters = h->d_counters.p;
     l.work = h->d_work.p;
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    launch_scan(l, &it, h->dt, grid_for(h, units), h->stream);
+    launch_scan(l, &it, h->dt, grid_for(h, units, false, l.levels), h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
@@ -129,47 +146,53 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_groups, uint64_t units)
     return 0;
 }
 
-// All scan launches of one flush: k <= 1 in one launch; k >= 2 in launches
-// of at most kGroupMax groups.
+// Appends the scan blocks of staged prefix i (k >= 2) to h_blocks at *nb;
+// false, with *nb unchanged, if they do not fit.
+bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb)
+{
+    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
+    const uint32_t live = h->h_batch[i].live;
+    const int ns = h->ct.count(live, mf_s);
+    uint32_t b = *nb;
+    auto push = [&](uint32_t js, int nc) {
+        for (int j0 = 0; j0 < nc; j0 += BLOCK_KIDS) {
+            if (b >= kBlockMax) return false;
+            h->h_blocks[b++] = ScanBlock{i, (uint16_t)js, (uint16_t)j0};
+        }
+        return true;
+    };
+    if (h->k == 2) {
+        if (!push(0, ns)) return false;
+    } else {
+        for (int js = 0; js < ns; js++) {
+            const uint32_t c = h->ct.at(live, mf_s, js);
+            if (!push((uint32_t)js, h->ct.count(live_after(live, c), mf_s1))) return false;
+        }
+    }
+    *nb = b;
+    return true;
+}
+
+// All scan launches of one flush: k <= 1 in one launch (a unit per prefix);
+// k >= 2 in launches of at most kBlockMax blocks.
 int scan_batch(pcagpu *h, uint32_t n)
 {
     if (h->k <= 1) return run_scan(h, n, 0, n);
-    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
-    uint32_t g = 0;
-    uint64_t acc = 0;
+    uint32_t nb = 0;
     for (uint32_t i = 0; i < n; i++) {
-        const uint32_t live = h->h_batch[i].live;
-        const int ns = h->ct.count(live, mf_s);
-        if (h->k == 2) {
-            h->h_groups[g] = ScanGroup{i, 0, 0};
-            h->h_goff[g] = acc;
-            acc += (uint64_t)ns;
-            g++;
-        } else {
-            for (int js = 0; js < ns; js++) {
-                const uint32_t c = h->ct.at(live, mf_s, js);
-                h->h_groups[g] = ScanGroup{i, (uint16_t)js, 0};
-                h->h_goff[g] = acc;
-                acc += (uint64_t)h->ct.count(live_after(live, c), mf_s1);
-                g++;
-            }
-        }
-        if (g + (uint32_t)CAND_LMAX + 64 > kGroupMax || i + 1 == n) {
-            h->h_goff[g] = acc;
-            int rc = run_scan(h, n, g, acc);
-            if (rc) return rc;
-            g = 0;
-            acc = 0;
-        }
+        if (add_prefix_blocks(h, i, &nb)) continue;
+        if (nb == 0) return PCAGPU_EOVERFLOW;
+        int rc = run_scan(h, n, nb, nb);
+        if (rc) return rc;
+        nb = 0;
+        i--;
     }
-    return 0;
+    return nb ? run_scan(h, n, nb, nb) : 0;
 }
 
 size_t handle_device_bytes(const pcagpu *h)
 {
-    return h->d_lists.bytes() + h->d_count.bytes() + h->d_split.bytes() + h->d_vg_insn.bytes() +
-           h->d_vg_j.bytes() + h->d_vg_rdmask.bytes() + h->d_vg_n.bytes() + h->d_cand_total.bytes() +
-           h->d_batch.bytes() + h->d_groups.bytes() + h->d_goff.bytes() + h->d_task.bytes() + h->d_items.bytes() +
+    return h->d_batch.bytes() + h->d_blocks.bytes() + h->d_groups.bytes() + h->d_goff.bytes() + h->d_task.bytes() + h->d_items.bytes() +
            h->d_hits.bytes() + h->d_hit_count.bytes() + h->d_counters.bytes() + h->d_work.bytes() +
            h->d_front[0].bytes() + h->d_front[1].bytes() + h->d_child_off.bytes() + h->d_keep_cnt.bytes() +
            h->d_scan_tmp.bytes() + h->d_table.bytes() + h->d_keep_bits.bytes() + h->d_q0.bytes() + h->d_qf.bytes() +
@@ -191,37 +214,18 @@ int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_
     return 0;
 }
 
-int build_groups(pcagpu *h, uint32_t *n_groups, uint64_t *n_units)
+int build_blocks(pcagpu *h, uint32_t *n_blocks, uint64_t *n_units)
 {
-    const uint32_t n = h->batch_n;
+    *n_blocks = 0;
     if (h->k <= 1) {
-        *n_groups = 0;
-        *n_units = n;
+        *n_units = h->batch_n;
         return 0;
     }
-    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
-    uint32_t g = 0;
-    uint64_t acc = 0;
-    for (uint32_t i = 0; i < n; i++) {
-        const uint32_t live = h->h_batch[i].live;
-        const int ns = h->ct.count(live, mf_s);
-        if (g + (uint32_t)ns + 1 > kGroupMax) return PCAGPU_EOVERFLOW;
-        if (h->k == 2) {
-            h->h_groups[g] = ScanGroup{i, 0, 0};
-            h->h_goff[g++] = acc;
-            acc += (uint64_t)ns;
-        } else {
-            for (int js = 0; js < ns; js++) {
-                const uint32_t c = h->ct.at(live, mf_s, js);
-                h->h_groups[g] = ScanGroup{i, (uint16_t)js, 0};
-                h->h_goff[g++] = acc;
-                acc += (uint64_t)h->ct.count(live_after(live, c), mf_s1);
-            }
-        }
-    }
-    h->h_goff[g] = acc;
-    *n_groups = g;
-    *n_units = acc;
+    uint32_t nb = 0;
+    for (uint32_t i = 0; i < h->batch_n; i++)
+        if (!add_prefix_blocks(h, i, &nb)) return PCAGPU_EOVERFLOW;
+    *n_blocks = nb;
+    *n_units = nb;
     return 0;
 }
 
@@ -268,7 +272,7 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     if (h->device < 0 || h->device >= ndev) { delete h; return PCAGPU_EINVAL; }
     h->batch_cap = h->opts.batch_capacity ? h->opts.batch_capacity : (1u << 20);
     h->hit_cap = h->opts.hit_capacity ? h->opts.hit_capacity : (1u << 20);
-    if (h->opts.waves_per_cu) h->waves_per_cu = h->waves_per_cu_multi = h->opts.waves_per_cu;
+    h->waves_per_cu = h->opts.waves_per_cu;
     std::memset(&h->stats, 0, sizeof(h->stats));
     int rc = 0;
     auto fail = [&](int code) { pcagpu_close(h); return code; };
@@ -279,44 +283,20 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PCAGPU_EHIP);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(PCAGPU_EHIP);
-    if (!h->opts.waves_per_cu) {
-        // persistent grid: as many resident workgroups as the scan kernel's
-        // register and LDS use allow on one CU
-        int blocks = 0;
-        if (scan_occupancy(false, &blocks) == hipSuccess && blocks > 0) h->waves_per_cu = (uint32_t)blocks * SCAN_WAVES;
-        blocks = 0;
-        if (scan_occupancy(true, &blocks) == hipSuccess && blocks > 0)
-            h->waves_per_cu_multi = (uint32_t)blocks * SCAN_WAVES;
-    }
-    h->ct.build();
     fill_devtask(h);
-    if ((rc = h->d_lists.ensure(h->ct.list.size())) || (rc = h->d_count.ensure(128)) ||
-        (rc = h->d_split.ensure(128)) || (rc = h->d_batch.ensure(h->batch_cap)) ||
-        (rc = h->d_groups.ensure(kGroupMax)) || (rc = h->d_goff.ensure(kGroupMax + 1)) ||
+    if (!(h->d_grammar = shared_grammar_dev(h->device))) return fail(PCAGPU_ENOMEM);
+    if ((rc = h->d_batch.ensure(h->batch_cap)) ||
+        (rc = h->d_blocks.ensure(kBlockMax)) ||
         (rc = h->d_hits.ensure(h->hit_cap)) ||
         (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(8)) || (rc = h->d_work.ensure(1)))
         return fail(rc);
     if (hipHostMalloc((void **)&h->h_batch, (size_t)h->batch_cap * sizeof(pca_prefix_t)) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_groups, (size_t)kGroupMax * sizeof(ScanGroup)) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_goff, ((size_t)kGroupMax + 1) * sizeof(uint64_t)) != hipSuccess)
+        hipHostMalloc((void **)&h->h_blocks, (size_t)kBlockMax * sizeof(ScanBlock)) != hipSuccess)
         return fail(PCAGPU_ENOMEM);
-    if (hipMemcpy(h->d_lists.p, h->ct.list.data(), h->ct.list.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->d_count.p, h->ct.n, sizeof(h->ct.n), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->d_split.p, h->ct.a, sizeof(h->ct.a), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
+    if (hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
         return fail(PCAGPU_EHIP);
     if ((rc = h->d_task.ensure(1)) || (rc = h->d_items.ensure(64))) return fail(rc);
     if (hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice) != hipSuccess)
-        return fail(PCAGPU_EHIP);
-    if ((rc = h->d_vg_insn.ensure(h->ct.vg_insn.size())) || (rc = h->d_vg_j.ensure(h->ct.vg_j.size())) ||
-        (rc = h->d_vg_rdmask.ensure(h->ct.vg_rdmask.size())) || (rc = h->d_vg_n.ensure(128)) ||
-        (rc = h->d_cand_total.ensure(128)))
-        return fail(rc);
-    if (hipMemcpy(h->d_vg_insn.p, h->ct.vg_insn.data(), h->ct.vg_insn.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->d_vg_j.p, h->ct.vg_j.data(), h->ct.vg_j.size() * 2, hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->d_vg_rdmask.p, h->ct.vg_rdmask.data(), h->ct.vg_rdmask.size(), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->d_vg_n.p, h->ct.vg_n, sizeof(h->ct.vg_n), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(h->d_cand_total.p, h->ct.cand_total, sizeof(h->ct.cand_total), hipMemcpyHostToDevice) != hipSuccess)
         return fail(PCAGPU_EHIP);
     *out = h;
     return 0;
@@ -327,15 +307,8 @@ void pcagpu_close(pcagpu_t *h)
     if (!h) return;
     (void)hipSetDevice(h->device);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    h->d_lists.release();
-    h->d_count.release();
-    h->d_split.release();
-    h->d_vg_insn.release();
-    h->d_vg_j.release();
-    h->d_vg_rdmask.release();
-    h->d_vg_n.release();
-    h->d_cand_total.release();
     h->d_batch.release();
+    h->d_blocks.release();
     h->d_groups.release();
     h->d_goff.release();
     h->d_task.release();
@@ -356,6 +329,7 @@ void pcagpu_close(pcagpu_t *h)
     h->d_match.release();
     h->d_match_count.release();
     if (h->h_batch) (void)hipHostFree(h->h_batch);
+    if (h->h_blocks) (void)hipHostFree(h->h_blocks);
     if (h->h_groups) (void)hipHostFree(h->h_groups);
     if (h->h_goff) (void)hipHostFree(h->h_goff);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -594,8 +568,13 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
         tf[2 * sl] = k;
         tf[2 * sl + 1] = q;
     }
-    if ((rc = h->d_q0.ensure(cap0)) || (rc = h->d_qf.ensure(2 * capf)) || (rc = h->d_match_count.ensure(1)))
+    if ((rc = h->d_q0.ensure(cap0)) || (rc = h->d_qf.ensure(2 * capf)) || (rc = h->d_match_count.ensure(1)) ||
+        (rc = h->d_groups.ensure(kGroupMax)) || (rc = h->d_goff.ensure(kGroupMax + 1)))
         return rc;
+    if (!h->h_groups && hipHostMalloc((void **)&h->h_groups, (size_t)kGroupMax * sizeof(ScanGroup)) != hipSuccess)
+        return PCAGPU_ENOMEM;
+    if (!h->h_goff && hipHostMalloc((void **)&h->h_goff, ((size_t)kGroupMax + 1) * sizeof(uint64_t)) != hipSuccess)
+        return PCAGPU_ENOMEM;
     if (h->d_match.n == 0 && (rc = h->d_match.ensure(1 << 16))) return rc;
     HIP_TRY(hipMemcpy(h->d_q0.p, t0.data(), cap0 * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(h->d_qf.p, tf.data(), 2 * capf * 8, hipMemcpyHostToDevice));

@@ -51,16 +51,20 @@ inline double now_s()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
-constexpr uint32_t kGroupMax = 1u << 20;   // scan groups per launch
+constexpr uint32_t kGroupMax = 1u << 20;   // match-pass groups per launch
+constexpr uint32_t kBlockMax = 1u << 21;   // scan blocks per launch
 
 }  // namespace pca
 
+using pca::CandDev;
 using pca::CandTable;
+using pca::GrammarDev;
 using pca::DevBuf;
 using pca::DevTask;
 using pca::HostHit;
 using pca::HitRec;
 using pca::MatchRec;
+using pca::ScanBlock;
 using pca::ScanGroup;
 using pca::ScanItem;
 
@@ -68,24 +72,21 @@ struct pcagpu {
     pca_task_t task;
     int T = 0;
     DevTask dt;
-    CandTable ct;
+    const CandTable &ct = pca::shared_grammar();   // immutable, shared by all handles
     int device = 0;
     int cu_count = 256;
     std::string devname;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     pcagpu_opts_t opts;
-    uint32_t waves_per_cu = 20;          // resident waves per CU, one-item scan launches
-    uint32_t waves_per_cu_multi = 16;    // the same for multi-item launches
+    uint32_t waves_per_cu = 0;           // opts.waves_per_cu, or 0: from the kernel's occupancy
+    // resident scan waves per CU by [multi-item][state levels], 0 = not yet asked
+    uint32_t scan_wpc[2][pca::MAXLEV + 1] = {};
 
-    DevBuf<uint16_t> d_lists;
-    DevBuf<int> d_count, d_split;
-    DevBuf<uint16_t> d_vg_insn, d_vg_j;
-    DevBuf<uint8_t> d_vg_rdmask;
-    DevBuf<int> d_vg_n;
-    DevBuf<uint32_t> d_cand_total;
+    const GrammarDev *d_grammar = nullptr;   // per device, shared, never freed
     DevBuf<pca_prefix_t> d_batch;
-    DevBuf<ScanGroup> d_groups;
+    DevBuf<ScanBlock> d_blocks;
+    DevBuf<ScanGroup> d_groups;      // match pass only, allocated on first use
     DevBuf<uint64_t> d_goff;
     DevBuf<DevTask> d_task;          // the task record, resident for the kernels
     DevBuf<ScanItem> d_items;
@@ -93,6 +94,7 @@ struct pcagpu {
     DevBuf<uint32_t> d_hit_count;
     DevBuf<unsigned long long> d_counters, d_work;
     pca_prefix_t *h_batch = nullptr;
+    ScanBlock *h_blocks = nullptr;
     ScanGroup *h_groups = nullptr;
     uint64_t *h_goff = nullptr;
     uint32_t batch_cap = 0;
@@ -124,13 +126,20 @@ struct pcagpu {
 
 namespace pca {
 int fill_devtask(pcagpu *h);
+// the grammar tables of the current device, uploaded on first use and kept
+// until the process ends (read-only, shared by every handle on the device)
+const GrammarDev *shared_grammar_dev(int device);
 CandDev cand_dev(const pcagpu *h);
-// persistent grid for `units` work units (multi: a launch of several items)
-int grid_for(pcagpu *h, uint64_t units, bool multi = false);
+// persistent grid for `units` work units of a scan launch (multi: several
+// items; levels: state levels per wave), or of the other persistent kernels
+// (levels 0: the one-item scan figure at 3 levels)
+int grid_for(pcagpu *h, uint64_t units, bool multi = false, uint32_t levels = 0);
 size_t handle_device_bytes(const pcagpu *h);
 // queue prefixes for the depth without flushing (the caller scans them)
 int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank);
-// group table of the staged batch into h_groups / h_goff; returns the group
-// count (0 for k <= 1) and the unit count, or an error if it does not fit
-int build_groups(pcagpu *h, uint32_t *n_groups, uint64_t *n_units);
+// scan blocks of the staged batch into h_blocks (k >= 2); returns the block
+// count and the unit count, or PCAGPU_EOVERFLOW if they do not fit one launch
+int build_blocks(pcagpu *h, uint32_t *n_blocks, uint64_t *n_units);
+// state levels per wave a scan at k suffix positions keeps in LDS
+inline uint32_t scan_levels(int k) { return k < 1 ? 1u : (uint32_t)k; }
 }  // namespace pca

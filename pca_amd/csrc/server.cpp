@@ -312,31 +312,29 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     // ---- every active task stages its next batch
     s->items.clear();
     uint64_t units = 0;
+    uint32_t levels = 1;
     std::vector<Slot *> in_launch;
     for (Slot &sl : s->slots) {
         if (!sl.busy || sl.next >= sl.frontier.size()) continue;
         pcagpu *h = sl.h;
         const uint32_t n = (uint32_t)std::min<size_t>(sl.frontier.size() - sl.next, h->batch_cap);
         if ((rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return rc;
-        uint32_t ng = 0;
+        uint32_t nb = 0;
         uint64_t nu = 0;
-        if ((rc = build_groups(h, &ng, &nu)) < 0) return rc;
+        if ((rc = build_blocks(h, &nb, &nu)) < 0) return rc;
         HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
                                s->stream));
-        if (ng) {
-            HIP_TRY(hipMemcpyAsync(h->d_groups.p, h->h_groups, (size_t)ng * sizeof(ScanGroup),
+        if (nb)
+            HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)nb * sizeof(ScanBlock),
                                    hipMemcpyHostToDevice, s->stream));
-            HIP_TRY(hipMemcpyAsync(h->d_goff.p, h->h_goff, ((size_t)ng + 1) * 8, hipMemcpyHostToDevice, s->stream));
-        }
+        levels = std::max(levels, scan_levels(h->k));
         HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), s->stream));
         ScanItem it;
         std::memset(&it, 0, sizeof(it));
         it.task = h->d_task.p;
         it.prefixes = h->d_batch.p;
-        it.groups = h->d_groups.p;
-        it.group_off = h->d_goff.p;
+        it.blocks = h->d_blocks.p;
         it.n_prefix = n;
-        it.n_groups = ng;
         it.unit_begin = units;
         it.n_units = nu;
         it.D = h->D;
@@ -361,12 +359,14 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         ScanLaunch l;
         l.items = s->d_items.p;
         l.n_items = (uint32_t)s->items.size();
+        l.levels = levels;
         l.n_units = units;
         l.cand = cand_dev(in_launch[0]->h);
         l.counters = s->d_counters.p;
         l.work = s->d_work.p;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
-        launch_scan(l, s->items.data(), in_launch[0]->h->dt, grid_for(in_launch[0]->h, units, s->items.size() > 1), s->stream);
+        launch_scan(l, s->items.data(), in_launch[0]->h->dt,
+                    grid_for(in_launch[0]->h, units, s->items.size() > 1, levels), s->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(s->ev1, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
