@@ -102,7 +102,10 @@ typedef struct {
     uint32_t batch_capacity;  /* prefixes per flush (default 1<<20)           */
     uint32_t hit_capacity;    /* device hit records per depth (default 1<<22) */
     uint32_t waves_per_cu;    /* persistent scan occupancy (0 = autotuned)    */
-    uint32_t reserved[4];
+    uint32_t fixed_test_order;/* 1: keep the initial device test order; 0:
+                                 reorder tests 2.. after every scan launch by
+                                 how often each ends a surviving candidate   */
+    uint32_t reserved[3];
 } pcagpu_opts_t;
 
 typedef struct pcagpu pcagpu_t;

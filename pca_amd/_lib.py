@@ -134,7 +134,7 @@ class Stats(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("batch_capacity", C.c_uint32), ("hit_capacity", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("waves_per_cu", C.c_uint32), ("fixed_test_order", C.c_uint32), ("reserved", C.c_uint32 * 3)]
 
 
 class SolveOpts(C.Structure):

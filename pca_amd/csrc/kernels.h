@@ -179,6 +179,9 @@ struct ScanLaunch {
     CandDev cand;
     unsigned long long *counters;    // [0] candidates [1] suffixes [2] tests [3] parents
     unsigned long long *work;
+    // one-item launches: per device test position, [2t] candidates that
+    // reached it in the survivor phase and [2t+1] those it ended (null: off)
+    unsigned long long *tstat;
 };
 
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);

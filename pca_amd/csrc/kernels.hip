@@ -242,7 +242,7 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const TestState *P,
                                                  uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
                                                  int lane, WaveCounters &wc, const uint8_t *regmatch,
-                                                 uint32_t *queue, const ParentSummary &ps)
+                                                 uint32_t *queue, uint32_t (*tst)[2], const ParentSummary &ps)
 {
     const int T = task.T;
     const int m = (live >> 1) & 127;
@@ -343,7 +343,8 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         for (int tt = t0; tt < T; tt++) {
             const uint64_t al = __ballot(ok != 0u);
             if (al == 0ull) break;
-            n_ev += (uint32_t)__popcll(al);
+            const uint32_t before = (uint32_t)__popcll(al);
+            n_ev += before;
             if ((ps.skipm >> tt) & 1ull) {
                 ok &= uni(regmatch[tt]);
                 continue;
@@ -376,6 +377,13 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
                 }
             }
             if (!match) ok = 0;
+            if (tst) {
+                const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
+                if (lane == 0) {
+                    tst[tt][0] += before;
+                    tst[tt][1] += before - after;
+                }
+            }
         }
         if (lane == 0) wc.tests += n_ev;
         if (ok != 0u) {
@@ -600,7 +608,10 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
     __shared__ uint8_t s_rm[SCAN_WAVES][MAXT];
     __shared__ int s_cnt[128], s_split[128];
     __shared__ uint32_t s_ctot[128];
+    __shared__ uint32_t s_tst[SCAN_WAVES][MAXT][2];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int i = threadIdx.x; i < SCAN_WAVES * MAXT * 2; i += SCAN_WG) (&s_tst[0][0][0])[i] = 0u;
+    uint32_t (*tst)[2] = (!MULTI && lch.tstat) ? s_tst[w] : nullptr;
     // per-mask tables live in LDS: no global round trip per parent
     for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
         s_cnt[i] = lch.cand.count[i];
@@ -740,7 +751,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                         wc.parents += lane == 0;
                         wave_sync();
                         if (ps.pm == 0u)
-                            scan_parent_bits(task, a, PP, lvL, pi, jpath, lane, wc, s_rm[w], s_queue[w], ps);
+                            scan_parent_bits(task, a, PP, lvL, pi, jpath, lane, wc, s_rm[w], s_queue[w], tst, ps);
                         else
                             scan_parent(task, a, PP, lvL, pi, jpath, lane, wc, s_queue[w], ps);
                     }
@@ -781,6 +792,14 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
         atomicAdd(&lch.counters[1], c1);
         atomicAdd(&lch.counters[2], c2);
         atomicAdd(&lch.counters[3], c3);
+    }
+    if (!MULTI && lch.tstat) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < MAXT * 2; i += SCAN_WG) {
+            uint32_t v = 0;
+            for (int ww = 0; ww < SCAN_WAVES; ww++) v += (&s_tst[ww][0][0])[i];
+            if (v) atomicAdd(&lch.tstat[i], (unsigned long long)v);
+        }
     }
 }
 

@@ -29,23 +29,52 @@ namespace pca {
 
 static_assert(NBKINDS == NBK && VGMAX == VG_MAX && LMAX == CAND_LMAX, "host and device grammar tables differ");
 
+void set_test_order(pcagpu *h, const std::vector<int> &order)
+{
+    for (int t = 0; t < h->T; t++) {
+        h->dt.tgt[t] = h->task.tests[order[t]].out[0];
+        h->dt.tgtw[t] = h->dt.tgt[t];
+        for (int i = 0; i < 12; i++)
+            if (kLdiImms[i] == h->dt.tgt[t]) h->dt.tgtw[t] |= 1u << (16 + i);
+        h->dt.root[t] = h->roots[order[t]];
+    }
+    h->order = order;
+}
+
 int fill_devtask(pcagpu *h)
 {
     std::memset(&h->dt, 0, sizeof(h->dt));
     h->dt.T = h->T;
     h->dt.n_in = h->task.num_inputs;
     h->dt.max_cycles = (int)std::min<uint64_t>(h->task.max_cycles, 255);
-    TestState roots[PCAGPU_MAX_TESTS];
-    root_states(&h->task, roots);
-    const std::vector<int> order = device_test_order(&h->task);
+    root_states(&h->task, h->roots);
+    std::memset(h->t_reach, 0, sizeof(h->t_reach));
+    std::memset(h->t_end, 0, sizeof(h->t_end));
+    set_test_order(h, device_test_order(&h->task));
+    return 0;
+}
+
+// Adaptive test order.  A launch reports, per device position, how many
+// candidates still alive after the first two tests reached it and how many
+// it ended; accumulated per task test, tests 2.. are re-sorted by that end
+// rate, highest first (a test not reached yet counts 1/2), so that the
+// tests which actually separate this task's near-misses run early.  The
+// first two positions stay: the per-op masks run there for every group.
+// The order decides only when a candidate stops being tested, never which
+// candidates pass.
+int adapt_test_order(pcagpu *h, const unsigned long long *ts)
+{
     for (int t = 0; t < h->T; t++) {
-        h->dt.tgt[t] = h->task.tests[order[t]].out[0];
-        h->dt.tgtw[t] = h->dt.tgt[t];
-        for (int i = 0; i < 12; i++)
-            if (kLdiImms[i] == h->dt.tgt[t]) h->dt.tgtw[t] |= 1u << (16 + i);
-        h->dt.root[t] = roots[order[t]];
+        h->t_reach[h->order[t]] += ts[2 * t];
+        h->t_end[h->order[t]] += ts[2 * t + 1];
     }
-    h->order = order;
+    if (h->T <= 3) return 0;
+    std::vector<int> ord = h->order;
+    auto rate = [&](int x) { return h->t_reach[x] ? (double)h->t_end[x] / (double)h->t_reach[x] : 0.5; };
+    std::stable_sort(ord.begin() + 2, ord.end(), [&](int x, int y) { return rate(x) > rate(y); });
+    if (ord == h->order) return 0;
+    set_test_order(h, ord);
+    HIP_TRY(hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -133,7 +162,10 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
     l.cand = cand_dev(h);
     l.counters = h->d_counters.p;
     l.work = h->d_work.p;
+    const bool adapt = !h->opts.fixed_test_order;
+    l.tstat = adapt ? h->d_tstat.p : nullptr;
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
+    if (adapt) HIP_TRY(hipMemsetAsync(h->d_tstat.p, 0, 2 * MAXT * sizeof(unsigned long long), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     launch_scan(l, &it, h->dt, grid_for(h, units, false, l.levels), h->stream);
     HIP_TRY(hipGetLastError());
@@ -143,6 +175,11 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);
     h->stats.launches++;
+    if (adapt) {
+        unsigned long long ts[2 * MAXT];
+        HIP_TRY(hipMemcpy(ts, h->d_tstat.p, sizeof(ts), hipMemcpyDeviceToHost));
+        return adapt_test_order(h, ts);
+    }
     return 0;
 }
 
@@ -193,7 +230,7 @@ int scan_batch(pcagpu *h, uint32_t n)
 size_t handle_device_bytes(const pcagpu *h)
 {
     return h->d_batch.bytes() + h->d_blocks.bytes() + h->d_groups.bytes() + h->d_goff.bytes() + h->d_task.bytes() + h->d_items.bytes() +
-           h->d_hits.bytes() + h->d_hit_count.bytes() + h->d_counters.bytes() + h->d_work.bytes() +
+           h->d_hits.bytes() + h->d_hit_count.bytes() + h->d_counters.bytes() + h->d_work.bytes() + h->d_tstat.bytes() +
            h->d_front[0].bytes() + h->d_front[1].bytes() + h->d_child_off.bytes() + h->d_keep_cnt.bytes() +
            h->d_scan_tmp.bytes() + h->d_table.bytes() + h->d_keep_bits.bytes() + h->d_q0.bytes() + h->d_qf.bytes() +
            h->d_match.bytes() + h->d_match_count.bytes();
@@ -288,7 +325,8 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     if ((rc = h->d_batch.ensure(h->batch_cap)) ||
         (rc = h->d_blocks.ensure(kBlockMax)) ||
         (rc = h->d_hits.ensure(h->hit_cap)) ||
-        (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(8)) || (rc = h->d_work.ensure(1)))
+        (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(8)) || (rc = h->d_work.ensure(1)) ||
+        (rc = h->d_tstat.ensure(2 * MAXT)))
         return fail(rc);
     if (hipHostMalloc((void **)&h->h_batch, (size_t)h->batch_cap * sizeof(pca_prefix_t)) != hipSuccess ||
         hipHostMalloc((void **)&h->h_blocks, (size_t)kBlockMax * sizeof(ScanBlock)) != hipSuccess)
@@ -316,6 +354,7 @@ void pcagpu_close(pcagpu_t *h)
     h->d_hits.release();
     h->d_hit_count.release();
     h->d_counters.release();
+    h->d_tstat.release();
     h->d_work.release();
     h->d_front[0].release();
     h->d_front[1].release();

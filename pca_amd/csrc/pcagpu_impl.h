@@ -93,6 +93,9 @@ struct pcagpu {
     DevBuf<HitRec> d_hits;
     DevBuf<uint32_t> d_hit_count;
     DevBuf<unsigned long long> d_counters, d_work;
+    DevBuf<unsigned long long> d_tstat;   // [2 * MAXT] survivor-phase reach / end counts of a launch
+    pca::TestState roots[PCAGPU_MAX_TESTS];   // per test (task order) state after the IN scaffold
+    uint64_t t_reach[PCAGPU_MAX_TESTS] = {}, t_end[PCAGPU_MAX_TESTS] = {};   // by task test index
     pca_prefix_t *h_batch = nullptr;
     ScanBlock *h_blocks = nullptr;
     ScanGroup *h_groups = nullptr;
@@ -126,6 +129,8 @@ struct pcagpu {
 
 namespace pca {
 int fill_devtask(pcagpu *h);
+// device test order: h->dt's per-test fields and h->order from `order`
+void set_test_order(pcagpu *h, const std::vector<int> &order);
 // the grammar tables of the current device, uploaded on first use and kept
 // until the process ends (read-only, shared by every handle on the device)
 const GrammarDev *shared_grammar_dev(int device);

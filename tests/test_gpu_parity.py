@@ -198,6 +198,26 @@ def test_solve_any_split_matches_exact_fixture(pca, name, maxd, split):
     assert [s[0] for s in res.solutions] == want["solutions"]
 
 
+@pytest.mark.parametrize("name,maxd,split", [("abs", 3, 1), ("is_power2_32", 2, 1), ("sign", 3, 1)])
+def test_adaptive_test_order_keeps_results(pca, name, maxd, split):
+    """Small batches make many scan launches, and the device test order is
+    re-sorted after each one: the solution set must equal the fixed-order
+    run and the exact fixture."""
+    want = _exact_fixture(name, maxd)
+    t = pca.Task.load(golden_task_path(name))
+    print(f"[adaptive-order] {name} d{maxd} split={split}", flush=True)
+    got = {}
+    for fixed in (True, False):
+        with pca.Solver(t, batch_capacity=4, fixed_test_order=fixed) as sv:
+            res = sv.solve(maxd, split=split, cap=1 << 16)
+            got[fixed] = ([s[0] for s in res.solutions], res.optimal_depth, res.num_solutions,
+                          sv.stats()["launches"], sv.stats()["tests_evaluated"])
+    print("fixed", got[True][3:], "adaptive", got[False][3:], flush=True)
+    assert got[True][:3] == got[False][:3]
+    assert got[False][:3] == (want["solutions"], want["optimal_depth"], want["num_solutions"])
+    assert got[False][3] > 4
+
+
 @pytest.mark.parametrize("opname", ["ADD", "SUB", "MUL", "DIV", "MOD", "AND", "OR", "XOR", "SHL", "SHR",
                                     "ASR", "CMP"])
 def test_device_alu_exhaustive_matches_oracle(pca, opname):
