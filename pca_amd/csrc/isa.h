@@ -136,7 +136,7 @@ PCA_HD uint32_t alu_eval(uint32_t op, uint32_t a, uint32_t b, uint32_t imm, uint
 
 // One test's state on the straight line of a kernel.  `resume` is the kernel
 // position where a taken forward BR lands; positions below it are skipped.
-struct TestState {
+struct alignas(4) TestState {
     uint16_t r[8];
     uint8_t flags;
     uint8_t resume;
@@ -168,6 +168,59 @@ PCA_HD void step_test(TestState &s, uint32_t insn, uint32_t pos)
     s.r[rd] = (uint16_t)v;
     s.flags = (uint8_t)f;
     s.r[0] = 0;
+}
+
+// The same state as five 32-bit words in the TestState byte layout: w[0..3]
+// hold r0..r7 (r[2i] in the low half of w[i]), w[4] = flags | resume << 8 |
+// frozen << 16 | exec << 24.  Kernels keep it in registers and step it with
+// the instruction (and so its register fields) uniform across the wave;
+// step_packed is step_test on this form.
+struct PackedState {
+    uint32_t w[5];
+};
+
+PCA_HD uint32_t ps_reg(const PackedState &s, uint32_t k)
+{
+    const uint32_t d = k >> 1;
+    const uint32_t v = d == 0 ? s.w[0] : (d == 1 ? s.w[1] : (d == 2 ? s.w[2] : s.w[3]));
+    return (k & 1u) ? v >> 16 : v & 0xFFFFu;
+}
+
+PCA_HD void ps_set_reg(PackedState &s, uint32_t k, uint32_t v)
+{
+    const uint32_t sh = (k & 1u) * 16u, keep = ~(0xFFFFu << sh), d = k >> 1;
+    for (uint32_t i = 0; i < 4; i++)
+        if (d == i) s.w[i] = (s.w[i] & keep) | ((v & 0xFFFFu) << sh);
+}
+
+PCA_HD void step_packed(PackedState &s, uint32_t insn, uint32_t pos)
+{
+    const uint32_t w4 = s.w[4];
+    const uint32_t flags = w4 & 0xFFu, resume = (w4 >> 8) & 0xFFu, frozen = (w4 >> 16) & 0xFFu;
+    if (frozen || resume > pos) return;
+    uint32_t nw4 = w4 + (1u << 24);   // exec++ (a byte: wraps like the uint8_t field)
+    const uint32_t op = insn_op(insn), rd = insn_rd(insn), rs = insn_rs(insn), rt = insn_rt(insn);
+    if (op == OP_BR) {
+        if (cond_taken(rd, flags))
+            nw4 = (nw4 & ~0xFF00u) | (((pos + 1u + (uint32_t)insn_simm(insn)) & 0xFFu) << 8);
+        s.w[4] = nw4;
+        return;
+    }
+    if (op == OP_CMP) {
+        s.w[4] = (nw4 & ~0xFFu) | flags_sub(ps_reg(s, rd), ps_reg(s, rs));
+        return;
+    }
+    const uint32_t a = ps_reg(s, op == OP_ADDI ? rd : rs), b = ps_reg(s, rt);
+    uint32_t f;
+    bool fault;
+    const uint32_t v = alu_eval(op, a, b, insn_imm(insn), flags, &f, &fault);
+    if (fault) {
+        s.w[4] = nw4 | (1u << 16);
+        return;
+    }
+    ps_set_reg(s, rd, v);
+    s.w[0] &= 0xFFFF0000u;   // r0 stays 0
+    s.w[4] = (nw4 & ~0xFFu) | (f & 0xFFu);
 }
 
 // Live mask after an instruction (tools/enumerate.c:427-433).

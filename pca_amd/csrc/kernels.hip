@@ -61,23 +61,42 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
 // Replays a prefix from the root into an LDS level (lanes = tests).  The
 // state is stepped in place in LDS so that the register index chosen by the
 // instruction addresses LDS, not a private array.
+__device__ __forceinline__ PackedState load_ps(const TestState *t)
+{
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(t);
+    PackedState s;
+#pragma unroll
+    for (int i = 0; i < 5; i++) s.w[i] = q[i];
+    return s;
+}
+
+__device__ __forceinline__ void store_ps(TestState *t, const PackedState &s)
+{
+    uint32_t *q = reinterpret_cast<uint32_t *>(t);
+#pragma unroll
+    for (int i = 0; i < 5; i++) q[i] = s.w[i];
+}
+
+// Replays a prefix from the root into an LDS level (lanes = tests), the
+// state held in registers (PackedState) while it steps.
 __device__ __forceinline__ void replay_prefix(const DevTask &task, const pca_prefix_t *p, int n,
                                               TestState *dst, int lane)
 {
     if (lane < task.T) {
-        TestState &st = dst[lane];
-        st = task.root[lane];
-        for (int q = 0; q < n; q++) step_test(st, p->insns[q], (uint32_t)q);
+        PackedState st = load_ps(&task.root[lane]);
+        for (int q = 0; q < n; q++) step_packed(st, p->insns[q], (uint32_t)q);
+        store_ps(&dst[lane], st);
     }
 }
 
+// One level down (lanes = tests): dst = src stepped by the wave-uniform insn.
 __device__ __forceinline__ void step_level(const DevTask &task, const TestState *src, TestState *dst,
                                            uint32_t insn, int pos, int lane)
 {
     if (lane < task.T) {
-        TestState &st = dst[lane];
-        st = src[lane];
-        step_test(st, insn, (uint32_t)pos);
+        PackedState st = load_ps(&src[lane]);
+        step_packed(st, insn, (uint32_t)pos);
+        store_ps(&dst[lane], st);
     }
 }
 
@@ -166,13 +185,9 @@ __device__ __forceinline__ void pair_eval(const DevTask &task, const TestState *
     rd_ok = rd_ok && v[0] == task.tgt[0] && v[1] == task.tgt[1];
 }
 
-// Scan every last-position candidate below one parent state P (kernel
-// position D-1).  Emits hits whose state has a target-matching register.
-//
-// Stage A runs tests 0-1 on all 64 lanes of a chunk; the few candidates
-// still alive are pushed (ballot prefix count) to a per-wave LDS queue, and
-// stage B runs the full early-exit test loop 64 queued survivors at a time,
-// so lanes that died on test 0 do not idle through the slow lanes' tests.
+// What the last-position scan below one parent state P (kernel position
+// D-1) needs to know about P; hits are last-position candidates whose state
+// has a target-matching register.
 struct ParentSummary {
     uint32_t pm;       // registers r1..r7 equal to the target on every test
     uint64_t skipm;    // tests on which the last instruction does not execute
@@ -181,28 +196,33 @@ struct ParentSummary {
 
 // Lanes = tests.  Also leaves per test the mask of registers equal to the
 // target (regmatch) in LDS for the value-group scan.
+// my_tgt: this lane's target (lane = test), held in a register by the caller.
 __device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, const TestState *P, uint32_t pos,
-                                                          int lane, uint8_t *regmatch)
+                                                          int lane, uint32_t my_tgt, uint8_t *regmatch)
 {
     const int T = task.T;
     const bool act = lane < T;
-    TestState st;
-    if (act) st = P[lane];
-    const uint32_t tgt = act ? task.tgt[lane] : 0u;
+    PackedState st = {};
+    if (act) st = load_ps(&P[lane]);
+    // each register pair against the target, both halves at once
+    const uint32_t tt = my_tgt | (my_tgt << 16);
     ParentSummary s;
     uint32_t pm = 0, rm = 0;
+#pragma unroll
     for (int r = 1; r < 8; r++) {
-        const bool eq = act && st.r[r] == tgt;
+        const uint32_t d = st.w[r >> 1] ^ tt;
+        const bool eq = act && ((r & 1) ? (d >> 16) == 0u : (d & 0xFFFFu) == 0u);
         rm |= (eq ? 1u : 0u) << r;
         pm |= (__ballot(act && !eq) == 0ull ? 1u : 0u) << r;
     }
     if (act) regmatch[lane] = (uint8_t)rm;
     s.pm = pm;
     // skipped by a pending BR, or the VM already stopped
-    const bool noexec = act && (st.frozen || st.resume > pos);
+    const uint32_t frozen = (st.w[4] >> 16) & 0xFFu, resume = (st.w[4] >> 8) & 0xFFu, exec = st.w[4] >> 24;
+    const bool noexec = act && (frozen || resume > pos);
     s.skipm = __ballot(noexec);
-    const bool cyc = act && ((uint32_t)st.exec + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
-    s.dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
+    const bool cyc = act && (exec + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
+    s.dead = __ballot(act && (frozen || cyc)) != 0ull ? 1u : 0u;
     return s;
 }
 
@@ -637,8 +657,10 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
     a.counters = lch.counters;
     a.work = lch.work;
     uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
+    uint32_t my_tgt = 0;                                   // lane = test: its target
     if constexpr (!MULTI) {
         taskp = &task1;
+        my_tgt = lane < task1.T ? task1.tgt[lane] : 0u;
         a.prefixes = one.prefixes;
         a.n_prefix = one.n_prefix;
         a.blocks = one.blocks;
@@ -680,6 +702,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
             it_end = it.unit_begin + it.n_units;
             cur_i = 0xFFFFFFFFu;
             cur_js = 0xFFFFFFFFu;
+            my_tgt = lane < taskp->T ? taskp->tgt[lane] : 0u;
         }
         const DevTask &task = *taskp;
         const int k = a.k;
@@ -747,7 +770,8 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                     {
                         const TestState *PP = lev(L);
                         const uint32_t lvL = uni(s_live[w][L]);
-                        const ParentSummary ps = summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, s_rm[w]);
+                        const ParentSummary ps =
+                            summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt, s_rm[w]);
                         wc.parents += lane == 0;
                         wave_sync();
                         if (ps.pm == 0u)

@@ -260,3 +260,15 @@ def test_shallow_task_redraws_constant_zero_divisor(pca):
         assert 8 <= t.num_tests <= 32
     a, b = taskgen.shallow_task(17, 12), taskgen.shallow_task(17, 12)
     assert bytes(a) == bytes(b)
+
+
+def test_packed_step_matches_step_test(tmp_path):
+    """The kernels step test states in a packed register form; it must equal
+    the reference-order step (isa.h step_test) byte for byte on random
+    states for every grammar instruction at every position."""
+    exe = tmp_path / "step_check"
+    src = os.path.join(ROOT, "tests", "native", "step_check.cpp")
+    subprocess.run(["g++", "-O2", "-std=c++17", src, "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok ")
