@@ -196,14 +196,11 @@ struct ParentSummary {
 
 // Lanes = tests.  Also leaves per test the mask of registers equal to the
 // target (regmatch) in LDS for the value-group scan.
-// my_tgt: this lane's target (lane = test), held in a register by the caller.
-__device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, const TestState *P, uint32_t pos,
-                                                          int lane, uint32_t my_tgt, uint8_t *regmatch)
+// my_tgt: this lane's target (lane = test), held in a register by the caller;
+// st: this lane's parent state (valid when act).
+__device__ __forceinline__ ParentSummary summarize_state(const DevTask &task, const PackedState &st, bool act,
+                                                         uint32_t pos, int lane, uint32_t my_tgt, uint8_t *regmatch)
 {
-    const int T = task.T;
-    const bool act = lane < T;
-    PackedState st = {};
-    if (act) st = load_ps(&P[lane]);
     // each register pair against the target, both halves at once
     const uint32_t tt = my_tgt | (my_tgt << 16);
     ParentSummary s;
@@ -224,6 +221,15 @@ __device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, c
     const bool cyc = act && (exec + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
     s.dead = __ballot(act && (frozen || cyc)) != 0ull ? 1u : 0u;
     return s;
+}
+
+__device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, const TestState *P, uint32_t pos,
+                                                          int lane, uint32_t my_tgt, uint8_t *regmatch)
+{
+    const bool act = lane < task.T;
+    PackedState st = {};
+    if (act) st = load_ps(&P[lane]);
+    return summarize_state(task, st, act, pos, lane, my_tgt, regmatch);
 }
 
 // jpath: the wave's LDS row of candidate indices at positions s..D-2.
@@ -259,6 +265,9 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 // DIV and MOD keep a per-lane rd mask, since a zero divisor (a fault) makes
 // a test behave like that for that lane only.  `a.cand.bits_init` points at
 // the kernel's LDS copy of the per-mask group bits.
+// tests run in phase A (wave masks) before the survivors go to the queue
+constexpr int SCAN_MASK_TESTS = 2;
+
 __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const TestState *P,
                                                  uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
                                                  int lane, WaveCounters &wc, const uint8_t *regmatch,
@@ -284,23 +293,22 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     const uint32_t addi_imm = (rb & 1u) ? (rb >> 1) + 1u : 0x10000u - ((rb >> 1) + 1u);
     uint32_t evals = 0;
     // phase A: the first tests by masks, one evaluation per op for all groups
-    const int TA = T < 2 ? T : 2;
+    const int TA = T < SCAN_MASK_TESTS ? T : SCAN_MASK_TESTS;
     int t = 0;
     for (; t < TA; t++) {
         const uint64_t dvm = __ballot(okd != 0u), mdm = __ballot(okm != 0u);
         uint64_t any = dvm | mdm;
+        uint32_t alive = (uint32_t)__popcll(dvm) + (uint32_t)__popcll(mdm);
 #pragma unroll
-        for (int q = 0; q < NBKINDS; q++) any |= mk[q];
+        for (int q = 0; q < NBKINDS; q++) {
+            any |= mk[q];
+            alive += (uint32_t)__popcll(mk[q]);
+        }
         if (any == 0ull) break;
         // group evaluations of this test; once a skipped test has narrowed
         // the rd values (rm_and), groups left without one are not told
         // apart here, so such parents are not counted further
-        if (rm_and == 0xFFu) {
-            uint32_t alive = (uint32_t)__popcll(dvm) + (uint32_t)__popcll(mdm);
-#pragma unroll
-            for (int q = 0; q < NBKINDS; q++) alive += (uint32_t)__popcll(mk[q]);
-            evals += alive;
-        }
+        if (rm_and == 0xFFu) evals += alive;
         if ((ps.skipm >> t) & 1ull) {
             const uint32_t rmt = uni(regmatch[t]);
             okd &= rmt;
@@ -335,6 +343,15 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
                 const int32_t qt = sx / sy;
                 if (((uint32_t)qt & 0xFFFFu) != tg) okd = 0;
                 if (((uint32_t)(sx - qt * sy) & 0xFFFFu) != tg) okm = 0;
+            }
+        }
+        if (tst && t >= 2) {   // positions 2.. are reordered by these counts
+            uint32_t after = (uint32_t)__popcll(__ballot(okd != 0u)) + (uint32_t)__popcll(__ballot(okm != 0u));
+#pragma unroll
+            for (int q = 0; q < NBKINDS; q++) after += (uint32_t)__popcll(mk[q]);
+            if (lane == 0) {
+                tst[t][0] += alive;
+                tst[t][1] += alive - after;
             }
         }
     }
@@ -752,9 +769,20 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
             cw = (uint32_t)lane < nb ? cand_at(a.cand, lvp, mfp, (int)j0 + lane) : 0u;
         }
         for (uint32_t ci = 0; ci < nb; ci++) {
+            // when the child is itself the scanned parent it is stepped and
+            // summarised from registers (stored once, for the scan)
+            ParentSummary ps_child = {};
+            const bool fused = k >= 2 && Lu == k - 1;
             if (k >= 2) {
                 const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
-                step_level(task, lev(Lu - 1), lev(Lu), c, a.s + Lu - 1, lane);
+                const bool act = lane < task.T;
+                PackedState st = {};
+                if (act) {
+                    st = load_ps(&lev(Lu - 1)[lane]);
+                    step_packed(st, c, (uint32_t)(a.s + Lu - 1));
+                    store_ps(&lev(Lu)[lane], st);
+                }
+                if (fused) ps_child = summarize_state(task, st, act, (uint32_t)(a.D - 1), lane, my_tgt, s_rm[w]);
                 if (lane == 0) {
                     s_live[w][Lu] = live_after(lvp, c);
                     s_idx[w][Lu] = j0 + ci;
@@ -771,7 +799,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                         const TestState *PP = lev(L);
                         const uint32_t lvL = uni(s_live[w][L]);
                         const ParentSummary ps =
-                            summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt, s_rm[w]);
+                            fused ? ps_child : summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt, s_rm[w]);
                         wc.parents += lane == 0;
                         wave_sync();
                         if (ps.pm == 0u)
@@ -829,13 +857,14 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
 
 // One-item launches: register budget of 5 waves per SIMD (at most 96
 // VGPRs), measured faster than the 4 waves of the unconstrained allocation
-// despite a few spills.  Multi-item launches keep the unconstrained budget.
+// despite a few spills.  Multi-item launches (more live context per unit):
+// 4 waves per SIMD.
 __global__ __launch_bounds__(SCAN_WG, 5) void scan_kernel_one(ScanLaunch lch, ScanItem one, DevTask task1)
 {
     scan_body<false>(lch, one, task1);
 }
 
-__global__ __launch_bounds__(SCAN_WG) void scan_kernel_multi(ScanLaunch lch, ScanItem one, DevTask task1)
+__global__ __launch_bounds__(SCAN_WG, 4) void scan_kernel_multi(ScanLaunch lch, ScanItem one, DevTask task1)
 {
     scan_body<true>(lch, one, task1);
 }
