@@ -80,8 +80,9 @@ def serving_bench(rates=(10, 50, 200), n_req=200, seed=7):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    # the first launches try the scan tiles (one each) before the winner is kept
+    ap.add_argument("--warmup", type=int, default=6)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--split", type=int, default=5)
@@ -166,7 +167,9 @@ def main():
                      "unit": "Tint-ops/s", "frac": achieved / PEAK_INT_OPS, "traffic": None},
         "kernel": {"name": "scan_kernel", "avg_launch_ms": avg_launch * 1e3, "launches": launches,
                    "candidates": cand, "suffixes": st["suffixes"], "tests_evaluated": st["tests_evaluated"],
-                   "parents": st["parents"], "raw_hits": st["raw_hits"], "kernel_candidates_per_s": cand / kernel_s if kernel_s else None},
+                   "parents": st["parents"], "raw_hits": st["raw_hits"],
+                   "kernel_candidates_per_s": cand / kernel_s if kernel_s else None,
+                   "tile": {"block_kids": st["scan_tile"] & 0xFF, "mask_tests": st["scan_tile"] >> 8}},
     }
     # HBM bytes per launch from the committed rocprofv3 PMC passes of this
     # kernel and workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)

@@ -95,6 +95,8 @@ typedef struct {
     uint64_t expand_ns;       /* wavefront kernels' device time              */
     uint64_t oep_collisions;  /* 64-bit key matches with a different state   */
     uint64_t parents;         /* parent states scanned (last-position lists) */
+    uint64_t scan_tile;       /* tile of the last scan launch: block_kids |
+                                 mask_tests << 8                             */
 } pcagpu_stats_t;
 
 typedef struct {
@@ -105,7 +107,9 @@ typedef struct {
     uint32_t fixed_test_order;/* 1: keep the initial device test order; 0:
                                  reorder tests 2.. after every scan launch by
                                  how often each ends a surviving candidate   */
-    uint32_t reserved[3];
+    uint32_t scan_tile;       /* 0: autotuned; else block_kids (1..64) |
+                                 mask_tests (1..3) << 8, fixed              */
+    uint32_t reserved[2];
 } pcagpu_opts_t;
 
 typedef struct pcagpu pcagpu_t;

@@ -126,7 +126,7 @@ class Solution(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "candidates", "suffixes", "tests_evaluated", "prefixes", "raw_hits", "kernel_ns",
-        "launches", "frontier_nodes", "frontier_kept", "expand_ns", "oep_collisions", "parents")]
+        "launches", "frontier_nodes", "frontier_kept", "expand_ns", "oep_collisions", "parents", "scan_tile")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -134,7 +134,8 @@ class Stats(C.Structure):
 
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("batch_capacity", C.c_uint32), ("hit_capacity", C.c_uint32),
-                ("waves_per_cu", C.c_uint32), ("fixed_test_order", C.c_uint32), ("reserved", C.c_uint32 * 3)]
+                ("waves_per_cu", C.c_uint32), ("fixed_test_order", C.c_uint32), ("scan_tile", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class SolveOpts(C.Structure):

@@ -79,17 +79,25 @@ struct ScanGroup {
     uint16_t pad;
 };
 
-// Work unit of a scan launch for k >= 2 suffix positions: up to 64
-// consecutive children of one node a level above the scanned parents'
-// first level -- children at position s of prefix `prefix` (k == 2), or at
-// position s+1 of the prefix followed by its position-s candidate js
-// (k >= 3).  The block holds children j0 .. min(j0 + 64, count) - 1.
+// Work unit of a scan launch for k >= 2 suffix positions: up to block_kids
+// (<= 64) consecutive children of one node a level above the scanned
+// parents' first level -- children at position s of prefix `prefix`
+// (k == 2), or at position s+1 of the prefix followed by its position-s
+// candidate js (k >= 3).  The block holds children j0 .. min(j0 +
+// block_kids, count) - 1.
 struct ScanBlock {
     uint32_t prefix;
     uint16_t js;
     uint16_t j0;
 };
-constexpr int BLOCK_KIDS = 64;
+constexpr int BLOCK_KIDS_MAX = 64;
+
+// Tile choice of a scan launch (picked by the host autotuner): children per
+// block, and tests run by wave masks before survivors go to the queue.
+struct ScanTile {
+    uint16_t block_kids;   // 1..64
+    uint16_t mask_tests;   // 1..3
+};
 
 struct ScanArgs {
     const pca_prefix_t *prefixes;
@@ -97,6 +105,7 @@ struct ScanArgs {
     const ScanBlock *blocks;     // k >= 2: one unit per block; k <= 1: one unit per prefix
     uint64_t n_units;
     int D, s, k;
+    ScanTile tile;
     CandDev cand;
     HitRec *hits;
     uint32_t hit_cap;
@@ -164,7 +173,8 @@ struct ScanItem {
     uint32_t pad0;
     uint64_t unit_begin;             // first global unit of this item
     uint64_t n_units;
-    int D, s, k, pad;
+    int D, s, k;
+    ScanTile tile;
     HitRec *hits;
     uint32_t *hit_count;
     uint32_t hit_cap;

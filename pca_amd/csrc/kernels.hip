@@ -265,9 +265,6 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 // DIV and MOD keep a per-lane rd mask, since a zero divisor (a fault) makes
 // a test behave like that for that lane only.  `a.cand.bits_init` points at
 // the kernel's LDS copy of the per-mask group bits.
-// tests run in phase A (wave masks) before the survivors go to the queue
-constexpr int SCAN_MASK_TESTS = 2;
-
 __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const TestState *P,
                                                  uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
                                                  int lane, WaveCounters &wc, const uint8_t *regmatch,
@@ -293,7 +290,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     const uint32_t addi_imm = (rb & 1u) ? (rb >> 1) + 1u : 0x10000u - ((rb >> 1) + 1u);
     uint32_t evals = 0;
     // phase A: the first tests by masks, one evaluation per op for all groups
-    const int TA = T < SCAN_MASK_TESTS ? T : SCAN_MASK_TESTS;
+    const int TA = T < (int)a.tile.mask_tests ? T : (int)a.tile.mask_tests;
     int t = 0;
     for (; t < TA; t++) {
         const uint64_t dvm = __ballot(okd != 0u), mdm = __ballot(okm != 0u);
@@ -683,6 +680,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
         a.blocks = one.blocks;
         a.n_units = one.n_units;
         a.D = one.D;
+        a.tile = one.tile;
         a.s = one.s;
         a.k = one.k;
         a.hits = one.hits;
@@ -710,6 +708,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
             a.blocks = it.blocks;
             a.n_units = it.n_units;
             a.D = it.D;
+            a.tile = it.tile;
             a.s = it.s;
             a.k = it.k;
             a.hits = it.hits;
@@ -764,7 +763,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
             }
             const int mfp = a.D - a.s - Lu;   // children at position s + Lu - 1
             const int nc = cand_count(a.cand, lvp, mfp);
-            nb = (int)j0 < nc ? (uint32_t)min(nc - (int)j0, BLOCK_KIDS) : 0u;
+            nb = (int)j0 < nc ? (uint32_t)min(nc - (int)j0, (int)a.tile.block_kids) : 0u;
             // the block's candidate words, one per lane, in one coalesced read
             cw = (uint32_t)lane < nb ? cand_at(a.cand, lvp, mfp, (int)j0 + lane) : 0u;
         }

@@ -134,7 +134,42 @@ int grid_for(pcagpu *h, uint64_t units, bool multi, uint32_t levels)
     return (int)std::max<uint64_t>(wgs, 1);
 }
 
-int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
+// Scan tile autotuner.  The tile (children per block, tests run by wave
+// masks) is chosen per suffix length k: the first launches of that shape
+// each try one candidate on real work, and the one with the least kernel
+// time per parent state scanned is kept for the rest of the handle's life.
+// A launch with too few parents to time does not count as a try.
+constexpr ScanTile kTiles[] = {{64, 2}, {64, 1}, {64, 3}, {32, 2}, {16, 2}};
+constexpr int kNTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
+constexpr uint64_t kTileMinParents = 200000;
+
+ScanTile current_tile(const pcagpu *h, int *cand)
+{
+    *cand = -1;
+    if (h->opts.scan_tile) {
+        const uint32_t bk = h->opts.scan_tile & 0xFFu, mt = (h->opts.scan_tile >> 8) & 0xFFu;
+        return ScanTile{(uint16_t)std::min<uint32_t>(std::max<uint32_t>(bk, 1u), BLOCK_KIDS_MAX),
+                        (uint16_t)std::min<uint32_t>(std::max<uint32_t>(mt, 1u), 3u)};
+    }
+    const int k = std::min(std::max(h->k, 0), (int)MAXLEV);
+    if (h->tile_best[k] >= 0) return kTiles[h->tile_best[k]];
+    *cand = h->tile_tried[k];
+    return kTiles[*cand];
+}
+
+void tile_feedback(pcagpu *h, int cand, float ms, uint64_t parents)
+{
+    const int k = std::min(std::max(h->k, 0), (int)MAXLEV);
+    if (cand < 0 || parents < kTileMinParents) return;
+    h->tile_cost[k][cand] = (double)ms * 1e6 / (double)parents;
+    if (++h->tile_tried[k] < kNTiles) return;
+    int best = 0;
+    for (int i = 1; i < kNTiles; i++)
+        if (h->tile_cost[k][i] < h->tile_cost[k][best]) best = i;
+    h->tile_best[k] = best;
+}
+
+int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile, int cand)
 {
     ScanItem it;
     std::memset(&it, 0, sizeof(it));
@@ -147,9 +182,13 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
     it.D = h->D;
     it.s = h->s;
     it.k = h->k;
+    it.tile = tile;
+    h->stats.scan_tile = (uint64_t)it.tile.block_kids | ((uint64_t)it.tile.mask_tests << 8);
     it.hits = h->d_hits.p;
     it.hit_count = h->d_hit_count.p;
     it.hit_cap = h->hit_cap;
+    unsigned long long par0 = 0;
+    if (cand >= 0) HIP_TRY(hipMemcpy(&par0, h->d_counters.p + 3, sizeof(par0), hipMemcpyDeviceToHost));
     if (n_blocks)
         HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
                                hipMemcpyHostToDevice, h->stream));
@@ -175,6 +214,11 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
     HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);
     h->stats.launches++;
+    if (cand >= 0) {
+        unsigned long long par1 = 0;
+        HIP_TRY(hipMemcpy(&par1, h->d_counters.p + 3, sizeof(par1), hipMemcpyDeviceToHost));
+        tile_feedback(h, cand, ms, par1 - par0);
+    }
     if (adapt) {
         unsigned long long ts[2 * MAXT];
         HIP_TRY(hipMemcpy(ts, h->d_tstat.p, sizeof(ts), hipMemcpyDeviceToHost));
@@ -185,14 +229,14 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units)
 
 // Appends the scan blocks of staged prefix i (k >= 2) to h_blocks at *nb;
 // false, with *nb unchanged, if they do not fit.
-bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb)
+bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb, int bk)
 {
     const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
     const uint32_t live = h->h_batch[i].live;
     const int ns = h->ct.count(live, mf_s);
     uint32_t b = *nb;
     auto push = [&](uint32_t js, int nc) {
-        for (int j0 = 0; j0 < nc; j0 += BLOCK_KIDS) {
+        for (int j0 = 0; j0 < nc; j0 += bk) {
             if (b >= kBlockMax) return false;
             h->h_blocks[b++] = ScanBlock{i, (uint16_t)js, (uint16_t)j0};
         }
@@ -214,17 +258,20 @@ bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb)
 // k >= 2 in launches of at most kBlockMax blocks.
 int scan_batch(pcagpu *h, uint32_t n)
 {
-    if (h->k <= 1) return run_scan(h, n, 0, n);
+    int cand;
+    ScanTile tile = current_tile(h, &cand);
+    if (h->k <= 1) return run_scan(h, n, 0, n, tile, cand);
     uint32_t nb = 0;
     for (uint32_t i = 0; i < n; i++) {
-        if (add_prefix_blocks(h, i, &nb)) continue;
+        if (add_prefix_blocks(h, i, &nb, tile.block_kids)) continue;
         if (nb == 0) return PCAGPU_EOVERFLOW;
-        int rc = run_scan(h, n, nb, nb);
+        int rc = run_scan(h, n, nb, nb, tile, cand);
         if (rc) return rc;
+        tile = current_tile(h, &cand);   // the tuner may have moved on
         nb = 0;
         i--;
     }
-    return nb ? run_scan(h, n, nb, nb) : 0;
+    return nb ? run_scan(h, n, nb, nb, tile, cand) : 0;
 }
 
 size_t handle_device_bytes(const pcagpu *h)
@@ -251,7 +298,7 @@ int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_
     return 0;
 }
 
-int build_blocks(pcagpu *h, uint32_t *n_blocks, uint64_t *n_units)
+int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units)
 {
     *n_blocks = 0;
     if (h->k <= 1) {
@@ -260,7 +307,7 @@ int build_blocks(pcagpu *h, uint32_t *n_blocks, uint64_t *n_units)
     }
     uint32_t nb = 0;
     for (uint32_t i = 0; i < h->batch_n; i++)
-        if (!add_prefix_blocks(h, i, &nb)) return PCAGPU_EOVERFLOW;
+        if (!add_prefix_blocks(h, i, &nb, tile.block_kids)) return PCAGPU_EOVERFLOW;
     *n_blocks = nb;
     *n_units = nb;
     return 0;
@@ -316,7 +363,7 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     if (hipSetDevice(h->device) != hipSuccess) return fail(PCAGPU_EHIP);
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, h->device) != hipSuccess) return fail(PCAGPU_EHIP);
-    h->devname = prop.name;
+    h->devname = prop.name[0] ? prop.name : prop.gcnArchName;
     h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PCAGPU_EHIP);
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(PCAGPU_EHIP);

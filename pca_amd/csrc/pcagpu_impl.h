@@ -65,6 +65,7 @@ using pca::HostHit;
 using pca::HitRec;
 using pca::MatchRec;
 using pca::ScanBlock;
+using pca::ScanTile;
 using pca::ScanGroup;
 using pca::ScanItem;
 
@@ -96,6 +97,11 @@ struct pcagpu {
     DevBuf<unsigned long long> d_tstat;   // [2 * MAXT] survivor-phase reach / end counts of a launch
     pca::TestState roots[PCAGPU_MAX_TESTS];   // per test (task order) state after the IN scaffold
     uint64_t t_reach[PCAGPU_MAX_TESTS] = {}, t_end[PCAGPU_MAX_TESTS] = {};   // by task test index
+    // scan tile autotuner, per suffix length k: candidates tried so far,
+    // their kernel ns per parent state, the winner (-1: still trying)
+    int tile_tried[pca::MAXLEV + 1] = {};
+    double tile_cost[pca::MAXLEV + 1][8] = {};
+    int tile_best[pca::MAXLEV + 1] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
     pca_prefix_t *h_batch = nullptr;
     ScanBlock *h_blocks = nullptr;
     ScanGroup *h_groups = nullptr;
@@ -144,7 +150,10 @@ size_t handle_device_bytes(const pcagpu *h);
 int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank);
 // scan blocks of the staged batch into h_blocks (k >= 2); returns the block
 // count and the unit count, or PCAGPU_EOVERFLOW if they do not fit one launch
-int build_blocks(pcagpu *h, uint32_t *n_blocks, uint64_t *n_units);
+int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units);
+// the scan tile for the handle's current suffix length: fixed by the
+// options, the tuner's winner, or (*cand >= 0) the candidate to try next
+ScanTile current_tile(const pcagpu *h, int *cand);
 // state levels per wave a scan at k suffix positions keeps in LDS
 inline uint32_t scan_levels(int k) { return k < 1 ? 1u : (uint32_t)k; }
 }  // namespace pca

@@ -321,7 +321,9 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         if ((rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return rc;
         uint32_t nb = 0;
         uint64_t nu = 0;
-        if ((rc = build_blocks(h, &nb, &nu)) < 0) return rc;
+        int cand;
+        const ScanTile tile = current_tile(h, &cand);   // no tuning inside shared launches
+        if ((rc = build_blocks(h, tile, &nb, &nu)) < 0) return rc;
         HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
                                s->stream));
         if (nb)
@@ -340,6 +342,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         it.D = h->D;
         it.s = h->s;
         it.k = h->k;
+        it.tile = tile;
         it.hits = h->d_hits.p;
         it.hit_count = h->d_hit_count.p;
         it.hit_cap = h->hit_cap;

@@ -218,6 +218,22 @@ def test_adaptive_test_order_keeps_results(pca, name, maxd, split):
     assert got[False][3] > 4
 
 
+@pytest.mark.parametrize("tile", [(64, 1), (64, 2), (64, 3), (32, 2), (16, 3), (1, 2)])
+@pytest.mark.parametrize("name,maxd,split", [("abs", 3, 0), ("sign", 3, 1), ("is_power2_32", 2, 0)])
+def test_scan_tiles_keep_results(pca, name, maxd, split, tile):
+    """Every scan tile (children per block, tests run by wave masks) the
+    autotuner may pick gives the exact fixture's solution set."""
+    want = _exact_fixture(name, maxd)
+    t = pca.Task.load(golden_task_path(name))
+    print(f"[tile] {name} d{maxd} split={split} tile={tile}", flush=True)
+    with pca.Solver(t, scan_tile=tile) as sv:
+        res = sv.solve(maxd, split=split, cap=1 << 16)
+        assert sv.stats()["scan_tile"] == tile[0] | (tile[1] << 8)
+    assert res.optimal_depth == want["optimal_depth"]
+    assert res.num_solutions == want["num_solutions"]
+    assert [s[0] for s in res.solutions] == want["solutions"]
+
+
 @pytest.mark.parametrize("opname", ["ADD", "SUB", "MUL", "DIV", "MOD", "AND", "OR", "XOR", "SHL", "SHR",
                                     "ASR", "CMP"])
 def test_device_alu_exhaustive_matches_oracle(pca, opname):
