@@ -109,7 +109,10 @@ typedef struct {
                                  how often each ends a surviving candidate   */
     uint32_t scan_tile;       /* 0: autotuned; else block_kids (1..64) |
                                  mask_tests (1..3) << 8, fixed              */
-    uint32_t reserved[2];
+    uint32_t eager_flush;     /* 1: plain stream calls per scan launch; 0: one
+                                 replayed HIP graph (copies, clears, kernel,
+                                 counter read-back)                          */
+    uint32_t reserved[1];
 } pcagpu_opts_t;
 
 typedef struct pcagpu pcagpu_t;

@@ -194,6 +194,26 @@ struct ScanLaunch {
     unsigned long long *tstat;
 };
 
+// Kernel-node form of a one-item scan launch, for a graph-captured flush:
+// the argument block the node's kernelParams point into (the graph calls
+// copy it), and the node parameters.
+struct ScanNodeArgs {
+    ScanLaunch l;
+    ScanItem it;
+    DevTask task;
+    void *ptrs[3];
+};
+void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &task, int grid, ScanNodeArgs *args,
+                      hipKernelNodeParams *p);
+// node that zeroes four word ranges p[i][0..n[i]) (32-bit words, n <= 256
+// each); the graph's replacement for memset nodes
+struct ZeroNodeArgs {
+    uint32_t *p0, *p1, *p2, *p3;
+    uint32_t n0, n1, n2, n3;
+    void *ptrs[8];
+};
+void zero_node_params(uint32_t *const ptr[4], const uint32_t n[4], ZeroNodeArgs *args, hipKernelNodeParams *p);
+
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
 // LDS bytes of a scan launch with `levels` state levels per wave
 size_t scan_dyn_lds(uint32_t levels);

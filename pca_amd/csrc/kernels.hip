@@ -1328,6 +1328,16 @@ __global__ void alu_digest_kernel(uint32_t op, uint32_t b_lo, uint32_t b_hi, uns
     if ((threadIdx.x & 63) == 0) atomicAdd(out, h);
 }
 
+__global__ void zero_words_kernel(uint32_t *p0, uint32_t n0, uint32_t *p1, uint32_t n1, uint32_t *p2, uint32_t n2,
+                                  uint32_t *p3, uint32_t n3)
+{
+    const uint32_t i = threadIdx.x;
+    if (i < n0) p0[i] = 0u;
+    if (i < n1) p1[i] = 0u;
+    if (i < n2) p2[i] = 0u;
+    if (i < n3) p3[i] = 0u;
+}
+
 __global__ void table_init_kernel(unsigned long long *t, uint64_t cap)
 {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1363,6 +1373,49 @@ hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
     const size_t lds = scan_dyn_lds(levels);
     if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_multi, SCAN_WG, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_one, SCAN_WG, lds);
+}
+
+void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &task, int grid, ScanNodeArgs *args,
+                      hipKernelNodeParams *p)
+{
+    args->l = l;
+    args->it = it;
+    args->task = task;
+    args->ptrs[0] = &args->l;
+    args->ptrs[1] = &args->it;
+    args->ptrs[2] = &args->task;
+    p->func = reinterpret_cast<void *>(scan_kernel_one);
+    p->gridDim = dim3((unsigned)grid);
+    p->blockDim = dim3(SCAN_WG);
+    p->sharedMemBytes = (unsigned)scan_dyn_lds(l.levels);
+    p->kernelParams = args->ptrs;
+    p->extra = nullptr;
+}
+
+void zero_node_params(uint32_t *const ptr[4], const uint32_t n[4], ZeroNodeArgs *args, hipKernelNodeParams *p)
+{
+    args->p0 = ptr[0];
+    args->p1 = ptr[1];
+    args->p2 = ptr[2];
+    args->p3 = ptr[3];
+    args->n0 = n[0];
+    args->n1 = n[1];
+    args->n2 = n[2];
+    args->n3 = n[3];
+    args->ptrs[0] = &args->p0;
+    args->ptrs[1] = &args->n0;
+    args->ptrs[2] = &args->p1;
+    args->ptrs[3] = &args->n1;
+    args->ptrs[4] = &args->p2;
+    args->ptrs[5] = &args->n2;
+    args->ptrs[6] = &args->p3;
+    args->ptrs[7] = &args->n3;
+    p->func = reinterpret_cast<void *>(zero_words_kernel);
+    p->gridDim = dim3(1);
+    p->blockDim = dim3(256);
+    p->sharedMemBytes = 0;
+    p->kernelParams = args->ptrs;
+    p->extra = nullptr;
 }
 
 void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask &task0, int grid,

@@ -169,62 +169,156 @@ void tile_feedback(pcagpu *h, int cand, float ms, uint64_t parents)
     h->tile_best[k] = best;
 }
 
-int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile, int cand)
+// Read-back area of one scan launch (pinned host memory): counters,
+// hit count, per-test survivor counts.
+struct LaunchOut {
+    unsigned long long cnt[4];
+    unsigned long long ts[2 * MAXT];
+    uint32_t hits;
+    uint32_t pad;
+};
+
+// One scan launch through the handle's graph (built on the first call):
+// prefix and block upload, clears of the launch counters, the kernel
+// between two event records, read-back of the counters.  Per launch only
+// the upload sizes and the kernel parameters change.
+int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uint32_t n, uint32_t n_blocks)
 {
-    ScanItem it;
-    std::memset(&it, 0, sizeof(it));
-    it.task = h->d_task.p;
-    it.prefixes = h->d_batch.p;
-    it.blocks = h->d_blocks.p;
-    it.n_prefix = n;
-    it.unit_begin = 0;
-    it.n_units = units;
-    it.D = h->D;
-    it.s = h->s;
-    it.k = h->k;
-    it.tile = tile;
-    h->stats.scan_tile = (uint64_t)it.tile.block_kids | ((uint64_t)it.tile.mask_tests << 8);
-    it.hits = h->d_hits.p;
-    it.hit_count = h->d_hit_count.p;
-    it.hit_cap = h->hit_cap;
-    unsigned long long par0 = 0;
-    if (cand >= 0) HIP_TRY(hipMemcpy(&par0, h->d_counters.p + 3, sizeof(par0), hipMemcpyDeviceToHost));
+    ScanNodeArgs ka;
+    hipKernelNodeParams kp;
+    scan_node_params(l, it, h->dt, grid, &ka, &kp);
+    const size_t pbytes = (size_t)std::max<uint32_t>(n, 1u) * sizeof(pca_prefix_t);
+    const size_t bbytes = (size_t)std::max<uint32_t>(n_blocks, 1u) * sizeof(ScanBlock);
+    LaunchOut *o = reinterpret_cast<LaunchOut *>(h->h_out);
+    if (!h->g_exec) {
+        hipGraphNode_t zero, ev0, ev1, r0, r1, r2;
+        uint32_t *zp_ptr[4] = {reinterpret_cast<uint32_t *>(h->d_work.p), reinterpret_cast<uint32_t *>(h->d_tstat.p),
+                               reinterpret_cast<uint32_t *>(h->d_counters.p), h->d_hit_count.p};
+        const uint32_t zp_n[4] = {2, 4 * MAXT, 8, 1};
+        ZeroNodeArgs za;
+        hipKernelNodeParams zp;
+        zero_node_params(zp_ptr, zp_n, &za, &zp);
+        HIP_TRY(hipGraphCreate(&h->g_graph, 0));
+        HIP_TRY(hipGraphAddMemcpyNode1D(&h->g_prefixes, h->g_graph, nullptr, 0, h->d_batch.p, h->h_batch, pbytes,
+                                        hipMemcpyHostToDevice));
+        HIP_TRY(hipGraphAddMemcpyNode1D(&h->g_blocks, h->g_graph, &h->g_prefixes, 1, h->d_blocks.p, h->h_blocks,
+                                        bbytes, hipMemcpyHostToDevice));
+        HIP_TRY(hipGraphAddKernelNode(&zero, h->g_graph, &h->g_blocks, 1, &zp));
+        HIP_TRY(hipGraphAddEventRecordNode(&ev0, h->g_graph, &zero, 1, h->ev0));
+        HIP_TRY(hipGraphAddKernelNode(&h->g_kernel, h->g_graph, &ev0, 1, &kp));
+        HIP_TRY(hipGraphAddEventRecordNode(&ev1, h->g_graph, &h->g_kernel, 1, h->ev1));
+        HIP_TRY(hipGraphAddMemcpyNode1D(&r0, h->g_graph, &ev1, 1, o->cnt, h->d_counters.p, sizeof(o->cnt),
+                                        hipMemcpyDeviceToHost));
+        HIP_TRY(hipGraphAddMemcpyNode1D(&r1, h->g_graph, &r0, 1, o->ts, h->d_tstat.p, sizeof(o->ts),
+                                        hipMemcpyDeviceToHost));
+        HIP_TRY(hipGraphAddMemcpyNode1D(&r2, h->g_graph, &r1, 1, &o->hits, h->d_hit_count.p, sizeof(o->hits),
+                                        hipMemcpyDeviceToHost));
+        HIP_TRY(hipGraphInstantiate(&h->g_exec, h->g_graph, nullptr, nullptr, 0));
+    } else {
+        HIP_TRY(hipGraphExecMemcpyNodeSetParams1D(h->g_exec, h->g_prefixes, h->d_batch.p, h->h_batch, pbytes,
+                                                  hipMemcpyHostToDevice));
+        HIP_TRY(hipGraphExecMemcpyNodeSetParams1D(h->g_exec, h->g_blocks, h->d_blocks.p, h->h_blocks, bbytes,
+                                                  hipMemcpyHostToDevice));
+        HIP_TRY(hipGraphExecKernelNodeSetParams(h->g_exec, h->g_kernel, &kp));
+    }
+    HIP_TRY(hipGraphLaunch(h->g_exec, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// The same launch with plain stream calls.
+int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uint32_t n, uint32_t n_blocks)
+{
+    LaunchOut *o = reinterpret_cast<LaunchOut *>(h->h_out);
+    HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
+                           h->stream));
     if (n_blocks)
         HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
                                hipMemcpyHostToDevice, h->stream));
-    HIP_TRY(hipMemcpyAsync(h->d_items.p, &it, sizeof(it), hipMemcpyHostToDevice, h->stream));
-    ScanLaunch l;
-    l.items = h->d_items.p;
-    l.n_items = 1;
-    l.levels = scan_levels(h->k);
-    l.n_units = units;
-    l.cand = cand_dev(h);
-    l.counters = h->d_counters.p;
-    l.work = h->d_work.p;
-    const bool adapt = !h->opts.fixed_test_order;
-    l.tstat = adapt ? h->d_tstat.p : nullptr;
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
-    if (adapt) HIP_TRY(hipMemsetAsync(h->d_tstat.p, 0, 2 * MAXT * sizeof(unsigned long long), h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_tstat.p, 0, 2 * MAXT * sizeof(unsigned long long), h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_counters.p, 0, 4 * sizeof(unsigned long long), h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
-    launch_scan(l, &it, h->dt, grid_for(h, units, false, l.levels), h->stream);
+    launch_scan(l, &it, h->dt, grid, h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipMemcpyAsync(o->cnt, h->d_counters.p, sizeof(o->cnt), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(o->ts, h->d_tstat.p, sizeof(o->ts), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(&o->hits, h->d_hit_count.p, sizeof(o->hits), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
-    float ms = 0.f;
-    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-    h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);
-    h->stats.launches++;
-    if (cand >= 0) {
-        unsigned long long par1 = 0;
-        HIP_TRY(hipMemcpy(&par1, h->d_counters.p + 3, sizeof(par1), hipMemcpyDeviceToHost));
-        tile_feedback(h, cand, ms, par1 - par0);
-    }
-    if (adapt) {
-        unsigned long long ts[2 * MAXT];
-        HIP_TRY(hipMemcpy(ts, h->d_tstat.p, sizeof(ts), hipMemcpyDeviceToHost));
-        return adapt_test_order(h, ts);
-    }
     return 0;
+}
+
+// One scan launch over the staged batch (n prefixes) and n_blocks blocks,
+// its hits appended to h->raw.  A launch whose hits overflow the device
+// buffer is repeated with a larger one: no hit is dropped.
+int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile, int cand)
+{
+    const LaunchOut *o = reinterpret_cast<const LaunchOut *>(h->h_out);
+    const bool adapt = !h->opts.fixed_test_order;
+    ScanItem it;
+    ScanLaunch l;
+    float ms = 0.f;
+    for (;;) {
+        std::memset(&it, 0, sizeof(it));
+        it.task = h->d_task.p;
+        it.prefixes = h->d_batch.p;
+        it.blocks = h->d_blocks.p;
+        it.n_prefix = n;
+        it.unit_begin = 0;
+        it.n_units = units;
+        it.D = h->D;
+        it.s = h->s;
+        it.k = h->k;
+        it.tile = tile;
+        it.hits = h->d_hits.p;
+        it.hit_count = h->d_hit_count.p;
+        it.hit_cap = h->hit_cap;
+        l.items = h->d_items.p;
+        l.n_items = 1;
+        l.levels = scan_levels(h->k);
+        l.n_units = units;
+        l.cand = cand_dev(h);
+        l.counters = h->d_counters.p;
+        l.work = h->d_work.p;
+        l.tstat = adapt ? h->d_tstat.p : nullptr;
+        const int grid = grid_for(h, units, false, l.levels);
+        int rc = h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks) : graph_scan(h, l, it, grid, n, n_blocks);
+        if (rc) return rc;
+        HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);
+        h->stats.launches++;
+        if (o->hits <= h->hit_cap) break;
+        const uint32_t want = o->hits + o->hits / 2 + 1024;
+        h->d_hits.release();
+        if ((rc = h->d_hits.ensure(want))) return rc;
+        h->hit_cap = want;
+    }
+    h->stats.scan_tile = (uint64_t)tile.block_kids | ((uint64_t)tile.mask_tests << 8);
+    h->stats.candidates += o->cnt[0];
+    h->stats.suffixes += o->cnt[1];
+    h->stats.tests_evaluated += o->cnt[2];
+    h->stats.parents += o->cnt[3];
+    const uint32_t hits = o->hits;
+    h->stats.raw_hits += hits;
+    if (hits) {
+        std::vector<HitRec> buf(hits);
+        HIP_TRY(hipMemcpy(buf.data(), h->d_hits.p, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost));
+        for (const HitRec &r : buf) {
+            HostHit x;
+            x.prefix = h->batch_first + r.prefix;
+            x.mask = r.mask;
+            x.flags = r.flags;
+            std::memcpy(x.j, r.j, sizeof(x.j));
+            h->raw.push_back(x);
+        }
+    }
+    tile_feedback(h, cand, ms, o->cnt[3]);
+    if (!adapt) return 0;
+    unsigned long long ts[2 * MAXT];
+    std::memcpy(ts, o->ts, sizeof(ts));
+    return adapt_test_order(h, ts);
 }
 
 // Appends the scan blocks of staged prefix i (k >= 2) to h_blocks at *nb;
@@ -313,12 +407,6 @@ int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units
     return 0;
 }
 
-int read_counters(pcagpu *h, unsigned long long out[4])
-{
-    HIP_TRY(hipMemcpy(out, h->d_counters.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-    return 0;
-}
-
 }  // namespace
 
 extern "C" {
@@ -376,7 +464,8 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
         (rc = h->d_tstat.ensure(2 * MAXT)))
         return fail(rc);
     if (hipHostMalloc((void **)&h->h_batch, (size_t)h->batch_cap * sizeof(pca_prefix_t)) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_blocks, (size_t)kBlockMax * sizeof(ScanBlock)) != hipSuccess)
+        hipHostMalloc((void **)&h->h_blocks, (size_t)kBlockMax * sizeof(ScanBlock)) != hipSuccess ||
+        hipHostMalloc(&h->h_out, sizeof(LaunchOut)) != hipSuccess)
         return fail(PCAGPU_ENOMEM);
     if (hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
         return fail(PCAGPU_EHIP);
@@ -415,6 +504,9 @@ void pcagpu_close(pcagpu_t *h)
     h->d_match.release();
     h->d_match_count.release();
     if (h->h_batch) (void)hipHostFree(h->h_batch);
+    if (h->g_exec) (void)hipGraphExecDestroy(h->g_exec);
+    if (h->g_graph) (void)hipGraphDestroy(h->g_graph);
+    if (h->h_out) (void)hipHostFree(h->h_out);
     if (h->h_blocks) (void)hipHostFree(h->h_blocks);
     if (h->h_groups) (void)hipHostFree(h->h_groups);
     if (h->h_goff) (void)hipHostFree(h->h_goff);
@@ -450,43 +542,8 @@ int pcagpu_flush(pcagpu_t *h)
     if (h->batch_n == 0) return (int)std::min<size_t>(h->raw.size(), 0x7FFFFFFF);
     (void)hipSetDevice(h->device);
     const uint32_t n = h->batch_n;
-    HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
-                           h->stream));
-    uint32_t hits = 0;
     int rc;
-    for (;;) {
-        HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), h->stream));
-        if ((rc = scan_batch(h, n))) return rc;
-        HIP_TRY(hipMemcpy(&hits, h->d_hit_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
-        if (hits <= h->hit_cap) break;
-        // grow and redo this batch: no hit may be dropped
-        uint32_t want = hits + hits / 2 + 1024;
-        h->d_hits.release();
-        if ((rc = h->d_hits.ensure(want))) return rc;
-        h->hit_cap = want;
-        // the counters of the discarded pass must not be counted twice
-        HIP_TRY(hipMemset(h->d_counters.p, 0, 4 * sizeof(unsigned long long)));
-    }
-    if (hits) {
-        std::vector<HitRec> buf(hits);
-        HIP_TRY(hipMemcpy(buf.data(), h->d_hits.p, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost));
-        for (const HitRec &r : buf) {
-            HostHit x;
-            x.prefix = h->batch_first + r.prefix;
-            x.mask = r.mask;
-            x.flags = r.flags;
-            std::memcpy(x.j, r.j, sizeof(x.j));
-            h->raw.push_back(x);
-        }
-    }
-    unsigned long long c[4];
-    if ((rc = read_counters(h, c))) return rc;
-    h->stats.candidates += c[0];
-    h->stats.suffixes += c[1];
-    h->stats.tests_evaluated += c[2];
-    h->stats.parents += c[3];
-    HIP_TRY(hipMemset(h->d_counters.p, 0, 4 * sizeof(unsigned long long)));
-    h->stats.raw_hits += hits;
+    if ((rc = scan_batch(h, n))) return rc;
     h->batch_first += n;
     h->batch_n = 0;
     return (int)std::min<size_t>(h->raw.size(), 0x7FFFFFFF);

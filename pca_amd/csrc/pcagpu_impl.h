@@ -102,6 +102,11 @@ struct pcagpu {
     int tile_tried[pca::MAXLEV + 1] = {};
     double tile_cost[pca::MAXLEV + 1][8] = {};
     int tile_best[pca::MAXLEV + 1] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+    // graph-captured scan launch (see graph_scan); built on first use
+    hipGraph_t g_graph = nullptr;
+    hipGraphExec_t g_exec = nullptr;
+    hipGraphNode_t g_prefixes = nullptr, g_blocks = nullptr, g_kernel = nullptr;
+    void *h_out = nullptr;                  // pinned read-back area of a launch
     pca_prefix_t *h_batch = nullptr;
     ScanBlock *h_blocks = nullptr;
     ScanGroup *h_groups = nullptr;

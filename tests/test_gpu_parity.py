@@ -218,6 +218,24 @@ def test_adaptive_test_order_keeps_results(pca, name, maxd, split):
     assert got[False][3] > 4
 
 
+@pytest.mark.parametrize("name,D,s", [("abs", 3, 1), ("relu", 3, 1), ("sign", 3, 2)])
+def test_graph_flush_replays_match_eager(pca, oracle, name, D, s):
+    """Flushes of 3 prefixes replay the scan graph many times, with the
+    library's own eager copies and syncs between replays; the raw hits of
+    every replay, in order, must equal plain stream launches' hits."""
+    t = pca.Task.load(golden_task_path(name))
+    prefixes, _, _ = pca.host_frontier(t, D, s)
+    print(f"[graph] {name} D{D} s{s} prefixes={len(prefixes)}", flush=True)
+    rows = {}
+    for eager in (True, False):
+        with pca.Solver(t, batch_capacity=3, eager_flush=eager, fixed_test_order=True, scan_tile=(64, 2)) as sv:
+            rows[eager] = sv.raw_depth(D, s, prefixes)
+            launches = sv.stats()["launches"]
+    assert launches >= len(prefixes) // 3
+    assert rows[False] == rows[True]
+    assert len(rows[True]) > 0
+
+
 @pytest.mark.parametrize("tile", [(64, 1), (64, 2), (64, 3), (32, 2), (16, 3), (1, 2)])
 @pytest.mark.parametrize("name,maxd,split", [("abs", 3, 0), ("sign", 3, 1), ("is_power2_32", 2, 0)])
 def test_scan_tiles_keep_results(pca, name, maxd, split, tile):
