@@ -77,6 +77,24 @@ def serving_bench(rates=(10, 50, 200), n_req=200, seed=7):
     return out
 
 
+def suite_bench(n_tasks, budget, seed=11):
+    """Config-3 batched suite: seeded random straight-line programs
+    (taskgen.random_task), every task submitted at once to the server (one
+    persistent launch per step over all active tasks) and, for comparison,
+    one at a time through Solver.solve."""
+    from pca_amd import taskgen, serving
+    tasks = [taskgen.random_task(seed, i) for i in range(n_tasks)]
+    a, wa, st = serving.suite_run(tasks, budget, max_slots=256, prefixes_per_item=1 << 20)
+    print("suite: server done", file=sys.stderr, flush=True)
+    b, wb = serving.sequential_suite(tasks, budget)
+    print("suite: one-at-a-time done", file=sys.stderr, flush=True)
+    return {"tasks": n_tasks, "depth_budget": budget, "generator": f"taskgen.random_task(seed {seed})",
+            "solved_within_budget": sum(1 for x in a if x[0] >= 0),
+            "server_wall_s": wa, "one_at_a_time_wall_s": wb, "server_kernel_s": st["kernel_ns"] * 1e-9,
+            "launches": st["launches"], "items_per_launch": st["items"] / max(st["launches"], 1),
+            "candidates": st["candidates"], "same_results": a == b}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +107,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--suite", type=int, default=1000, help="config-3 suite size (0: skip)")
+    ap.add_argument("--suite-budget", type=int, default=3)
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -179,6 +199,8 @@ def main():
             line["roofline"]["traffic"] = json.load(f)["traffic_bytes_per_launch"]
     if not args.no_serving:
         line["serving"] = serving_bench(n_req=args.serving_requests)
+    if args.suite:
+        line["suite"] = suite_bench(args.suite, args.suite_budget)
     if not args.no_cpu:
         cb = cpu_baseline(task_path, args.cpu_seconds)
         line["cpu_baseline"] = cb

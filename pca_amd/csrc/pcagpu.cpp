@@ -331,7 +331,7 @@ bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb, int bk)
     uint32_t b = *nb;
     auto push = [&](uint32_t js, int nc) {
         for (int j0 = 0; j0 < nc; j0 += bk) {
-            if (b >= kBlockMax) return false;
+            if (b >= h->block_cap) return false;
             h->h_blocks[b++] = ScanBlock{i, (uint16_t)js, (uint16_t)j0};
         }
         return true;
@@ -348,6 +348,22 @@ bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb, int bk)
     return true;
 }
 
+// Block buffers (host and device) at full size on the first scan with
+// k >= 2; handles that only scan single suffix positions never need them.
+int ensure_block_buffers(pcagpu *h)
+{
+    if (h->block_cap >= kBlockMax) return 0;
+    int rc;
+    if ((rc = h->d_blocks.ensure(kBlockMax))) return rc;
+    if (h->h_blocks) (void)hipHostFree(h->h_blocks);
+    h->h_blocks = nullptr;
+    h->block_cap = 0;
+    if (hipHostMalloc((void **)&h->h_blocks, (size_t)kBlockMax * sizeof(ScanBlock)) != hipSuccess)
+        return PCAGPU_ENOMEM;
+    h->block_cap = kBlockMax;
+    return 0;
+}
+
 // All scan launches of one flush: k <= 1 in one launch (a unit per prefix);
 // k >= 2 in launches of at most kBlockMax blocks.
 int scan_batch(pcagpu *h, uint32_t n)
@@ -355,6 +371,8 @@ int scan_batch(pcagpu *h, uint32_t n)
     int cand;
     ScanTile tile = current_tile(h, &cand);
     if (h->k <= 1) return run_scan(h, n, 0, n, tile, cand);
+    int rc0 = ensure_block_buffers(h);
+    if (rc0) return rc0;
     uint32_t nb = 0;
     for (uint32_t i = 0; i < n; i++) {
         if (add_prefix_blocks(h, i, &nb, tile.block_kids)) continue;
@@ -399,6 +417,8 @@ int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units
         *n_units = h->batch_n;
         return 0;
     }
+    int rc = ensure_block_buffers(h);
+    if (rc) return rc;
     uint32_t nb = 0;
     for (uint32_t i = 0; i < h->batch_n; i++)
         if (!add_prefix_blocks(h, i, &nb, tile.block_kids)) return PCAGPU_EOVERFLOW;
@@ -458,13 +478,13 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     fill_devtask(h);
     if (!(h->d_grammar = shared_grammar_dev(h->device))) return fail(PCAGPU_ENOMEM);
     if ((rc = h->d_batch.ensure(h->batch_cap)) ||
-        (rc = h->d_blocks.ensure(kBlockMax)) ||
+        (rc = h->d_blocks.ensure(1)) ||
         (rc = h->d_hits.ensure(h->hit_cap)) ||
         (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(8)) || (rc = h->d_work.ensure(1)) ||
         (rc = h->d_tstat.ensure(2 * MAXT)))
         return fail(rc);
     if (hipHostMalloc((void **)&h->h_batch, (size_t)h->batch_cap * sizeof(pca_prefix_t)) != hipSuccess ||
-        hipHostMalloc((void **)&h->h_blocks, (size_t)kBlockMax * sizeof(ScanBlock)) != hipSuccess ||
+        hipHostMalloc((void **)&h->h_blocks, sizeof(ScanBlock)) != hipSuccess ||
         hipHostMalloc(&h->h_out, sizeof(LaunchOut)) != hipSuccess)
         return fail(PCAGPU_ENOMEM);
     if (hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)

@@ -109,6 +109,7 @@ struct pcagpu {
     void *h_out = nullptr;                  // pinned read-back area of a launch
     pca_prefix_t *h_batch = nullptr;
     ScanBlock *h_blocks = nullptr;
+    uint32_t block_cap = 0;          // h_blocks / d_blocks records (kBlockMax once a k >= 2 scan ran)
     ScanGroup *h_groups = nullptr;
     uint64_t *h_goff = nullptr;
     uint32_t batch_cap = 0;

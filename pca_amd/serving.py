@@ -137,6 +137,30 @@ def sequential_run(tasks, budgets, arrivals):
     return recs
 
 
+def suite_run(tasks, budget, timeout=1200.0, **server_kw):
+    """A batched suite: every task submitted at once, each server step one
+    persistent launch over all active tasks.  Returns per-task
+    (optimal_depth, num_solutions), the wall time and the server stats."""
+    with Server(**server_kw) as sv:
+        t0 = time.perf_counter()
+        ids = {sv.submit(t, budget): i for i, t in enumerate(tasks)}
+        out = [None] * len(tasks)
+        for r in sv.drain(timeout):
+            out[ids[r["id"]]] = (r["optimal_depth"], r["num_solutions"])
+        return out, time.perf_counter() - t0, sv.stats()
+
+
+def sequential_suite(tasks, budget):
+    """The same suite one task at a time through Solver.solve."""
+    out = []
+    t0 = time.perf_counter()
+    for t in tasks:
+        with Solver(t) as s:
+            r = s.solve(budget, cap=1)
+        out.append((r.optimal_depth, r.num_solutions))
+    return out, time.perf_counter() - t0
+
+
 def latency_stats(recs):
     lat = sorted(done - arr for arr, done, _, _ in recs)
     if not lat:
