@@ -130,6 +130,18 @@ void pcagpu_disasm(uint16_t insn, char *buf, size_t len);
 /* tools/enumerate.c:206-267 gen_candidates (branch mode, use_branches=1):
  * the ordered candidate list at kernel position pos of a depth-D tree. */
 int  pcagpu_gen_candidates(uint16_t *out, int max, uint8_t live, int pos, int kernel_depth);
+/* Search-state records for the value network (include/pcavalue.h): for
+ * each of n kernel prefixes (insns[i*PCAGPU_MAX_KDEPTH ..], len[i] grammar
+ * instructions at kernel positions 0..len[i]-1) and each test t < 32, the
+ * 16 words rec[(i*32 + t)*16 ..] = {input0, input1, target, r0..r7, flags,
+ * frozen (a DIV/MOD fault stopped the VM), resume (pending forward branch
+ * target), 0, 0} of the state after the IN scaffold and the prefix (the state
+ * the reference serialises, src/search_state.h:38-61); tests >= num_tests
+ * are all zero.  live[i] = the prefix's live mask (r1..r_n_in plus every
+ * register written).  No counterpart in the reference's C API: its Python
+ * feature extractor reads search_state_t (python/search_eval.py:206-251). */
+int  pcagpu_state_records(const pca_task_t *task, const uint16_t *insns, const uint8_t *len, int n,
+                          uint16_t *rec, uint8_t *live);
 
 /* ---- device handle: replaces the constant uploads and buffers of
  *      gpu/hybrid.cu:592-621, :662-680 ---- */

@@ -172,13 +172,15 @@ class ServerStats(C.Structure):
 
 _LIB = None
 
-# (name, restype, argtypes) for every symbol include/pcagpu.h declares
+# (name, restype, argtypes) for every symbol include/*.h declares
 SIGNATURES = [
     ("pcagpu_task_load", C.c_int, [C.c_char_p, C.POINTER(Task)]),
     ("pcagpu_task_parse", C.c_int, [C.c_char_p, C.POINTER(Task)]),
     ("pcagpu_task_evaluate", C.c_int, [C.POINTER(Task), C.POINTER(C.c_uint16), C.c_int]),
     ("pcagpu_disasm", None, [C.c_uint16, C.c_char_p, C.c_size_t]),
     ("pcagpu_gen_candidates", C.c_int, [C.POINTER(C.c_uint16), C.c_int, C.c_uint8, C.c_int, C.c_int]),
+    ("pcagpu_state_records", C.c_int, [C.POINTER(Task), C.POINTER(C.c_uint16), C.POINTER(C.c_uint8), C.c_int,
+                                       C.POINTER(C.c_uint16), C.POINTER(C.c_uint8)]),
     ("pcagpu_open", C.c_int, [C.POINTER(Task), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     ("pcagpu_close", None, [C.c_void_p]),
     ("pcagpu_device_name", C.c_char_p, [C.c_void_p]),
@@ -203,6 +205,26 @@ SIGNATURES = [
     ("pcagpu_server_solutions", C.c_long, [C.c_void_p, C.c_uint64, C.POINTER(Solution), C.c_long]),
     ("pcagpu_server_get_stats", C.c_int, [C.c_void_p, C.POINTER(ServerStats)]),
     ("pcagpu_server_clock", C.c_double, [C.c_void_p]),
+    # include/pcavalue.h: device pointers and streams as integers
+    ("pcav_gemm", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_int,
+                            C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p]),
+    ("pcav_features", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_relu_backward", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_pool_forward", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                    C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_pool_backward", C.c_int, [C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                     C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_colsum", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
+    ("pcav_bce", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_adam", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
+                            C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p]),
+    ("pcav_to_bf16", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_malloc", C.c_void_p, [C.c_int, C.c_size_t]),
+    ("pcav_free", None, [C.c_void_p]),
+    ("pcav_copy_to_device", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("pcav_copy_to_host", C.c_int, [C.c_void_p, C.c_void_p, C.c_size_t]),
+    ("pcav_clear", C.c_int, [C.c_void_p, C.c_size_t]),
+    ("pcav_synchronize", C.c_int, []),
 ]
 
 

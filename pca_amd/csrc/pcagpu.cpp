@@ -440,6 +440,41 @@ int pcagpu_gen_candidates(uint16_t *out, int max, uint8_t live, int pos, int ker
     return gen_candidates(out, max, live, pos, kernel_depth);
 }
 
+int pcagpu_state_records(const pca_task_t *task, const uint16_t *insns, const uint8_t *len, int n, uint16_t *rec,
+                         uint8_t *live)
+{
+    if (!task || n < 0 || (n && (!insns || !len || !rec || !live))) return PCAGPU_EINVAL;
+    if (task->num_tests < 0 || task->num_tests > PCAGPU_MAX_TESTS || task->num_inputs < 0 || task->num_inputs > 7)
+        return PCAGPU_EINVAL;
+    TestState roots[PCAGPU_MAX_TESTS];
+    root_states(task, roots);
+    uint32_t live0 = 0;
+    for (int i = 0; i < task->num_inputs; i++) live0 |= 1u << (i + 1);
+    for (int i = 0; i < n; i++) {
+        const uint16_t *p = insns + (size_t)i * PCAGPU_MAX_KDEPTH;
+        const int L = len[i];
+        if (L > PCAGPU_MAX_KDEPTH) return PCAGPU_EINVAL;
+        uint32_t lv = live0;
+        for (int q = 0; q < L; q++) lv = live_after(lv, p[q]);
+        live[i] = (uint8_t)lv;
+        uint16_t *r = rec + (size_t)i * 32 * 16;
+        std::memset(r, 0, 32 * 16 * sizeof(uint16_t));
+        for (int t = 0; t < task->num_tests; t++) {
+            TestState s = roots[t];
+            for (int q = 0; q < L; q++) step_test(s, p[q], (uint32_t)q);
+            uint16_t *w = r + t * 16;
+            w[0] = task->num_inputs > 0 ? task->tests[t].in[0] : 0;
+            w[1] = task->num_inputs > 1 ? task->tests[t].in[1] : 0;
+            w[2] = task->tests[t].out[0];
+            for (int k = 0; k < 8; k++) w[3 + k] = s.r[k];
+            w[11] = s.flags;
+            w[12] = s.frozen;
+            w[13] = s.resume;
+        }
+    }
+    return 0;
+}
+
 int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **out)
 {
     if (!task || !out) return PCAGPU_EINVAL;

@@ -1,0 +1,361 @@
+// value.hip -- BF16 MFMA kernels of the value network (include/pcavalue.h).
+//
+// GEMM: one 256-thread workgroup computes a 64 x 64 tile of C as four
+// 32 x 32 wave tiles with v_mfma_f32_32x32x16_bf16 (gfx950), FP32
+// accumulators in registers.  Each K step stages a 64 x 32 slab of A and of
+// B in LDS, both k-contiguous per row/column so that a lane's MFMA operand
+// (8 consecutive k of one row of A or one column of B) is one 16-byte LDS
+// read; the transposed storage modes only change how the slab is gathered
+// from HBM.  Out-of-range rows, columns and k are zero-filled, so any
+// M, N, K works.  The epilogue adds the bias, applies ReLU and stores FP32
+// and/or BF16.
+//
+// Operand layout of v_mfma_f32_32x32x16_bf16 (lane l of 64): A: row l % 32,
+// k = 8 * (l / 32) .. +7; B: column l % 32, the same k; the 16 accumulators
+// hold C rows 8 * (r / 4) + 4 * (l / 32) + r % 4 of column l % 32.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <algorithm>
+
+#include "../../include/pcagpu.h"
+#include "../../include/pcavalue.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int GM = 64, GN = 64, GK = 32;
+constexpr int LDS_K = GK + 8;   // row stride in halves: 80 bytes, 16-byte aligned rows
+
+__device__ __forceinline__ uint16_t to_bf16(float x)
+{
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);   // round to nearest even (inputs are finite)
+    return (uint16_t)(u >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
+                                                   int a_trans, const uint16_t *__restrict__ B, int ldb, int b_trans,
+                                                   const float *__restrict__ bias, int relu, float *C32, int ldc,
+                                                   int accumulate, uint16_t *C16, int ldc16)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t As[GM][LDS_K];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[GN][LDS_K];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int wm = w >> 1, wn = w & 1;
+    const int m0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
+    f32x16 acc = {};
+    for (int k0 = 0; k0 < K; k0 += GK) {
+        // stage A (64 rows x 32 k) and B (64 columns x 32 k): 8 elements per thread each
+        for (int e = tid; e < GM * GK; e += 256) {
+            int r, k;
+            if (a_trans) { r = e % GM; k = e / GM; }   // consecutive threads along m: contiguous in A
+            else { r = e / GK; k = e % GK; }           // consecutive threads along k: contiguous in A
+            const int gm = m0 + r, gk = k0 + k;
+            uint16_t v = 0;
+            if (gm < M && gk < K) v = a_trans ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
+            As[r][k] = v;
+        }
+        for (int e = tid; e < GN * GK; e += 256) {
+            int c, k;
+            if (b_trans) { c = e / GK; k = e % GK; }
+            else { c = e % GN; k = e / GN; }
+            const int gn = n0 + c, gk = k0 + k;
+            uint16_t v = 0;
+            if (gn < N && gk < K) v = b_trans ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
+            Bs[c][k] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int kk = 0; kk < GK; kk += 16) {
+            const int ko = kk + 8 * (lane >> 5);
+            bf16x8 a, b;
+            __builtin_memcpy(&a, &As[wm * 32 + (lane & 31)][ko], 16);
+            __builtin_memcpy(&b, &Bs[wn * 32 + (lane & 31)][ko], 16);
+            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+        }
+        __syncthreads();
+    }
+    const int col = n0 + wn * 32 + (lane & 31);
+    if (col >= N) return;
+    const float bv = bias ? bias[col] : 0.f;
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+        if (row >= M) continue;
+        float v = acc[r] + bv;
+        if (relu) v = v > 0.f ? v : 0.f;
+        if (C32) {
+            float *p = C32 + (int64_t)row * ldc + col;
+            *p = accumulate ? *p + v : v;
+        }
+        if (C16) C16[(int64_t)row * ldc16 + col] = to_bf16(v);
+    }
+}
+
+__global__ void relu_backward_kernel(int64_t n, const uint16_t *act, float *grad, uint16_t *out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const bool on = act[i] != 0 && !(act[i] & 0x8000u);
+        const float g = on ? grad[i] : 0.f;
+        grad[i] = g;
+        out[i] = to_bf16(g);
+    }
+}
+
+// one thread per (state, test, feature word): 180 features from 16 words
+__global__ void features_kernel(int B, const uint16_t *rec, uint16_t *X)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;   // (b*32 + t) * 180 + f
+    if (i >= (int64_t)B * 32 * 180) return;
+    const int64_t row = i / 180;
+    const int f = (int)(i % 180);
+    const uint16_t *r = rec + row * 16;
+    // f < 176: bit f % 16 of word f / 16 (input0, input1, target, r0..r7);
+    // 176..179: flag bits 0..3
+    const uint32_t word = f < 176 ? r[f >> 4] : r[11];
+    const uint32_t bit = f < 176 ? (uint32_t)(f & 15) : (uint32_t)(f - 176);
+    X[i] = ((word >> bit) & 1u) ? (uint16_t)0x3F80u : (uint16_t)0;   // bf16 1.0 / 0.0
+}
+
+// one workgroup per state, one thread per channel (C + E <= 1024)
+__global__ void pool_forward_kernel(int T, int C, int E, const uint16_t *H, const uint8_t *nvalid,
+                                    const int32_t *depth, const float *emb, uint16_t *Z)
+{
+    const int b = blockIdx.x, c = threadIdx.x;
+    const int nv = nvalid[b];
+    uint16_t *z = Z + (int64_t)b * (C + E);
+    if (c < C) {
+        float s = 0.f;
+        for (int t = 0; t < nv; t++) s += from_bf16(H[((int64_t)b * T + t) * C + c]);
+        z[c] = to_bf16(nv ? s / (float)nv : 0.f);
+    } else if (c < C + E) {
+        z[c] = to_bf16(emb[depth[b] * E + (c - C)]);
+    }
+}
+
+__global__ void pool_backward_kernel(int T, int C, int E, const float *dZ, const uint8_t *nvalid,
+                                     const int32_t *depth, float *dH, float *demb)
+{
+    const int b = blockIdx.x, c = threadIdx.x;
+    const int nv = nvalid[b];
+    const float *dz = dZ + (int64_t)b * (C + E);
+    if (c < C) {
+        const float g = nv ? dz[c] / (float)nv : 0.f;
+        for (int t = 0; t < T; t++) dH[((int64_t)b * T + t) * C + c] = t < nv ? g : 0.f;
+    } else if (c < C + E) {
+        atomicAdd(&demb[depth[b] * E + (c - C)], dz[c]);
+    }
+}
+
+// 64 columns x 256 row-chunks per workgroup, partial sums added atomically
+__global__ void colsum_kernel(int M, int N, const float *G, int ldg, float *out)
+{
+    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int r0 = blockIdx.y * 1024 + (threadIdx.x >> 6);
+    if (c >= N) return;
+    float s = 0.f;
+    for (int r = r0; r < M && r < (int)(blockIdx.y + 1) * 1024; r += 4) s += G[(int64_t)r * ldg + c];
+    atomicAdd(&out[c], s);
+}
+
+__global__ void fill_f32_kernel(int n, float *p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.f;
+}
+
+// one workgroup: loss reduction in LDS
+__global__ void bce_kernel(int B, const float *y, const float *label, float *dy, float *loss)
+{
+    __shared__ float part[256];
+    float s = 0.f;
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        const float x = y[b], t = label[b];
+        // softplus(x) - t*x, written to stay finite for large |x|
+        s += fmaxf(x, 0.f) - t * x + log1pf(expf(-fabsf(x)));
+        dy[b] = (1.f / (1.f + expf(-x)) - t) / (float)B;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[0] = part[0] / (float)B;
+}
+
+__global__ void adam_kernel(int64_t n, float *p, const float *g, float *m, float *v, uint16_t *p16, float lr,
+                            float b1, float b2, float eps, float c1, float c2)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        const float pi = p[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+        p[i] = pi;
+        if (p16) p16[i] = to_bf16(pi);
+    }
+}
+
+__global__ void to_bf16_kernel(int64_t n, const float *x, uint16_t *y)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = to_bf16(x[i]);
+}
+
+inline hipStream_t st(void *s) { return reinterpret_cast<hipStream_t>(s); }
+inline int grid1d(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 65535); }
+// error state of the launches just made; an entry point first clears any
+// error a previous, unrelated runtime call of this thread left behind
+inline int status() { return hipGetLastError() == hipSuccess ? PCAGPU_OK : PCAGPU_EHIP; }
+inline void clear_error() { (void)hipGetLastError(); }
+
+}  // namespace
+
+extern "C" {
+
+int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, const uint16_t *B, int ldb,
+              int b_trans, const float *bias, int relu, float *C32, int ldc, int accumulate, uint16_t *C16,
+              int ldc16, void *stream)
+{
+    if (M < 0 || N < 0 || K < 0 || !A || !B || (!C32 && !C16)) return PCAGPU_EINVAL;
+    if (M == 0 || N == 0) return PCAGPU_OK;
+    const dim3 grid((unsigned)((N + GN - 1) / GN), (unsigned)((M + GM - 1) / GM));
+    if (grid.y > 65535u) return PCAGPU_EINVAL;
+    clear_error();
+    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st(stream), M, N, K, A, lda, a_trans, B, ldb, b_trans, bias,
+                       relu, C32, ldc, accumulate, C16, ldc16);
+    return status();
+}
+
+int pcav_features(int B, const uint16_t *rec, uint16_t *X, void *stream)
+{
+    if (B < 0 || !rec || !X) return PCAGPU_EINVAL;
+    if (B == 0) return PCAGPU_OK;
+    const int64_t n = (int64_t)B * 32 * 180;
+    clear_error();
+    hipLaunchKernelGGL(features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st(stream), B, rec, X);
+    return status();
+}
+
+int pcav_relu_backward(int64_t n, const uint16_t *act16, float *grad, uint16_t *out16, void *stream)
+{
+    if (n < 0 || !act16 || !grad || !out16) return PCAGPU_EINVAL;
+    if (n == 0) return PCAGPU_OK;
+    clear_error();
+    hipLaunchKernelGGL(relu_backward_kernel, dim3(grid1d(n)), dim3(256), 0, st(stream), n, act16, grad, out16);
+    return status();
+}
+
+int pcav_pool_forward(int B, int T, int C, int E, const uint16_t *H, const uint8_t *nvalid, const int32_t *depth,
+                      const float *emb, uint16_t *Z, void *stream)
+{
+    if (B < 0 || T <= 0 || C <= 0 || E < 0 || C + E > 1024 || !H || !nvalid || !Z || (E && (!depth || !emb)))
+        return PCAGPU_EINVAL;
+    if (B == 0) return PCAGPU_OK;
+    const int threads = ((C + E + 63) / 64) * 64;
+    clear_error();
+    hipLaunchKernelGGL(pool_forward_kernel, dim3(B), dim3(threads), 0, st(stream), T, C, E, H, nvalid, depth, emb,
+                       Z);
+    return status();
+}
+
+int pcav_pool_backward(int B, int T, int C, int E, const float *dZ, const uint8_t *nvalid, const int32_t *depth,
+                       float *dH, float *demb, void *stream)
+{
+    if (B < 0 || T <= 0 || C <= 0 || E < 0 || C + E > 1024 || !dZ || !nvalid || !dH || (E && (!depth || !demb)))
+        return PCAGPU_EINVAL;
+    if (B == 0) return PCAGPU_OK;
+    const int threads = ((C + E + 63) / 64) * 64;
+    clear_error();
+    hipLaunchKernelGGL(pool_backward_kernel, dim3(B), dim3(threads), 0, st(stream), T, C, E, dZ, nvalid, depth, dH,
+                       demb);
+    return status();
+}
+
+int pcav_colsum(int M, int N, const float *G, int ldg, float *out, void *stream)
+{
+    if (M < 0 || N <= 0 || !G || !out) return PCAGPU_EINVAL;
+    clear_error();
+    hipLaunchKernelGGL(fill_f32_kernel, dim3((N + 255) / 256), dim3(256), 0, st(stream), N, out);
+    if (M == 0) return status();
+    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 1023) / 1024));
+    if (grid.y > 65535u) return PCAGPU_EINVAL;
+    clear_error();
+    hipLaunchKernelGGL(colsum_kernel, grid, dim3(256), 0, st(stream), M, N, G, ldg, out);
+    return status();
+}
+
+int pcav_bce(int B, const float *logit, const float *label, float *dlogit, float *loss, void *stream)
+{
+    if (B <= 0 || !logit || !label || !dlogit || !loss) return PCAGPU_EINVAL;
+    clear_error();
+    hipLaunchKernelGGL(bce_kernel, dim3(1), dim3(256), 0, st(stream), B, logit, label, dlogit, loss);
+    return status();
+}
+
+int pcav_adam(int64_t n, float *p, const float *g, float *m, float *v, uint16_t *p16, float lr, float beta1,
+              float beta2, float eps, int step, void *stream)
+{
+    if (n < 0 || !p || !g || !m || !v || step < 1) return PCAGPU_EINVAL;
+    if (n == 0) return PCAGPU_OK;
+    const float c1 = 1.f - powf(beta1, (float)step), c2 = 1.f - powf(beta2, (float)step);
+    clear_error();
+    hipLaunchKernelGGL(adam_kernel, dim3(grid1d(n)), dim3(256), 0, st(stream), n, p, g, m, v, p16, lr, beta1, beta2,
+                       eps, c1, c2);
+    return status();
+}
+
+void *pcav_malloc(int device, size_t bytes)
+{
+    void *p = nullptr;
+    if (hipSetDevice(device) != hipSuccess) return nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 1) != hipSuccess) return nullptr;
+    return p;
+}
+
+void pcav_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+int pcav_copy_to_device(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return PCAGPU_OK;
+    if (!dst || !src) return PCAGPU_EINVAL;
+    return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess ? PCAGPU_OK : PCAGPU_EHIP;
+}
+
+int pcav_copy_to_host(void *dst, const void *src, size_t bytes)
+{
+    if (!bytes) return PCAGPU_OK;
+    if (!dst || !src) return PCAGPU_EINVAL;
+    return hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost) == hipSuccess ? PCAGPU_OK : PCAGPU_EHIP;
+}
+
+int pcav_clear(void *dst, size_t bytes)
+{
+    if (!bytes) return PCAGPU_OK;
+    if (!dst) return PCAGPU_EINVAL;
+    return hipMemset(dst, 0, bytes) == hipSuccess ? PCAGPU_OK : PCAGPU_EHIP;
+}
+
+int pcav_synchronize(void) { return hipDeviceSynchronize() == hipSuccess ? PCAGPU_OK : PCAGPU_EHIP; }
+
+int pcav_to_bf16(int64_t n, const float *x, uint16_t *y16, void *stream)
+{
+    if (n < 0 || !x || !y16) return PCAGPU_EINVAL;
+    if (n == 0) return PCAGPU_OK;
+    clear_error();
+    hipLaunchKernelGGL(to_bf16_kernel, dim3(grid1d(n)), dim3(256), 0, st(stream), n, x, y16);
+    return status();
+}
+
+}  // extern "C"

@@ -210,6 +210,49 @@ def _shallow_draw(rng, max_len, gen_candidates):
     return ins, outs
 
 
+def chain_task(seed, index, length=3):
+    """A task whose generating program is a dependency chain of `length`
+    search-grammar instructions (ALU, LDI, ADDI; no CMP / BR): each reads the
+    register the previous one wrote and the last one's register is the
+    output, so the optimum is rarely much shorter than the source (the
+    training and evaluation tasks of the value network).  1 or 2 inputs,
+    8-32 seeded uniform tests."""
+    from ._lib import gen_candidates
+    rng = Xoshiro256((seed * 6151 + index * 92821 + 5) & M64)
+    while True:
+        n_in = rng.randint(1, 2)
+        live = sum(1 << (i + 1) for i in range(n_in))
+        last = rng.randint(1, n_in)
+        prog = [(23 << 11) | ((i + 1) << 8) | i for i in range(n_in)]
+        for pos in range(length):
+            cands = []
+            for c in gen_candidates(live, pos, length):
+                op, rd, rs, rt = (c >> 11) & 31, (c >> 8) & 7, (c >> 5) & 7, (c >> 2) & 7
+                if op in (13, 19) or rd == 0:
+                    continue
+                reads_last = (op == 18 and rd == last) or (op not in (16, 18) and last in (rs, rt))
+                if reads_last:
+                    cands.append(c)
+            c = cands[rng.next() % len(cands)]
+            prog.append(c)
+            last = (c >> 8) & 7
+            live |= 1 << last
+        prog += [(24 << 11) | (last << 8), 1 << 11]
+        n_tests = rng.randint(8, 32)
+        ins, outs, misses = [], [], 0
+        while len(ins) < n_tests and misses < 256:
+            x = tuple(rng.u16() for _ in range(n_in))
+            y = _interp(prog, x)
+            if y is None:
+                misses += 1
+                continue
+            misses = 0
+            ins.append(x)
+            outs.append(y)
+        if len(ins) == n_tests:
+            return Task.from_io(f"chain_{seed}_{index}", ins, outs, max_cycles=100)
+
+
 def shallow_task(seed, index, max_len=3):
     """A serving request: the outputs of a seeded random program of 1..max_len
     search-grammar instructions (ALU, LDI, ADDI over live registers) on 1 or 2

@@ -140,10 +140,13 @@ def test_oracle_flag_closed_forms(oracle):
 
 # ----------------------------------------------------------- product host
 def test_abi_exports_every_declared_symbol(pca):
-    with open(os.path.join(ROOT, "include", "pcagpu.h")) as f:
-        hdr = f.read()
-    names = set(re.findall(r"\b(pcagpu_[a-z_0-9]+)\s*\(", hdr))
-    assert len(names) >= 15
+    hdr = ""
+    for h in sorted(os.listdir(os.path.join(ROOT, "include"))):
+        if h.endswith(".h"):
+            with open(os.path.join(ROOT, "include", h)) as f:
+                hdr += f.read()
+    names = set(re.findall(r"\b((?:pcagpu|pcav)_[a-z_0-9]+)\s*\(", hdr))
+    assert len(names) >= 25
     lib = C.CDLL(os.path.join(ROOT, "pca_amd", "libpcagpu.so"))
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
@@ -272,3 +275,38 @@ def test_packed_step_matches_step_test(tmp_path):
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stdout + out.stderr
     assert out.stdout.startswith("ok ")
+
+
+def test_value_oracle_gradients():
+    """The value-network oracle's backward pass (float64, no bf16 rounding)
+    against central finite differences of its own loss, on a small batch:
+    pins the oracle the GPU kernels are checked against."""
+    import numpy as np
+    from oracle import value_oracle as vo
+    rng = np.random.default_rng(3)
+    n = 180 * 512 + 512 + 512 * 256 + 256 + 9 * 16 + 272 * 256 + 256 + 256 + 1
+    p = rng.normal(0, 0.05, n)
+    B = 3
+    rec = rng.integers(0, 1 << 16, size=(B, 32, 16)).astype(np.uint16)
+    rec[..., 11] &= 15
+    rec[..., 12:] = 0
+    X = vo.features(rec)
+    nvalid = [32, 7, 19]
+    depth = [0, 3, 8]
+    label = [1.0, 0.0, 1.0]
+    _, _, g = vo.forward_backward(p, X, nvalid, depth, label, bf16_mode=False)
+    idx = np.concatenate([rng.integers(0, n, 150), [n - 1, n - 257, 180 * 512 + 3, n - 300]])
+    h = 1e-6
+    for i in idx:
+        pp, pm = p.copy(), p.copy()
+        pp[i] += h
+        pm[i] -= h
+        lp = vo.forward_backward(pp, X, nvalid, depth, label, bf16_mode=False)[1]
+        lm = vo.forward_backward(pm, X, nvalid, depth, label, bf16_mode=False)[1]
+        fd = (lp - lm) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+    # features: bit f of word f // 16, least significant first, flags last
+    assert X[0, 3] == (rec[0, 0, 0] >> 3) & 1
+    assert X[1, 16 + 5] == (rec[0, 1, 1] >> 5) & 1
+    assert X[2, 48 + 16 * 7 + 15] == (rec[0, 2, 10] >> 15) & 1
+    assert X[0, 178] == (rec[0, 0, 11] >> 2) & 1

@@ -1,0 +1,176 @@
+"""Value-network kernels (include/pcavalue.h) on the GPU against numpy and
+the CPU oracle (oracle/value_oracle.py), training, and the guided search
+against the same search without the network."""
+
+import numpy as np
+import pytest
+
+
+pytestmark = pytest.mark.gpu
+
+from oracle import value_oracle as vo  # noqa: E402
+
+
+def _bf16_bits(x):
+    """float array -> (bf16 bit patterns as int16, their float64 values)."""
+    a = np.asarray(x, dtype=np.float32)
+    u = a.view(np.uint32).astype(np.uint64)
+    u = ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+    return u.view(np.int16), (u.astype(np.uint32) << 16).view(np.float32).astype(np.float64)
+
+
+@pytest.mark.parametrize("M,N,K", [(70, 45, 37), (128, 64, 96), (1, 1, 1), (200, 300, 180), (33, 1, 512)])
+@pytest.mark.parametrize("at,bt", [(0, 0), (1, 0), (0, 1), (1, 1)])
+def test_gemm_matches_numpy(pca, M, N, K, at, bt):
+    from pca_amd._lib import lib
+    from pca_amd.value import DeviceArray
+    rng = np.random.default_rng(M * 7 + N * 3 + K + 11 * at + 5 * bt)
+    a = rng.normal(0, 1, (K, M) if at else (M, K))
+    b = rng.normal(0, 1, (N, K) if bt else (K, N))
+    bias = rng.normal(0, 1, N).astype(np.float32)
+    a16, af = _bf16_bits(a)
+    b16, bf = _bf16_bits(b)
+    A = DeviceArray(a16.shape, np.int16)
+    A.upload(a16)
+    Bm = DeviceArray(b16.shape, np.int16)
+    Bm.upload(b16)
+    bias_d = DeviceArray(N, np.float32)
+    bias_d.upload(bias)
+    C32 = DeviceArray((M, N), np.float32)
+    C32.upload(np.full((M, N), 3.0, dtype=np.float32))
+    C16 = DeviceArray((M, N), np.uint16)
+    C16.zero()
+    print(f"[gemm] M={M} N={N} K={K} at={at} bt={bt}", flush=True)
+    rc = lib().pcav_gemm(M, N, K, A.ptr, M if at else K, at, Bm.ptr, K if bt else N, bt, bias_d.ptr, 1, C32.ptr, N, 1,
+                         C16.ptr, N, None)
+    assert rc == 0
+    assert lib().pcav_synchronize() == 0
+    ref = (af.T if at else af) @ (bf.T if bt else bf) + bias
+    ref = np.maximum(ref, 0)
+    got32 = C32.numpy().astype(np.float64)
+    assert np.allclose(got32, 3.0 + ref, rtol=1e-4, atol=1e-3 * np.sqrt(K))
+    got16 = (C16.numpy().astype(np.uint32) << 16).view(np.float32)
+    assert np.allclose(got16, ref, rtol=1e-2, atol=1e-2)
+
+
+def _records(pca, names=("abs", "sign", "is_power2_32", "max")):
+    from pca_amd.value import state_records
+    from pca_amd._lib import gen_candidates
+    from conftest import golden_task_path
+    recs, nvalid, depth = [], [], []
+    for name in names:
+        t = pca.Task.load(golden_task_path(name))
+        live = sum(1 << (i + 1) for i in range(t.num_inputs))
+        progs = [[c] for c in gen_candidates(live, 0, 3)[:5]] + [[]]
+        r, _ = state_records(t, progs)
+        recs.append(r)
+        nvalid += [t.num_tests] * len(progs)
+        depth += [len(p) for p in progs]
+    return np.concatenate(recs), np.array(nvalid, dtype=np.uint8), np.array(depth, dtype=np.int32)
+
+
+def test_features_match_oracle(pca):
+    from pca_amd.value import ValueNet
+    rec, nvalid, depth = _records(pca)
+    net = ValueNet(seed=1)
+    X, _, _ = net.upload(rec, nvalid, depth)
+    got = (X.numpy().astype(np.uint32) << 16).view(np.float32)
+    assert np.array_equal(got, vo.features(rec).astype(np.float32))
+
+
+def test_forward_backward_match_oracle(pca):
+    """Logits, loss and every parameter gradient of one batch against the
+    oracle rounding at the same BF16 points (FP32 accumulation order is the
+    only difference)."""
+    from pca_amd.value import ValueNet
+    rec, nvalid, depth = _records(pca)
+    B = len(nvalid)
+    label = (np.arange(B) % 3 == 0).astype(np.float32)
+    net = ValueNet(seed=2)
+    X, nv, dp = net.upload(rec, nvalid, depth)
+    y = net.forward(X, nv, dp).numpy().reshape(-1)
+    loss = net.backward(label)
+    p = net.parameters()
+    y_ref, loss_ref, g_ref = vo.forward_backward(p, vo.features(rec), nvalid, depth, label, bf16_mode=True)
+    print(f"[value] B={B} loss gpu {loss:.6f} oracle {loss_ref:.6f}", flush=True)
+    assert np.allclose(y, y_ref, rtol=2e-2, atol=2e-2)
+    assert abs(loss - loss_ref) < 1e-2
+    g = net.gradients().astype(np.float64)
+    off = 0
+    for name, shape in net.LAYOUT:
+        n = int(np.prod(shape))
+        a, b = g[off:off + n], g_ref[off:off + n]
+        rel = np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-12)
+        print(f"  grad {name}: |ref| {np.linalg.norm(b):.3e} rel err {rel:.2e}", flush=True)
+        assert rel < 3e-2, name
+        off += n
+
+
+def test_training_reduces_loss_and_guides_search(pca):
+    """Train on labelled states of exhaustively solved tasks, then compare
+    guided best-first search with the same search ordered breadth-first
+    (no network) on held-out tasks, both with a budget of 500 expansions:
+    the network must find more verified optimal programs, and fewer
+    expansions on the tasks both find; the exhaustive solver's optimum
+    bounds every length found."""
+    from pca_amd import taskgen
+    from pca_amd.value import ValueNet, labelled_states, state_records, guided_search, verify_program
+
+    def solved(seed, n, length):
+        out = []
+        for i in range(n):
+            t = taskgen.chain_task(seed, i, length)
+            with pca.Solver(t) as sv:
+                r = sv.solve(length, cap=4096)
+            if r.optimal_depth >= 2:
+                out.append((t, r.optimal_depth, [list(s[0]) for s in r.solutions]))
+        return out
+
+    import time
+    t0 = time.time()
+    train = solved(101, 1200, 3)
+    progs, depths, labels, tasks_idx = [], [], [], []
+    recs, nvalid = [], []
+    for t, d, sols in train:
+        # solutions come back as full programs; keep the kernel part
+        kernels = [s[t.num_inputs:t.num_inputs + d] for s in sols]
+        p, dp, lb = labelled_states(t, d, kernels, negatives_per_node=24, seed=len(recs))
+        r, _ = state_records(t, p)
+        recs.append(r)
+        nvalid += [t.num_tests] * len(p)
+        depths += dp
+        labels += lb
+    rec = np.concatenate(recs)
+    nvalid = np.array(nvalid, dtype=np.uint8)
+    depths = np.array(depths, dtype=np.int32)
+    labels = np.array(labels, dtype=np.float32)
+    print(f"[train] tasks {len(train)} states {len(labels)} positive {int(labels.sum())} "
+          f"data {time.time() - t0:.1f}s", flush=True)
+    net = ValueNet(seed=0)
+    t0 = time.time()
+    losses = net.train(rec, nvalid, depths, labels, steps=1500, batch=512, lr=1e-3)
+    print(f"[train] loss {losses[0]:.3f} -> {losses[-1]:.3f} in {time.time() - t0:.1f}s", flush=True)
+    assert losses[-1] < 0.7 * losses[0]
+
+    held = solved(202, 40, 3)
+    exp_guided, exp_plain, found_g, found_p = 0, 0, 0, 0
+    t0 = time.time()
+    for t, d, _ in held:
+        g, eg = guided_search(t, d, net=net, budget=500)
+        pl, ep = guided_search(t, d, net=None, budget=500)
+        print(f"  d*={d} guided {eg} {'found' if g else '-'} plain {ep} {'found' if pl else '-'}", flush=True)
+        if g is not None and pl is not None:     # expansions on the tasks both searches solve
+            exp_guided += eg
+            exp_plain += ep
+        for res in (g, pl):
+            if res is not None:
+                prog, reg = res
+                assert verify_program(t, prog, reg)
+                assert len(prog) >= d          # the exhaustive optimum is a lower bound
+        found_g += g is not None and len(g[0]) == d
+        found_p += pl is not None and len(pl[0]) == d
+    print(f"[guided] held-out {len(held)}: optimal found guided {found_g} plain {found_p}, "
+          f"expansions where both found one: guided {exp_guided} plain {exp_plain} ({time.time() - t0:.1f}s)",
+          flush=True)
+    assert found_g > found_p
+    assert exp_guided * 2 < exp_plain
