@@ -95,6 +95,57 @@ def suite_bench(n_tasks, budget, seed=11):
             "candidates": st["candidates"], "same_results": a == b}
 
 
+def value_bench(batch=512, steps=30, seed=5):
+    """Value network on the GPU (BF16 MFMA kernels): training and inference
+    throughput in search states per second on synthetic states (random
+    register files of is_power2_32's tests), and the layer-1 forward GEMM
+    (M = 32 tests x batch, N = 512, K = 180) against the dense BF16 MFMA
+    peak."""
+    import numpy as np
+    from pca_amd import taskgen
+    from pca_amd._lib import lib
+    from pca_amd.value import ValueNet, state_records, PER_TEST, H1, MAX_T
+    rng = np.random.default_rng(seed)
+    task = taskgen.is_power2_32(42)
+    rec, _ = state_records(task, [[]] * batch)
+    rec[:, :task.num_tests, 3:11] = rng.integers(0, 1 << 16, size=(batch, task.num_tests, 8))
+    nvalid = np.full(batch, task.num_tests, dtype=np.uint8)
+    depth = rng.integers(0, 8, size=batch).astype(np.int32)
+    label = (rng.random(batch) < 0.3).astype(np.float32)
+    net = ValueNet(seed=seed)
+    X, nv, dp = net.upload(rec, nvalid, depth)
+
+    def timed(fn, n):
+        fn()
+        lib().pcav_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(n):
+            fn()
+        lib().pcav_synchronize()
+        return (time.perf_counter() - t0) / n
+
+    def train_step():
+        net.forward(X, nv, dp)
+        net.backward(label)
+        net.adam_step()
+
+    t_train = timed(train_step, steps)
+    t_infer = timed(lambda: net.forward(X, nv, dp), steps)
+    # the layer-1 GEMM at a large batch (4096 states: M = 131072 rows)
+    big = 4096
+    rec_b = np.repeat(rec, big // batch, axis=0)
+    Xb, _, _ = net.upload(rec_b, np.repeat(nvalid, big // batch), np.repeat(depth, big // batch))
+    h1 = net._buf("h1", (big * MAX_T, H1), np.uint16)
+    t_gemm = timed(lambda: net._gemm(big * MAX_T, H1, PER_TEST, Xb, PER_TEST, 0, net.v16["W1"], H1, 0,
+                                     bias=net.views["b1"], relu=True, C16=h1, ldc16=H1), 50)
+    flops = 2.0 * big * MAX_T * H1 * PER_TEST
+    return {"batch_states": batch, "train_states_per_s": batch / t_train, "infer_states_per_s": batch / t_infer,
+            "gemm_layer1": {"M": big * MAX_T, "N": H1, "K": PER_TEST, "us": t_gemm * 1e6,
+                            "tflops": flops / t_gemm / 1e12, "peak_tflops": 2500.0,
+                            "frac": flops / t_gemm / 2.5e15},
+            "params": net.n_params, "timing": "host clock around device-synchronised repetitions"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -108,6 +159,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--suite", type=int, default=1000, help="config-3 suite size (0: skip)")
+    ap.add_argument("--no-value", action="store_true")
     ap.add_argument("--suite-budget", type=int, default=3)
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
@@ -201,6 +253,9 @@ def main():
         line["serving"] = serving_bench(n_req=args.serving_requests)
     if args.suite:
         line["suite"] = suite_bench(args.suite, args.suite_budget)
+    if not args.no_value:
+        line["value_net"] = value_bench()
+        print("value net done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         cb = cpu_baseline(task_path, args.cpu_seconds)
         line["cpu_baseline"] = cb

@@ -1,14 +1,16 @@
 // value.hip -- BF16 MFMA kernels of the value network (include/pcavalue.h).
 //
-// GEMM: one 256-thread workgroup computes a 64 x 64 tile of C as four
-// 32 x 32 wave tiles with v_mfma_f32_32x32x16_bf16 (gfx950), FP32
-// accumulators in registers.  Each K step stages a 64 x 32 slab of A and of
-// B in LDS, both k-contiguous per row/column so that a lane's MFMA operand
-// (8 consecutive k of one row of A or one column of B) is one 16-byte LDS
-// read; the transposed storage modes only change how the slab is gathered
-// from HBM.  Out-of-range rows, columns and k are zero-filled, so any
-// M, N, K works.  The epilogue adds the bias, applies ReLU and stores FP32
-// and/or BF16.
+// GEMM: one 256-thread workgroup computes a 128 x 128 tile of C, each wave
+// 64 x 64 as 2 x 2 blocks of v_mfma_f32_32x32x16_bf16 (gfx950), FP32
+// accumulators in registers.  Each K step stages a 128 x 32 slab of A and
+// of B in LDS, both k-contiguous per row/column so that a lane's MFMA
+// operand (8 consecutive k of one row of A or one column of B) is one
+// 16-byte LDS read; the slab is gathered from HBM in 8-byte pieces along
+// whichever dimension is contiguous in memory (the transposed storage modes
+// only change that).  Out-of-range rows, columns and k are zero-filled, so
+// any M, N, K works.  Few output tiles with a long K (weight gradients) are
+// split along K across workgroups with atomic FP32 adds.  The epilogue adds
+// the bias, applies ReLU and stores FP32 and/or BF16.
 //
 // Operand layout of v_mfma_f32_32x32x16_bf16 (lane l of 64): A: row l % 32,
 // k = 8 * (l / 32) .. +7; B: column l % 32, the same k; the 16 accumulators
@@ -26,7 +28,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-constexpr int GM = 64, GN = 64, GK = 32;
+constexpr int GM = 128, GN = 128, GK = 32;
 constexpr int LDS_K = GK + 8;   // row stride in halves: 80 bytes, 16-byte aligned rows
 
 __device__ __forceinline__ uint16_t to_bf16(float x)
@@ -38,8 +40,58 @@ __device__ __forceinline__ uint16_t to_bf16(float x)
 
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const uint16_t *__restrict__ A, int lda,
-                                                   int a_trans, const uint16_t *__restrict__ B, int ldb, int b_trans,
+// Four consecutive elements p[0..3] of a row that holds `avail` more valid
+// elements from p: one 8-byte load when all four are valid and p is 8-byte
+// aligned, else element by element (zero past the end).
+__device__ __forceinline__ void load4(const uint16_t *p, int avail, uint16_t v[4])
+{
+    if (avail >= 4 && (reinterpret_cast<uintptr_t>(p) & 7u) == 0) {
+        const uint2 q = *reinterpret_cast<const uint2 *>(p);
+        v[0] = (uint16_t)q.x;
+        v[1] = (uint16_t)(q.x >> 16);
+        v[2] = (uint16_t)q.y;
+        v[3] = (uint16_t)(q.y >> 16);
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++) v[j] = j < avail ? p[j] : (uint16_t)0;
+    }
+}
+
+// Stages a 128 x 32 slab into S[row][k] (k contiguous) from a matrix whose
+// storage is KCONTIG: X[row * ld + k] (k contiguous in memory) or else
+// X[k * ld + row] (rows contiguous); rows >= R or k >= K read as zero.
+template <bool KCONTIG>
+__device__ __forceinline__ void stage(uint16_t (*S)[LDS_K], const uint16_t *X, int ld, int r0, int R, int k0, int K,
+                                      int tid)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const int c = tid + i * 256;   // 1024 chunks of 4 elements
+        uint16_t v[4];
+        if (KCONTIG) {
+            const int r = c >> 3, kc = (c & 7) * 4;
+            const int gr = r0 + r, gk = k0 + kc;
+            if (gr < R && gk < K) load4(X + (int64_t)gr * ld + gk, K - gk, v);
+            else v[0] = v[1] = v[2] = v[3] = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) S[r][kc + j] = v[j];
+        } else {
+            const int k = c >> 5, rc = (c & 31) * 4;
+            const int gr = r0 + rc, gk = k0 + k;
+            if (gr < R && gk < K) load4(X + (int64_t)gk * ld + gr, R - gr, v);
+            else v[0] = v[1] = v[2] = v[3] = 0;
+#pragma unroll
+            for (int j = 0; j < 4; j++) S[rc + j][k] = v[j];
+        }
+    }
+}
+
+// C tile of 128 x 128 per workgroup, 64 x 64 per wave (2 x 2 MFMA blocks);
+// blockIdx.z selects a K range (split K: partial sums added atomically).
+template <bool AT, bool BT>
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_per_split,
+                                                   const uint16_t *__restrict__ A, int lda,
+                                                   const uint16_t *__restrict__ B, int ldb,
                                                    const float *__restrict__ bias, int relu, float *C32, int ldc,
                                                    int accumulate, uint16_t *C16, int ldc16)
 {
@@ -48,53 +100,63 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, const ui
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
     const int m0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
-    f32x16 acc = {};
-    for (int k0 = 0; k0 < K; k0 += GK) {
-        // stage A (64 rows x 32 k) and B (64 columns x 32 k): 8 elements per thread each
-        for (int e = tid; e < GM * GK; e += 256) {
-            int r, k;
-            if (a_trans) { r = e % GM; k = e / GM; }   // consecutive threads along m: contiguous in A
-            else { r = e / GK; k = e % GK; }           // consecutive threads along k: contiguous in A
-            const int gm = m0 + r, gk = k0 + k;
-            uint16_t v = 0;
-            if (gm < M && gk < K) v = a_trans ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
-            As[r][k] = v;
-        }
-        for (int e = tid; e < GN * GK; e += 256) {
-            int c, k;
-            if (b_trans) { c = e / GK; k = e % GK; }
-            else { c = e % GN; k = e / GN; }
-            const int gn = n0 + c, gk = k0 + k;
-            uint16_t v = 0;
-            if (gn < N && gk < K) v = b_trans ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
-            Bs[c][k] = v;
-        }
+    const int kz0 = blockIdx.z * k_per_split;
+    const int kz1 = min(K, kz0 + k_per_split);
+    const bool split = gridDim.z > 1;
+    f32x16 acc[2][2] = {};
+    for (int k0 = kz0; k0 < kz1; k0 += GK) {
+        stage<!AT>(As, A, lda, m0, M, k0, kz1, tid);   // A: M x K, k contiguous unless transposed
+        stage<BT>(Bs, B, ldb, n0, N, k0, kz1, tid);    // B: K x N, n contiguous unless transposed
         __syncthreads();
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
             const int ko = kk + 8 * (lane >> 5);
-            bf16x8 a, b;
-            __builtin_memcpy(&a, &As[wm * 32 + (lane & 31)][ko], 16);
-            __builtin_memcpy(&b, &Bs[wn * 32 + (lane & 31)][ko], 16);
-            acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                __builtin_memcpy(&a[i], &As[wm * 64 + i * 32 + (lane & 31)][ko], 16);
+                __builtin_memcpy(&b[i], &Bs[wn * 64 + i * 32 + (lane & 31)][ko], 16);
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
-    const int col = n0 + wn * 32 + (lane & 31);
-    if (col >= N) return;
-    const float bv = bias ? bias[col] : 0.f;
 #pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int row = m0 + wm * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-        if (row >= M) continue;
-        float v = acc[r] + bv;
-        if (relu) v = v > 0.f ? v : 0.f;
-        if (C32) {
-            float *p = C32 + (int64_t)row * ldc + col;
-            *p = accumulate ? *p + v : v;
-        }
-        if (C16) C16[(int64_t)row * ldc16 + col] = to_bf16(v);
+    for (int j = 0; j < 2; j++) {
+        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
+        if (col >= N) continue;
+        const float bv = bias ? bias[col] : 0.f;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
+                if (row >= M) continue;
+                float v = acc[i][j][r];
+                if (split) {
+                    atomicAdd(C32 + (int64_t)row * ldc + col, v);
+                    continue;
+                }
+                v += bv;
+                if (relu) v = v > 0.f ? v : 0.f;
+                if (C32) {
+                    float *p = C32 + (int64_t)row * ldc + col;
+                    *p = accumulate ? *p + v : v;
+                }
+                if (C16) C16[(int64_t)row * ldc16 + col] = to_bf16(v);
+            }
     }
+}
+
+// zeroes rows x cols of a row-major FP32 matrix with leading dimension ld
+__global__ void fill_rows_kernel(int rows, int cols, int ld, float *p)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < (int64_t)rows * cols) p[(i / cols) * ld + (i % cols)] = 0.f;
 }
 
 __global__ void relu_backward_kernel(int64_t n, const uint16_t *act, float *grad, uint16_t *out)
@@ -227,11 +289,32 @@ int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, cons
 {
     if (M < 0 || N < 0 || K < 0 || !A || !B || (!C32 && !C16)) return PCAGPU_EINVAL;
     if (M == 0 || N == 0) return PCAGPU_OK;
-    const dim3 grid((unsigned)((N + GN - 1) / GN), (unsigned)((M + GM - 1) / GM));
-    if (grid.y > 65535u) return PCAGPU_EINVAL;
+    const int tiles_m = (M + GM - 1) / GM, tiles_n = (N + GN - 1) / GN;
+    if (tiles_m > 65535) return PCAGPU_EINVAL;
+    // split K when few output tiles face a long K (weight gradients): the
+    // partial sums go to C32 atomically, so only a plain FP32 output can
+    // take it
+    int splits = 1;
+    const int tiles = tiles_m * tiles_n;
+    if (C32 && !C16 && !bias && !relu && tiles < 512 && K >= 2048)
+        splits = std::min(std::min((K + 1023) / 1024, (1024 + tiles - 1) / tiles), 64);
+    const int kps = ((K + splits - 1) / splits + GK - 1) / GK * GK;
+    splits = std::max(1, (K + kps - 1) / kps);
     clear_error();
-    hipLaunchKernelGGL(gemm_kernel, grid, dim3(256), 0, st(stream), M, N, K, A, lda, a_trans, B, ldb, b_trans, bias,
-                       relu, C32, ldc, accumulate, C16, ldc16);
+    if (splits > 1 && !accumulate) {
+        const int64_t n = (int64_t)M * N;
+        hipLaunchKernelGGL(fill_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st(stream), M, N, ldc,
+                           C32);
+    }
+    const dim3 grid((unsigned)tiles_n, (unsigned)tiles_m, (unsigned)splits);
+#define PCAV_GEMM(AT, BT)                                                                                       \
+    hipLaunchKernelGGL((gemm_kernel<AT, BT>), grid, dim3(256), 0, st(stream), M, N, K, kps, A, lda, B, ldb, bias, \
+                       relu, C32, ldc, accumulate, C16, ldc16)
+    if (a_trans && b_trans) PCAV_GEMM(true, true);
+    else if (a_trans) PCAV_GEMM(true, false);
+    else if (b_trans) PCAV_GEMM(false, true);
+    else PCAV_GEMM(false, false);
+#undef PCAV_GEMM
     return status();
 }
 

@@ -53,6 +53,29 @@ def test_gemm_matches_numpy(pca, M, N, K, at, bt):
     assert np.allclose(got16, ref, rtol=1e-2, atol=1e-2)
 
 
+@pytest.mark.parametrize("M,N,K", [(180, 512, 8192), (256, 1, 4096), (272, 256, 3000)])
+@pytest.mark.parametrize("acc", [0, 1])
+def test_gemm_split_k_matches_numpy(pca, M, N, K, acc):
+    """Weight-gradient shape (A stored K x M, long K): the K range is split
+    across workgroups and the partial sums added atomically."""
+    from pca_amd._lib import lib
+    from pca_amd.value import DeviceArray
+    rng = np.random.default_rng(M + N + K + acc)
+    a16, af = _bf16_bits(rng.normal(0, 1, (K, M)))
+    b16, bf = _bf16_bits(rng.normal(0, 1, (K, N)))
+    A = DeviceArray(a16.shape, np.int16)
+    A.upload(a16)
+    Bm = DeviceArray(b16.shape, np.int16)
+    Bm.upload(b16)
+    C32 = DeviceArray((M, N), np.float32)
+    C32.upload(np.full((M, N), 2.0, dtype=np.float32))
+    print(f"[gemm-splitk] M={M} N={N} K={K} acc={acc}", flush=True)
+    assert lib().pcav_gemm(M, N, K, A.ptr, M, 1, Bm.ptr, N, 0, None, 0, C32.ptr, N, acc, None, 0, None) == 0
+    assert lib().pcav_synchronize() == 0
+    ref = af.T @ bf + (2.0 if acc else 0.0)
+    assert np.allclose(C32.numpy(), ref, rtol=1e-4, atol=1e-3 * np.sqrt(K))
+
+
 def _records(pca, names=("abs", "sign", "is_power2_32", "max")):
     from pca_amd.value import state_records
     from pca_amd._lib import gen_candidates
