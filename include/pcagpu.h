@@ -39,6 +39,8 @@ extern "C" {
                                       key: the exact path refuses to guess   */
 #define PCAGPU_ENODEV         -7   /* no GPU visible                          */
 #define PCAGPU_EIO            -8   /* task file unreadable                    */
+#define PCAGPU_EVERIFY        -9   /* a device result failed host
+                                      re-verification (an internal error)    */
 
 /* Task record, byte-compatible with task_spec_t (tasks/spec.h:20-34). */
 typedef struct {
@@ -231,6 +233,54 @@ typedef struct {
  * depth.  Returns 0 or an error. */
 int  pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *opts, pcagpu_result_t *res,
                   pca_solution_t *out, long cap);
+
+/* ---- exhaustive leaf completion (src/search_state.c:339-453) ----
+ * The branchless search state of the reference's learned-search loop and
+ * the brute-force completion it calls near the depth budget
+ * (python/bootstrap.py:85-113 via python/search_eval.py:171-181). */
+
+/* Byte-compatible with search_state_t (src/search_state.h:19-25, 546 B):
+ * per-test registers r0..r7 and flags, the live mask (bit i: r_i is an input
+ * or was written), and the kernel instructions executed so far. */
+typedef struct {
+    uint16_t regs[PCAGPU_MAX_TESTS][8];
+    uint8_t  flags[PCAGPU_MAX_TESTS];
+    uint8_t  live;
+    uint8_t  depth;
+} pca_search_state_t;
+
+/* src/search_state.c:34-46 search_state_init_root: inputs in r1..r_n. */
+void    pcagpu_state_init_root(const pca_task_t *task, pca_search_state_t *s);
+/* src/search_state.c:49-187 search_state_step: 1 with *dst the state after
+ * insn on every test, or 0 (dst unspecified) if insn is not one of ADD..ASR,
+ * LDI, ADDI or a DIV/MOD divides by zero on some test. */
+int     pcagpu_state_step(const pca_task_t *task, const pca_search_state_t *src, uint16_t insn,
+                          pca_search_state_t *dst);
+/* src/search_state.c:223-239 search_state_leaf_outputs: mask of live
+ * registers r1..r7 equal to the target on every test. */
+uint8_t pcagpu_state_leaf_outputs(const pca_task_t *task, const pca_search_state_t *s);
+/* src/search_state.c:280-330 search_gen_candidates_branchless: the ordered
+ * candidate list of a state with live mask `live`. */
+int     pcagpu_gen_candidates_branchless(uint16_t *out, int max, uint8_t live);
+
+/* src/search_state.c:418-445 search_exhaustive_complete on h's device (the
+ * handle's task is the reference's search_ctx_t): the shortest completion
+ * of `start` by at most max_depth - start->depth branchless instructions,
+ * and among those the first in the reference's DFS order, with its OEP
+ * (first visitor of each register/flag state per depth, live mask ignored)
+ * on exact keys.  Returns 1 and the instructions (*solution_len of them,
+ * 0 if start already solves; solution_insns holds PCAGPU_MAX_KDEPTH), 0 if
+ * there is none, or an error: PCAGPU_EINVAL also when max_depth -
+ * start->depth exceeds PCAGPU_MAX_KDEPTH (the reference clamps at 16).  The
+ * reference's 32-bit OEP table can prune a distinct state on a hash
+ * collision or stop pruning when its probe window fills; exact keys do
+ * neither, so results differ from it only where that happens. */
+int     pcagpu_complete(pcagpu_t *h, const pca_search_state_t *start, int max_depth,
+                        uint16_t *solution_insns, int *solution_len);
+/* src/search_state.c:447-450 search_exhaustive_states_explored: children
+ * generated by the last pcagpu_complete on h (every level's full child
+ * list, so not the reference's DFS count, which stops at the first hit). */
+uint64_t pcagpu_complete_states_explored(pcagpu_t *h);
 
 /* ---- continuous-batching serving (the request loop above the path) ----
  * No reference counterpart: the reference solves one task per process.

@@ -26,6 +26,12 @@ class OrTask(C.Structure):
                 ("tests", OrTest * 32), ("num_tests", C.c_int)]
 
 
+class OrState(C.Structure):
+    """search_state_t (src/search_state.h:19-25), 546 bytes."""
+    _fields_ = [("regs", (C.c_uint16 * 8) * 32), ("flags", C.c_uint8 * 32), ("live", C.c_uint8),
+                ("depth", C.c_uint8)]
+
+
 class OrOpts(C.Structure):
     _fields_ = [("use_branches", C.c_int), ("oep_mode", C.c_int), ("oep_bits", C.c_int),
                 ("max_solutions", C.c_int), ("start_depth", C.c_int), ("max_depth", C.c_int),
@@ -58,6 +64,12 @@ def lib():
         L.or_task_parse.argtypes = [C.c_char_p, C.POINTER(OrTask)]
         L.or_task_evaluate.argtypes = [C.POINTER(OrTask), C.POINTER(C.c_uint16), C.c_int]
         L.or_disasm.argtypes = [C.c_uint16, C.c_char_p, C.c_size_t]
+        L.or_state_root.argtypes = [C.POINTER(OrTask), C.POINTER(OrState)]
+        L.or_state_step.argtypes = [C.POINTER(OrTask), C.POINTER(OrState), C.c_uint16, C.POINTER(OrState)]
+        L.or_state_leaf.argtypes = [C.POINTER(OrTask), C.POINTER(OrState)]
+        L.or_state_leaf.restype = C.c_uint8
+        L.or_complete.argtypes = [C.POINTER(OrTask), C.POINTER(OrState), C.c_int, C.c_int, C.POINTER(C.c_uint16),
+                                  C.POINTER(C.c_int), C.POINTER(C.c_uint64)]
         L.or_disasm.restype = None
         L.or_gen_candidates.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_uint8, C.c_int, C.c_int, C.c_int]
         L.or_gen_candidates_branchless.argtypes = [C.POINTER(C.c_uint16), C.c_int, C.c_uint8]
@@ -180,3 +192,40 @@ def replay_state(task, kernel):
     out = (C.c_uint16 * (9 * 32))()
     lib().or_replay_state(C.byref(task), arr, len(kernel), out)
     return [tuple(out[t * 9:(t + 1) * 9]) for t in range(task.num_tests)]
+
+
+def state_root(task):
+    st = OrState()
+    lib().or_state_root(C.byref(task), C.byref(st))
+    return st
+
+
+def state_step(task, st, insn):
+    """The child state, or None if the step is rejected."""
+    out = OrState()
+    return out if lib().or_state_step(C.byref(task), C.byref(st), insn, C.byref(out)) else None
+
+
+def state_leaf(task, st):
+    return lib().or_state_leaf(C.byref(task), C.byref(st))
+
+
+def state_after(task, insns):
+    """Root state stepped through insns (None if a step is rejected)."""
+    st = state_root(task)
+    for c in insns:
+        st = state_step(task, st, c)
+        if st is None:
+            return None
+    return st
+
+
+def complete(task, st, max_depth, oep=OEP_EXACT):
+    """search_exhaustive_complete: (found, insns, explored)."""
+    ins = (C.c_uint16 * 16)()
+    n = C.c_int()
+    ex = C.c_uint64()
+    rc = lib().or_complete(C.byref(task), C.byref(st), max_depth, oep, ins, C.byref(n), C.byref(ex))
+    if rc < 0:
+        raise MemoryError("or_complete")
+    return bool(rc), list(ins[:n.value]), ex.value

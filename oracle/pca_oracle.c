@@ -1180,3 +1180,112 @@ uint64_t or_alu_digest(int op, uint32_t b_lo, uint32_t b_hi)
         }
     return h;
 }
+
+/* ------------------------------------------------- exhaustive completion */
+/* src/search_state.c:223-239: the live registers r1..r7 equal to the target
+ * on every test. */
+static uint8_t ss_leaf(const or_task_t *t, const ss_t *st)
+{
+    uint8_t out = 0;
+    for (int r = 1; r < 8; r++) {
+        if (!(st->live & (1u << r))) continue;
+        int ok = 1;
+        for (int k = 0; k < t->num_tests && ok; k++)
+            if (st->regs[k][r] != t->tests[k].out[0]) ok = 0;
+        if (ok) out |= (uint8_t)(1u << r);
+    }
+    return out;
+}
+
+/* src/search_state.c:372-415: DFS to `target` appended instructions; the
+ * leaf check runs on every child, the OEP table of the parent position on
+ * children that still have room below them. */
+static int complete_dfs(solver_t *s, const ss_t *st, int pos, int target, uint16_t *path)
+{
+    const or_task_t *t = s->task;
+    uint16_t cands[4096];
+    uint8_t key[OR_MAX_TESTS * 17];
+    int nc = or_gen_candidates_branchless(cands, 4096, st->live);
+    for (int i = 0; i < nc; i++) {
+        ss_t child;
+        if (!ss_step(t, st, cands[i], &child)) continue;
+        s->tried++;
+        if (ss_leaf(t, &child)) {
+            path[pos] = cands[i];
+            return pos + 1;
+        }
+        if (pos + 1 >= target) continue;
+        ss_key(t, &child, key);
+        uint64_t h64 = or_fnv64_bytes(0xCBF29CE484222325ULL, key, (size_t)t->num_tests * 17);
+        if (oep_check(s, pos, (uint32_t)h64, key)) continue;
+        path[pos] = cands[i];
+        int len = complete_dfs(s, &child, pos + 1, target, path);
+        if (len > 0) return len;
+    }
+    return 0;
+}
+
+/* src/search_state.c:418-445 */
+int or_complete(const or_task_t *task, const or_state_t *start, int max_depth, int oep_mode,
+                uint16_t *insns, int *len, uint64_t *explored)
+{
+    *len = 0;
+    if (explored) *explored = 0;
+    ss_t root;
+    memcpy(&root, start, sizeof(root));
+    if (ss_leaf(task, &root)) return 1;
+    int max_r = max_depth - (int)root.depth;
+    if (max_r <= 0) return 0;
+    if (max_r > 16) max_r = 16;
+    or_opts_t o;
+    or_opts_default(&o);
+    o.oep_mode = oep_mode;
+    o.oep_bits = 20;
+    or_result_t res;
+    memset(&res, 0, sizeof(res));
+    solver_t *s = calloc(1, sizeof(solver_t));
+    if (!s) return -1;
+    solver_setup(s, task, &o, &res);
+    uint16_t path[16];
+    int found = 0;
+    for (int target = 1; target <= max_r && !found; target++) {
+        clear_tables(s);
+        int n = complete_dfs(s, &root, 0, target, path);
+        if (n > 0) {
+            memcpy(insns, path, (size_t)n * sizeof(uint16_t));
+            *len = n;
+            found = 1;
+        }
+    }
+    if (explored) *explored = s->tried;
+    solver_teardown(s);
+    free(s);
+    return found;
+}
+
+/* src/search_state.c:34-46 */
+void or_state_root(const or_task_t *task, or_state_t *st)
+{
+    memset(st, 0, sizeof(*st));
+    for (int k = 0; k < task->num_tests; k++)
+        for (int i = 0; i < task->num_inputs && i < 7; i++) st->regs[k][i + 1] = task->tests[k].in[i];
+    for (int i = 0; i < task->num_inputs && i < 7; i++) st->live |= (uint8_t)(1u << (i + 1));
+}
+
+int or_state_step(const or_task_t *task, const or_state_t *src, uint16_t insn, or_state_t *dst)
+{
+    ss_t a, b;
+    memcpy(&a, src, sizeof(a));
+    if (!ss_step(task, &a, insn, &b)) return 0;
+    memcpy(dst, &b, sizeof(b));
+    return 1;
+}
+
+uint8_t or_state_leaf(const or_task_t *task, const or_state_t *st)
+{
+    ss_t a;
+    memcpy(&a, st, sizeof(a));
+    return ss_leaf(task, &a);
+}
+
+typedef char or_state_layout_check[(sizeof(or_state_t) == 546 && sizeof(ss_t) == sizeof(or_state_t)) ? 1 : -1];

@@ -15,6 +15,7 @@
  *                   src/search_state.c:280-330 (branchless mode)
  *   IDDFS ......... tools/enumerate.c:271-461, :537-662
  *   search state .. src/search_state.c:15-239
+ *   completion .... src/search_state.c:339-445 (search_exhaustive_complete)
  *
  * OEP modes:
  *   OR_OEP_EXACT  keys are the full per-test state bytes (regs r0..r7 lo/hi
@@ -166,6 +167,33 @@ uint64_t or_alu_digest(int op, uint32_t b_lo, uint32_t b_hi);
  * out must hold num_tests*9 uint16 (flags stored in the 9th). */
 void or_replay_state(const or_task_t *task, const uint16_t *kernel, int len,
                      uint16_t *out);
+
+/* ---- exhaustive leaf completion (src/search_state.c:339-445) ---- */
+
+/* Byte layout identical to search_state_t (src/search_state.h:19-25, 546 B):
+ * per-test registers and flags, live mask, kernel instructions so far. */
+typedef struct {
+    uint16_t regs[OR_MAX_TESTS][8];
+    uint8_t  flags[OR_MAX_TESTS];
+    uint8_t  live;
+    uint8_t  depth;
+} or_state_t;
+
+/* search_state_init_root / search_state_step / search_state_leaf_outputs
+ * (src/search_state.c:34-46, :49-187, :223-239). */
+void    or_state_root(const or_task_t *task, or_state_t *st);
+int     or_state_step(const or_task_t *task, const or_state_t *src, uint16_t insn, or_state_t *dst);
+uint8_t or_state_leaf(const or_task_t *task, const or_state_t *st);
+
+/* search_exhaustive_complete: IDDFS over 1..max_depth - start->depth
+ * appended branchless instructions (at most 16), per-position OEP tables
+ * cleared per iteration; returns 1 with the first completion in DFS order
+ * (insns, *len; len 0 when the start state already solves), 0 if none, -1
+ * on allocation failure.  oep_mode OR_OEP_REF32 is the reference's 2^20
+ * table of 32-bit keys with 32 probes; OR_OEP_EXACT uses full state bytes.
+ * *explored counts successful child steps (search_exhaustive_states_explored). */
+int or_complete(const or_task_t *task, const or_state_t *start, int max_depth, int oep_mode,
+                uint16_t *insns, int *len, uint64_t *explored);
 
 #ifdef __cplusplus
 }

@@ -16,6 +16,7 @@ library or a GPU is missing.
 from ._lib import (PcaGpuError, Task, Prefix, Solution, Stats, lib, load_library,
                    disasm, gen_candidates, host_frontier)
 from .solver import Solver, SolveResult
+from . import search_state
 
 __all__ = ["PcaGpuError", "Task", "Prefix", "Solution", "Stats", "lib", "load_library",
-           "disasm", "gen_candidates", "host_frontier", "Solver", "SolveResult"]
+           "disasm", "gen_candidates", "host_frontier", "Solver", "SolveResult", "search_state"]

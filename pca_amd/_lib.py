@@ -24,6 +24,7 @@ ERRORS = {
     -6: "ECOLLISION (64-bit OEP key shared by two distinct states)",
     -7: "ENODEV (no GPU visible)",
     -8: "EIO (task file unreadable)",
+    -9: "EVERIFY (a device result failed host re-verification)",
 }
 
 
@@ -123,6 +124,13 @@ class Solution(C.Structure):
         return [self.prog[i] for i in range(self.len)]
 
 
+class SearchState(C.Structure):
+    """search_state_t (src/search_state.h:19-25), 546 bytes: per-test
+    registers and flags, live mask, kernel instructions so far."""
+    _fields_ = [("regs", (C.c_uint16 * 8) * MAX_TESTS), ("flags", C.c_uint8 * MAX_TESTS), ("live", C.c_uint8),
+                ("depth", C.c_uint8)]
+
+
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "candidates", "suffixes", "tests_evaluated", "prefixes", "raw_hits", "kernel_ns",
@@ -197,6 +205,13 @@ SIGNATURES = [
                                         C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)]),
     ("pcagpu_solve", C.c_int, [C.c_void_p, C.POINTER(SolveOpts), C.POINTER(SolveRes),
                                C.POINTER(Solution), C.c_long]),
+    ("pcagpu_state_init_root", None, [C.POINTER(Task), C.POINTER(SearchState)]),
+    ("pcagpu_state_step", C.c_int, [C.POINTER(Task), C.POINTER(SearchState), C.c_uint16, C.POINTER(SearchState)]),
+    ("pcagpu_state_leaf_outputs", C.c_uint8, [C.POINTER(Task), C.POINTER(SearchState)]),
+    ("pcagpu_gen_candidates_branchless", C.c_int, [C.POINTER(C.c_uint16), C.c_int, C.c_uint8]),
+    ("pcagpu_complete", C.c_int, [C.c_void_p, C.POINTER(SearchState), C.c_int, C.POINTER(C.c_uint16),
+                                  C.POINTER(C.c_int)]),
+    ("pcagpu_complete_states_explored", C.c_uint64, [C.c_void_p]),
     ("pcagpu_server_open", C.c_int, [C.POINTER(ServerOpts), C.POINTER(C.c_void_p)]),
     ("pcagpu_server_close", None, [C.c_void_p]),
     ("pcagpu_server_submit", C.c_int, [C.c_void_p, C.POINTER(Task), C.c_int, C.POINTER(C.c_uint64)]),

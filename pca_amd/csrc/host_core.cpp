@@ -256,19 +256,23 @@ bool alu_redundant(uint32_t op, uint32_t rd, uint32_t rs, uint32_t rt)
 }
 
 // Appends the non-BR part of the list: ALU block, CMP block -> returns the
-// split point; then LDI and ADDI blocks.
-int build_list(uint16_t *out, uint32_t live, int *split)
+// split point; then LDI and ADDI blocks.  branchless: the search-state
+// grammar (src/search_state.c:280-330), the same blocks in the same order
+// without DIV, MOD and CMP.
+int build_list(uint16_t *out, uint32_t live, int *split, bool branchless = false)
 {
     auto readable = [live](uint32_t r) { return r == 0 || (live >> r & 1u); };
     int n = 0;
-    for (uint32_t op : kAlu)
+    for (uint32_t op : kAlu) {
+        if (branchless && (op == OP_DIV || op == OP_MOD)) continue;
         for (uint32_t rd = 1; rd < 8; rd++)
             for (uint32_t rs = 0; rs < 8; rs++) {
                 if (!readable(rs)) continue;
                 for (uint32_t rt = commutative(op) ? rs : 0; rt < 8; rt++)
                     if (readable(rt) && !alu_redundant(op, rd, rs, rt)) out[n++] = enc_r(op, rd, rs, rt);
             }
-    for (uint32_t rd = 0; rd < 8; rd++) {
+    }
+    for (uint32_t rd = 0; rd < 8 && !branchless; rd++) {
         if (!readable(rd)) continue;
         for (uint32_t rs = 0; rs < 8; rs++)
             if (readable(rs) && rs != rd) out[n++] = enc_r(OP_CMP, rd, rs, 0);
@@ -282,6 +286,16 @@ int build_list(uint16_t *out, uint32_t live, int *split)
     return n;
 }
 }  // namespace
+
+int gen_candidates_branchless(uint16_t *out, int max, uint32_t live)
+{
+    uint16_t tmp[CAND_LMAX];
+    int split;
+    int n = build_list(tmp, live & 0xFEu, &split, true);
+    int k = 0;
+    for (int j = 0; j < n && k < max; j++) out[k++] = tmp[j];
+    return k;
+}
 
 int gen_candidates(uint16_t *out, int max, uint32_t live, int pos, int kernel_depth)
 {
@@ -298,7 +312,7 @@ int gen_candidates(uint16_t *out, int max, uint32_t live, int pos, int kernel_de
     return k;
 }
 
-void CandTable::build()
+void CandTable::build(bool branchless)
 {
     list.assign((size_t)128 * CAND_LMAX, 0);
     vg_insn.assign((size_t)128 * VG_MAX, 0);
@@ -307,7 +321,7 @@ void CandTable::build()
     for (int m = 0; m < 128; m++) {
         const uint32_t live = (uint32_t)m << 1;
         uint16_t *L = &list[(size_t)m * CAND_LMAX];
-        n[m] = build_list(L, live, &a[m]);
+        n[m] = build_list(L, live, &a[m], branchless);
         cand_total[m] = 0;
         std::vector<uint16_t> keys;
         for (int j = 0; j < n[m]; j++) {
@@ -394,7 +408,17 @@ const CandTable &shared_grammar()
 {
     static const CandTable table = [] {
         CandTable t;
-        t.build();
+        t.build(false);
+        return t;
+    }();
+    return table;
+}
+
+const CandTable &shared_grammar_branchless()
+{
+    static const CandTable table = [] {
+        CandTable t;
+        t.build(true);
         return t;
     }();
     return table;

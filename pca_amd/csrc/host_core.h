@@ -37,6 +37,8 @@ void disasm(uint16_t insn, char *buf, size_t len);
 
 // Ordered branch-mode grammar at one kernel position (tools/enumerate.c:206-267).
 int gen_candidates(uint16_t *out, int max, uint32_t live, int pos, int kernel_depth);
+// search_gen_candidates_branchless (src/search_state.c:280-330)
+int gen_candidates_branchless(uint16_t *out, int max, uint32_t live);
 
 // The grammar factored for arithmetic indexing on the device: for every
 // live mask m over r1..r7 (live = m << 1) the non-BR list in reference
@@ -67,7 +69,7 @@ struct CandTable {
     uint32_t cand_total[128];        // sum over the list of live out-registers
     std::vector<uint64_t> bits_init;    // [128][NBK] groups present, one bit each
     std::vector<uint16_t> bits_gidx;    // [128][NBK][64] group index of a bit (0xFFFF: none)
-    void build();
+    void build(bool branchless);
     int count(uint32_t live, int mf) const { return n[(live >> 1) & 127] + (mf > 0 ? 6 * mf : 0); }
     uint16_t at(uint32_t live, int mf, int j) const
     {
@@ -93,6 +95,9 @@ std::vector<int> device_test_order(const pca_task_t *t);
 // The grammar tables, built once per process on first use (thread-safe) and
 // never modified afterwards.
 const CandTable &shared_grammar();
+// The search-state grammar's tables (branchless: no DIV, MOD, CMP or BR),
+// built and kept the same way; count/at are called with mf = 0.
+const CandTable &shared_grammar_branchless();
 
 // Per-test root state: the registers after the IN scaffold
 // (tools/enumerate.c:598-604) computed by running it on the VM.

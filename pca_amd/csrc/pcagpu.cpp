@@ -78,14 +78,15 @@ int adapt_test_order(pcagpu *h, const unsigned long long *ts)
     return 0;
 }
 
-const GrammarDev *shared_grammar_dev(int device)
+const GrammarDev *shared_grammar_dev(int device, bool branchless)
 {
     static std::mutex mu;
     static std::map<int, const GrammarDev *> tables;
     std::lock_guard<std::mutex> lock(mu);
-    auto it = tables.find(device);
+    const int key = 2 * device + (branchless ? 1 : 0);
+    auto it = tables.find(key);
     if (it != tables.end()) return it->second;
-    const CandTable &ct = shared_grammar();
+    const CandTable &ct = branchless ? shared_grammar_branchless() : shared_grammar();
     std::unique_ptr<GrammarDev> img(new (std::nothrow) GrammarDev);
     if (!img) return nullptr;
     std::memset(img.get(), 0, sizeof(GrammarDev));
@@ -105,15 +106,16 @@ const GrammarDev *shared_grammar_dev(int device)
         (void)hipFree(d);
         return nullptr;
     }
-    tables[device] = d;
+    tables[key] = d;
     return d;
 }
 
-CandDev cand_dev(const pcagpu *h)
+CandDev cand_dev(const GrammarDev *g)
 {
-    const GrammarDev *g = h->d_grammar;
     return CandDev{g, g->count, g->split, g->cand_total, &g->bits_init[0][0]};
 }
+
+CandDev cand_dev(const pcagpu *h) { return cand_dev(h->d_grammar); }
 
 int grid_for(pcagpu *h, uint64_t units, bool multi, uint32_t levels)
 {
@@ -1092,6 +1094,81 @@ extern "C" void pcagpu_reset_stats(pcagpu_t *h)
 }
 
 // ------------------------------------------------------------ wavefront
+namespace pca {
+
+int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_t n, int D, int pos,
+                 uint64_t *kept_out)
+{
+    int rc;
+    ExpandArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.parents = h->d_front[cur].p;
+    a.n_parents = n;
+    a.D = D;
+    a.pos = pos;
+    a.cand = cd;
+    if ((rc = h->d_child_off.ensure(n + 1)) || (rc = h->d_keep_cnt.ensure(n + 1)) ||
+        (rc = h->d_scan_tmp.ensure(exclusive_scan_tmp_words(n) + 16)))
+        return rc;
+    HIP_TRY(hipMemsetAsync(h->d_child_off.p, 0, (n + 1) * sizeof(uint64_t), h->stream));
+    HIP_TRY(hipEventRecord(h->ev0, h->stream));
+    launch_expand_count(a, h->d_child_off.p, h->stream);
+    launch_exclusive_scan(h->d_child_off.p, n, h->d_scan_tmp.p, h->stream);
+    uint64_t children = 0;
+    HIP_TRY(hipMemcpyAsync(&children, h->d_child_off.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    uint64_t tcap = 1024;
+    while (tcap < 2 * children) tcap <<= 1;
+    if ((rc = h->d_table.ensure(2 * tcap)) || (rc = h->d_keep_bits.ensure(children / 32 + 8)))
+        return rc;
+    // keys 0 (empty), min slots all-ones so atomicMin keeps the smallest index
+    launch_table_init(h->d_table.p, tcap, h->stream);
+    HIP_TRY(hipMemsetAsync(h->d_keep_bits.p, 0, (children / 32 + 8) * sizeof(uint32_t), h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_counters.p + 4, 0, 2 * sizeof(unsigned long long), h->stream));
+    a.child_off = h->d_child_off.p;
+    a.table = h->d_table.p;
+    a.table_mask = tcap - 1;
+    a.keep_bits = h->d_keep_bits.p;
+    a.keep_count = h->d_keep_cnt.p;
+    a.counters = h->d_counters.p + 4;
+    a.work = h->d_work.p;
+    const int grid = grid_for(h, n);
+    HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
+    launch_expand_insert(dt, a, grid, h->stream);
+    HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_keep_cnt.p, 0, (n + 1) * sizeof(uint64_t), h->stream));
+    launch_expand_survive(dt, a, grid, h->stream);
+    launch_exclusive_scan(h->d_keep_cnt.p, n, h->d_scan_tmp.p, h->stream);
+    uint64_t kept = 0;
+    unsigned long long ctr[2] = {0, 0};
+    HIP_TRY(hipMemcpyAsync(&kept, h->d_keep_cnt.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY(hipMemcpyAsync(ctr, h->d_counters.p + 4, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    if (ctr[0] || ctr[1]) {
+        h->stats.oep_collisions += ctr[0];
+        return ctr[0] ? PCAGPU_ECOLLISION : PCAGPU_EOVERFLOW;
+    }
+    const int nxt = cur ^ 1;
+    if ((rc = h->d_front[nxt].ensure(kept ? kept : 1))) return rc;
+    a.keep_off = h->d_keep_cnt.p;
+    a.out = h->d_front[nxt].p;
+    HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
+    launch_expand_compact(a, grid, h->stream);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipEventRecord(h->ev1, h->stream));
+    HIP_TRY(hipStreamSynchronize(h->stream));
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->stats.expand_ns += (uint64_t)((double)ms * 1e6);
+    h->stats.frontier_nodes += children;
+    h->stats.frontier_kept += kept;
+    *kept_out = kept;
+    return 0;
+}
+
+}  // namespace pca
+
 extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, long cap)
 {
     if (!h) return PCAGPU_EINVAL;
@@ -1106,72 +1183,12 @@ extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, lo
     HIP_TRY(hipMemcpy(h->d_front[0].p, &root, sizeof(root), hipMemcpyHostToDevice));
     uint64_t n = 1;
     int cur = 0;
+    const CandDev cd = cand_dev(h);
     for (int pos = 0; pos < s; pos++) {
-        ExpandArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.parents = h->d_front[cur].p;
-        a.n_parents = n;
-        a.D = D;
-        a.pos = pos;
-        a.cand = cand_dev(h);
-        if ((rc = h->d_child_off.ensure(n + 1)) || (rc = h->d_keep_cnt.ensure(n + 1)) ||
-            (rc = h->d_scan_tmp.ensure(exclusive_scan_tmp_words(n) + 16)))
-            return rc;
-        HIP_TRY(hipMemsetAsync(h->d_child_off.p, 0, (n + 1) * sizeof(uint64_t), h->stream));
-        HIP_TRY(hipEventRecord(h->ev0, h->stream));
-        launch_expand_count(a, h->d_child_off.p, h->stream);
-        launch_exclusive_scan(h->d_child_off.p, n, h->d_scan_tmp.p, h->stream);
-        uint64_t children = 0;
-        HIP_TRY(hipMemcpyAsync(&children, h->d_child_off.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        uint64_t tcap = 1024;
-        while (tcap < 2 * children) tcap <<= 1;
-        if ((rc = h->d_table.ensure(2 * tcap)) || (rc = h->d_keep_bits.ensure(children / 32 + 8)))
-            return rc;
-        // keys 0 (empty), min slots all-ones so atomicMin keeps the smallest index
-        launch_table_init(h->d_table.p, tcap, h->stream);
-        HIP_TRY(hipMemsetAsync(h->d_keep_bits.p, 0, (children / 32 + 8) * sizeof(uint32_t), h->stream));
-        HIP_TRY(hipMemsetAsync(h->d_counters.p + 4, 0, 2 * sizeof(unsigned long long), h->stream));
-        a.child_off = h->d_child_off.p;
-        a.table = h->d_table.p;
-        a.table_mask = tcap - 1;
-        a.keep_bits = h->d_keep_bits.p;
-        a.keep_count = h->d_keep_cnt.p;
-        a.counters = h->d_counters.p + 4;
-        a.work = h->d_work.p;
-        const int grid = grid_for(h, n);
-        HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
-        launch_expand_insert(h->dt, a, grid, h->stream);
-        HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
-        HIP_TRY(hipMemsetAsync(h->d_keep_cnt.p, 0, (n + 1) * sizeof(uint64_t), h->stream));
-        launch_expand_survive(h->dt, a, grid, h->stream);
-        launch_exclusive_scan(h->d_keep_cnt.p, n, h->d_scan_tmp.p, h->stream);
         uint64_t kept = 0;
-        unsigned long long ctr[2] = {0, 0};
-        HIP_TRY(hipMemcpyAsync(&kept, h->d_keep_cnt.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipMemcpyAsync(ctr, h->d_counters.p + 4, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                               h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        if (ctr[0] || ctr[1]) {
-            h->stats.oep_collisions += ctr[0];
-            return ctr[0] ? PCAGPU_ECOLLISION : PCAGPU_EOVERFLOW;
-        }
-        const int nxt = cur ^ 1;
-        if ((rc = h->d_front[nxt].ensure(kept ? kept : 1))) return rc;
-        a.keep_off = h->d_keep_cnt.p;
-        a.out = h->d_front[nxt].p;
-        HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
-        launch_expand_compact(a, grid, h->stream);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(h->ev1, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-        h->stats.expand_ns += (uint64_t)((double)ms * 1e6);
-        h->stats.frontier_nodes += children;
-        h->stats.frontier_kept += kept;
+        if ((rc = expand_level(h, h->dt, cd, cur, n, D, pos, &kept))) return rc;
         n = kept;
-        cur = nxt;
+        cur ^= 1;
         if (n == 0) break;
     }
     if (out && n) {

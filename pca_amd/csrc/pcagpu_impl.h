@@ -85,6 +85,8 @@ struct pcagpu {
     uint32_t scan_wpc[2][pca::MAXLEV + 1] = {};
 
     const GrammarDev *d_grammar = nullptr;   // per device, shared, never freed
+    const GrammarDev *d_grammar_bl = nullptr;   // branchless tables (leaf completion), same lifetime
+    uint64_t complete_explored = 0;          // children generated by the last pcagpu_complete
     DevBuf<pca_prefix_t> d_batch;
     DevBuf<ScanBlock> d_blocks;
     DevBuf<ScanGroup> d_groups;      // match pass only, allocated on first use
@@ -145,8 +147,15 @@ int fill_devtask(pcagpu *h);
 void set_test_order(pcagpu *h, const std::vector<int> &order);
 // the grammar tables of the current device, uploaded on first use and kept
 // until the process ends (read-only, shared by every handle on the device)
-const GrammarDev *shared_grammar_dev(int device);
+const GrammarDev *shared_grammar_dev(int device, bool branchless = false);
+CandDev cand_dev(const GrammarDev *g);
 CandDev cand_dev(const pcagpu *h);
+// One wavefront level: the children of the n records in d_front[cur] at
+// kernel position pos (of depth D) on task record dt, first visitor of
+// every distinct state kept (exact keys), written in rank order to
+// d_front[cur ^ 1]; *kept = their count.
+int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_t n, int D, int pos,
+                 uint64_t *kept);
 // persistent grid for `units` work units of a scan launch (multi: several
 // items; levels: state levels per wave), or of the other persistent kernels
 // (levels 0: the one-item scan figure at 3 levels)

@@ -97,6 +97,23 @@ class Solver:
                            timed_out=bool(res.timed_out), seconds=res.seconds,
                            candidates=list(res.candidates), prefixes=list(res.prefixes))
 
+    # exhaustive leaf completion (src/search_state.c:418-450) -------------
+    def exhaustive_complete(self, state, max_depth):
+        """search_exhaustive_complete on the GPU, as PCALib.exhaustive_complete
+        (python/search_eval.py:171-181): (found, sol_len, [insns])."""
+        ins = (C.c_uint16 * MAX_KDEPTH)()
+        n = C.c_int()
+        rc = lib().pcagpu_complete(self._h, C.byref(state), max_depth, ins, C.byref(n))
+        if rc < 0:
+            raise PcaGpuError(rc, "pcagpu_complete")
+        if rc == 0:
+            return False, 0, []
+        return True, n.value, list(ins[:n.value])
+
+    def exhaustive_states_explored(self):
+        """Children generated by the last exhaustive_complete."""
+        return int(lib().pcagpu_complete_states_explored(self._h))
+
     # the drop-in seam ----------------------------------------------------
     def frontier(self, target_depth, split, cap=1 << 24):
         """GPU wavefront producer: exact-OEP survivors at the split, DFS order."""
