@@ -146,6 +146,57 @@ def value_bench(batch=512, steps=30, seed=5):
             "params": net.n_params, "timing": "host clock around device-synchronised repetitions"}
 
 
+def completion_bench(reps=3):
+    """Leaf completion (search_exhaustive_complete) on the GPU: the
+    two-input fixture states with a budget of 3 instructions (the reference
+    loop's completion radius), per-call time against the reference
+    library's one-core time recorded with the fixtures."""
+    import statistics
+    from pca_amd import Task, Solver, search_state
+    cdir = os.path.join(ROOT, "tests", "golden", "complete")
+    with open(os.path.join(cdir, "cases.json")) as f:
+        cases = [c for c in json.load(f)["cases"] if c["kind"] == "heavy"]
+    out = []
+    for c in cases:
+        task = Task.load(os.path.join(cdir, "tasks", c["task"] + ".json"))
+        st = search_state.after(task, c["path"])
+        with Solver(task) as sv:
+            sv.exhaustive_complete(st, c["max_depth"])   # first call: tables, buffers
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                found, n, ins = sv.exhaustive_complete(st, c["max_depth"])
+                ts.append(time.perf_counter() - t0)
+            explored = sv.exhaustive_states_explored()
+        assert (found, ins) == (c["found"], c["insns"]), c["task"]
+        ms = statistics.median(ts) * 1e3
+        out.append({"task": c["task"], "path": c["path"], "max_depth": c["max_depth"], "found": found,
+                    "gpu_ms": ms, "gpu_children": explored, "reference_s": c["seconds"],
+                    "reference_explored": c["explored"], "speedup": c["seconds"] / (ms / 1e3)})
+    res = {"workload": "two-input tasks, start states 2 instructions deep, budget 3 (R = 3)",
+           "cases": out, "gpu_ms_total": sum(x["gpu_ms"] for x in out),
+           "reference_s_total": sum(x["reference_s"] for x in out),
+           "reference_timing": "reference search_exhaustive_complete, one core of the build host"}
+    res["speedup_total"] = res["reference_s_total"] / (res["gpu_ms_total"] / 1e3)
+    return res
+
+
+def completion_cpu_baseline():
+    """The oracle's restatement of search_exhaustive_complete (32-bit OEP
+    table as in the reference, one thread) on this host, on the two-input
+    budget-3 fixture state the reference finishes fastest."""
+    from oracle import oracle as O
+    cdir = os.path.join(ROOT, "tests", "golden", "complete")
+    with open(os.path.join(cdir, "cases.json")) as f:
+        cases = [c for c in json.load(f)["cases"] if c["kind"] == "heavy"]
+    c = min(cases, key=lambda x: x["seconds"])
+    ot = O.load_task(os.path.join(cdir, "tasks", c["task"] + ".json"))
+    t0 = time.perf_counter()
+    O.complete(ot, O.state_after(ot, c["path"]), c["max_depth"], O.OEP_REF32)
+    return {"task": c["task"], "path": c["path"], "max_depth": c["max_depth"], "seconds": time.perf_counter() - t0,
+            "cores": 1, "kind": "port"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -160,6 +211,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--suite", type=int, default=1000, help="config-3 suite size (0: skip)")
     ap.add_argument("--no-value", action="store_true")
+    ap.add_argument("--no-complete", action="store_true")
     ap.add_argument("--suite-budget", type=int, default=3)
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
@@ -256,7 +308,12 @@ def main():
     if not args.no_value:
         line["value_net"] = value_bench()
         print("value net done", file=sys.stderr, flush=True)
+    if not args.no_complete:
+        line["leaf_completion"] = completion_bench()
+        print("leaf completion done", file=sys.stderr, flush=True)
     if not args.no_cpu:
+        if not args.no_complete:
+            line["leaf_completion"]["cpu_baseline"] = completion_cpu_baseline()
         cb = cpu_baseline(task_path, args.cpu_seconds)
         line["cpu_baseline"] = cb
         if cb:
