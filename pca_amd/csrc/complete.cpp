@@ -106,6 +106,10 @@ int scan_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_
         l.work = h->d_work.p;
         l.tstat = nullptr;
         const int grid = grid_for(h, n, false, l.levels);
+        // a frontier record is a cheap unit (one parent): take runs of them,
+        // about 8 runs per wave
+        const uint64_t waves = (uint64_t)grid * SCAN_WAVES;
+        l.grab = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(n / (waves * 8), 1), 32);
         HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
         HIP_TRY(hipMemsetAsync(h->d_counters.p, 0, 4 * sizeof(unsigned long long), h->stream));
         HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), h->stream));

@@ -159,6 +159,12 @@ struct ExpandArgs {
     pca_prefix_t *out;
     unsigned long long *counters;    // [0] collisions [1] probe overflow
     unsigned long long *work;
+    // insert / survive: each parent's child list is cut into `slices` runs
+    // of whole 64-child blocks, one work unit each (a frontier of few
+    // parents still fills the chip); a wave takes `grab` consecutive units
+    // per atomic on `work` (0 or 1: one)
+    uint32_t slices;
+    uint32_t grab;
 };
 
 void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out, hipStream_t st);
@@ -192,6 +198,11 @@ struct ScanLaunch {
     // one-item launches: per device test position, [2t] candidates that
     // reached it in the survivor phase and [2t+1] those it ended (null: off)
     unsigned long long *tstat;
+    // consecutive work units a wave takes per atomic on `work` (0 or 1: one);
+    // more for launches of many cheap units, whose single work counter
+    // would otherwise serialise the waves
+    uint32_t grab;
+    uint32_t pad;
 };
 
 // Kernel-node form of a one-item scan launch, for a graph-captured flush:

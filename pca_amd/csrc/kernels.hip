@@ -689,11 +689,18 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
         it_begin = 0;
         it_end = one.n_units;
     }
+    const uint64_t grab = lch.grab > 1u ? lch.grab : 1u;
+    uint64_t ug_next = 0, ug_end = 0;   // the wave's current run of units
     for (;;) {
-        unsigned long long ug = 0;
-        if (lane == 0) ug = atomicAdd(lch.work, 1ull);
-        ug = uni64(ug);
-        if (ug >= lch.n_units) break;
+        if (ug_next >= ug_end) {
+            unsigned long long g0 = 0;
+            if (lane == 0) g0 = atomicAdd(lch.work, (unsigned long long)grab);
+            g0 = uni64(g0);
+            if (g0 >= lch.n_units) break;
+            ug_next = g0;
+            ug_end = min((uint64_t)g0 + grab, (uint64_t)lch.n_units);
+        }
+        const uint64_t ug = ug_next++;
         if (MULTI && (ug < it_begin || ug >= it_end)) {
             // the item holding this unit: largest ii with unit_begin <= ug
             uint32_t lo = 0, hi = lch.n_items;
@@ -917,25 +924,59 @@ __global__ void expand_count_kernel(ExpandArgs a, uint64_t *counts)
 // Pass 1: every non-BR child inserts its 64-bit key with atomicMin of its
 // global child index (= DFS rank order), so the slot ends at the first
 // visitor whatever the arrival order.
+// The work loop of the insert and survive passes: units are (parent, slice)
+// pairs, taken `grab` at a time; the slice covers children [jb, je) of the
+// parent, whole 64-child blocks (the last one may be partial).
+struct ExpandUnits {
+    uint64_t next = 0, end = 0;
+    uint32_t S, grab;
+    uint64_t n_units;
+    __device__ ExpandUnits(const ExpandArgs &a, bool sliced)
+        : S(sliced && a.slices > 1u ? a.slices : 1u), grab(a.grab > 1u ? a.grab : 1u), n_units(a.n_parents * S) {}
+    // false when the launch's units are exhausted
+    __device__ bool take(const ExpandArgs &a, int lane, uint64_t *parent, uint32_t *slice)
+    {
+        if (next >= end) {
+            unsigned long long g0 = 0;
+            if (lane == 0) g0 = atomicAdd(a.work, (unsigned long long)grab);
+            g0 = uni64(g0);
+            if (g0 >= n_units) return false;
+            next = g0;
+            end = min((uint64_t)g0 + grab, n_units);
+        }
+        const uint64_t x = next++;
+        *parent = x / S;
+        *slice = (uint32_t)(x - *parent * S);
+        return true;
+    }
+    __device__ void range(int n, uint32_t slice, int *jb, int *je) const
+    {
+        const int per = (((n + (int)S - 1) / (int)S) + 63) & ~63;
+        *jb = min(n, (int)slice * per);
+        *je = min(n, *jb + per);
+    }
+};
+
 __global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, ExpandArgs a)
 {
     __shared__ TestState s_st[SCAN_WAVES][MAXT];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int mf = a.D - a.pos - 1;
-    for (;;) {
-        unsigned long long u = 0;
-        if (lane == 0) u = atomicAdd(a.work, 1ull);
-        u = __shfl(u, 0);
-        if (u >= a.n_parents) break;
+    ExpandUnits units(a, true);
+    uint64_t u;
+    uint32_t slice;
+    while (units.take(a, lane, &u, &slice)) {
         const pca_prefix_t *P = &a.parents[u];
         const uint32_t live = uni(P->live);
         replay_prefix(task, P, a.pos, s_st[w], lane);
         wave_sync();
         const int n = cand_count(a.cand, live, mf);
+        int jb, je;
+        units.range(n, slice, &jb, &je);
         const uint64_t base = a.child_off[u];
-        for (int j0 = 0; j0 < n; j0 += 64) {
+        for (int j0 = jb; j0 < je; j0 += 64) {
             const int j = j0 + lane;
-            if (j >= n) continue;
+            if (j >= je) continue;
             const uint32_t c = cand_at(a.cand, live, mf, j);
             if (insn_op(c) == OP_BR) continue;
             uint64_t h = 0x243F6A8885A308D3ull;
@@ -971,22 +1012,23 @@ __global__ __launch_bounds__(SCAN_WG) void expand_survive_kernel(DevTask task, E
     __shared__ TestState s_st[SCAN_WAVES][MAXT];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int mf = a.D - a.pos - 1;
-    for (;;) {
-        unsigned long long u = 0;
-        if (lane == 0) u = atomicAdd(a.work, 1ull);
-        u = __shfl(u, 0);
-        if (u >= a.n_parents) break;
+    ExpandUnits units(a, true);
+    uint64_t u;
+    uint32_t slice;
+    while (units.take(a, lane, &u, &slice)) {
         const pca_prefix_t *P = &a.parents[u];
         const uint32_t live = uni(P->live);
         replay_prefix(task, P, a.pos, s_st[w], lane);
         wave_sync();
         const int n = cand_count(a.cand, live, mf);
+        int jb, je;
+        units.range(n, slice, &jb, &je);
         const uint64_t base = a.child_off[u];
         unsigned long long kept = 0;
-        for (int j0 = 0; j0 < n; j0 += 64) {
+        for (int j0 = jb; j0 < je; j0 += 64) {
             const int j = j0 + lane;
             bool keep = false;
-            if (j < n) {
+            if (j < je) {
                 const uint32_t c = cand_at(a.cand, live, mf, j);
                 if (insn_op(c) == OP_BR) {
                     keep = true;
@@ -1041,14 +1083,14 @@ __global__ __launch_bounds__(SCAN_WG) void expand_survive_kernel(DevTask task, E
             if (lane == 0) {
                 const uint64_t wi = bit0 >> 5;
                 const uint32_t sh = (uint32_t)(bit0 & 31u);
-                const uint64_t valid = (n - j0) >= 64 ? ~0ull : ((1ull << (n - j0)) - 1ull);
+                const uint64_t valid = (je - j0) >= 64 ? ~0ull : ((1ull << (je - j0)) - 1ull);
                 const unsigned long long bv = b & valid;
                 atomicOr(&a.keep_bits[wi], (uint32_t)(bv << sh));
                 atomicOr(&a.keep_bits[wi + 1], (uint32_t)((bv << sh) >> 32));
                 if (sh) atomicOr(&a.keep_bits[wi + 2], (uint32_t)(bv >> (64 - sh)));
             }
         }
-        if (lane == 0) a.keep_count[u] = kept;
+        if (lane == 0 && kept) atomicAdd((unsigned long long *)&a.keep_count[u], kept);
         wave_sync();
     }
 }
@@ -1059,11 +1101,10 @@ __global__ __launch_bounds__(SCAN_WG) void expand_compact_kernel(ExpandArgs a)
 {
     const int lane = threadIdx.x & 63;
     const int mf = a.D - a.pos - 1;
-    for (;;) {
-        unsigned long long u = 0;
-        if (lane == 0) u = atomicAdd(a.work, 1ull);
-        u = __shfl(u, 0);
-        if (u >= a.n_parents) break;
+    ExpandUnits units(a, false);
+    uint64_t u;
+    uint32_t slice;
+    while (units.take(a, lane, &u, &slice)) {
         const pca_prefix_t P = a.parents[u];
         const int n = cand_count(a.cand, P.live, mf);
         const uint64_t base = a.child_off[u];

@@ -285,6 +285,8 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile 
         l.counters = h->d_counters.p;
         l.work = h->d_work.p;
         l.tstat = adapt ? h->d_tstat.p : nullptr;
+        l.grab = 1;
+        l.pad = 0;
         const int grid = grid_for(h, units, false, l.levels);
         int rc = h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks) : graph_scan(h, l, it, grid, n, n_blocks);
         if (rc) return rc;
@@ -1133,11 +1135,18 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     a.counters = h->d_counters.p + 4;
     a.work = h->d_work.p;
     const int grid = grid_for(h, n);
+    // insert / survive: slice the child lists of a small frontier so that
+    // every resident wave has work, and take runs of units from a large one
+    const uint64_t resident = (uint64_t)grid_for(h, UINT64_MAX / 2) * SCAN_WAVES;
+    a.slices = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(resident / std::max<uint64_t>(n, 1), 1), 16);
+    const int grid_s = grid_for(h, n * a.slices);
+    a.grab = (uint32_t)std::min<uint64_t>(
+        std::max<uint64_t>(n * a.slices / ((uint64_t)grid_s * SCAN_WAVES * 8), 1), 32);
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
-    launch_expand_insert(dt, a, grid, h->stream);
+    launch_expand_insert(dt, a, grid_s, h->stream);
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_keep_cnt.p, 0, (n + 1) * sizeof(uint64_t), h->stream));
-    launch_expand_survive(dt, a, grid, h->stream);
+    launch_expand_survive(dt, a, grid_s, h->stream);
     launch_exclusive_scan(h->d_keep_cnt.p, n, h->d_scan_tmp.p, h->stream);
     uint64_t kept = 0;
     unsigned long long ctr[2] = {0, 0};
@@ -1154,6 +1163,7 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     a.keep_off = h->d_keep_cnt.p;
     a.out = h->d_front[nxt].p;
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
+    a.grab = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(n / ((uint64_t)grid * SCAN_WAVES * 8), 1), 32);
     launch_expand_compact(a, grid, h->stream);
     HIP_TRY(hipGetLastError());
     HIP_TRY(hipEventRecord(h->ev1, h->stream));

@@ -368,6 +368,8 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         l.counters = s->d_counters.p;
         l.work = s->d_work.p;
         l.tstat = nullptr;
+        l.grab = 1;
+        l.pad = 0;
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
         launch_scan(l, s->items.data(), in_launch[0]->h->dt,
                     grid_for(in_launch[0]->h, units, s->items.size() > 1, levels), s->stream);
