@@ -1099,7 +1099,7 @@ extern "C" void pcagpu_reset_stats(pcagpu_t *h)
 namespace pca {
 
 int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_t n, int D, int pos,
-                 uint64_t *kept_out)
+                 uint64_t *kept_out, uint64_t max_children)
 {
     int rc;
     ExpandArgs a;
@@ -1119,6 +1119,10 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     uint64_t children = 0;
     HIP_TRY(hipMemcpyAsync(&children, h->d_child_off.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipStreamSynchronize(h->stream));
+    if (max_children && children > max_children) {
+        *kept_out = 0;
+        return 1;   // not expanded: d_front[cur] is untouched
+    }
     uint64_t tcap = 1024;
     while (tcap < 2 * children) tcap <<= 1;
     if ((rc = h->d_table.ensure(2 * tcap)) || (rc = h->d_keep_bits.ensure(children / 32 + 8)))
@@ -1177,6 +1181,33 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     return 0;
 }
 
+// The exact-OEP frontier of the depth-D tree, level by level from the root
+// down to split s_max, into d_front[*cur]; stops early (*s_done < s_max) at
+// the first level whose child lists would exceed max_children (0: no cap).
+// Returns the record count or an error.
+long gpu_frontier(pcagpu *h, int D, int s_max, uint64_t max_children, int *s_done, int *cur_out)
+{
+    int rc;
+    pca_prefix_t root;
+    std::memset(&root, 0, sizeof(root));
+    for (int i = 0; i < h->task.num_inputs; i++) root.live |= (uint8_t)(1u << (i + 1));
+    if ((rc = h->d_front[0].ensure(1))) return rc;
+    HIP_TRY(hipMemcpy(h->d_front[0].p, &root, sizeof(root), hipMemcpyHostToDevice));
+    uint64_t n = 1;
+    int cur = 0, pos = 0;
+    const CandDev cd = cand_dev(h);
+    for (; pos < s_max && n > 0; pos++) {
+        uint64_t kept = 0;
+        if ((rc = expand_level(h, h->dt, cd, cur, n, D, pos, &kept, max_children)) < 0) return rc;
+        if (rc == 1) break;
+        n = kept;
+        cur ^= 1;
+    }
+    *s_done = n > 0 ? pos : s_max;
+    *cur_out = cur;
+    return (long)n;
+}
+
 }  // namespace pca
 
 extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, long cap)
@@ -1184,23 +1215,10 @@ extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, lo
     if (!h) return PCAGPU_EINVAL;
     if (D < 0 || D > PCAGPU_MAX_KDEPTH || s < 0 || s > D) return PCAGPU_EINVAL;
     (void)hipSetDevice(h->device);
-    int rc;
-    // level 0: the root
-    pca_prefix_t root;
-    std::memset(&root, 0, sizeof(root));
-    for (int i = 0; i < h->task.num_inputs; i++) root.live |= (uint8_t)(1u << (i + 1));
-    if ((rc = h->d_front[0].ensure(1))) return rc;
-    HIP_TRY(hipMemcpy(h->d_front[0].p, &root, sizeof(root), hipMemcpyHostToDevice));
-    uint64_t n = 1;
-    int cur = 0;
-    const CandDev cd = cand_dev(h);
-    for (int pos = 0; pos < s; pos++) {
-        uint64_t kept = 0;
-        if ((rc = expand_level(h, h->dt, cd, cur, n, D, pos, &kept))) return rc;
-        n = kept;
-        cur ^= 1;
-        if (n == 0) break;
-    }
+    int s_done = 0, cur = 0;
+    const long nl = gpu_frontier(h, D, s, 0, &s_done, &cur);
+    if (nl < 0) return nl;
+    const uint64_t n = (uint64_t)nl;
     if (out && n) {
         const uint64_t m = std::min<uint64_t>(n, (uint64_t)std::max<long>(cap, 0));
         if (m) HIP_TRY(hipMemcpy(out, h->d_front[cur].p, m * sizeof(pca_prefix_t), hipMemcpyDeviceToHost));
@@ -1220,24 +1238,30 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
     const int maxd = std::min(o->max_depth, PCAGPU_MAX_PROG - 2 - h->task.num_inputs);
     for (int D = std::max(o->start_depth, 0); D <= maxd && D <= PCAGPU_MAX_KDEPTH; D++) {
         int s = o->split < 0 ? (D > 0 ? D - 1 : 0) : std::min(o->split, D);
-        int rc = pcagpu_begin_depth(h, D, s);
-        if (rc) return rc;
+        int rc;
         const uint64_t cand0 = h->stats.candidates, pre0 = h->stats.prefixes;
         bool stopped = false;
         if (o->producer == 0) {
-            long n = pcagpu_frontier(h, D, s, nullptr, 0);
+            // GPU wavefront to the split; with the automatic split it stops at
+            // the deepest level whose child lists stay within kAutoChildren
+            // (the split only moves work between the frontier and the scan;
+            // the solutions are the same)
+            (void)hipSetDevice(h->device);
+            int cur = 0, s_done = s;
+            const long n = gpu_frontier(h, D, s, o->split < 0 ? kAutoChildren : 0, &s_done, &cur);
             if (n < 0) return (int)n;
+            s = s_done;
             std::vector<pca_prefix_t> f((size_t)n);
-            if (n) {
-                long m = pcagpu_frontier(h, D, s, f.data(), n);
-                if (m < 0) return (int)m;
-            }
+            if (n) HIP_TRY(hipMemcpy(f.data(), h->d_front[cur].p, (size_t)n * sizeof(pca_prefix_t),
+                                     hipMemcpyDeviceToHost));
+            if ((rc = pcagpu_begin_depth(h, D, s))) return rc;
             for (long i = 0; i < n; i += h->batch_cap) {
                 uint32_t c = (uint32_t)std::min<long>(h->batch_cap, n - i);
                 if ((rc = pcagpu_submit(h, f.data() + i, c, (uint64_t)i))) return rc;
                 if (o->time_limit > 0 && now_s() - t0 > o->time_limit) { stopped = true; break; }
             }
         } else {
+            if ((rc = pcagpu_begin_depth(h, D, s))) return rc;
             uint64_t rank = 0;
             DfsStats ds;
             int err = 0;

@@ -154,8 +154,14 @@ CandDev cand_dev(const pcagpu *h);
 // kernel position pos (of depth D) on task record dt, first visitor of
 // every distinct state kept (exact keys), written in rank order to
 // d_front[cur ^ 1]; *kept = their count.
+// With max_children > 0, a level whose child lists would be longer is not
+// expanded: returns 1 and leaves d_front[cur] as it was.
 int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_t n, int D, int pos,
-                 uint64_t *kept);
+                 uint64_t *kept, uint64_t max_children = 0);
+long gpu_frontier(pcagpu *h, int D, int s_max, uint64_t max_children, int *s_done, int *cur);
+// child-list cap of one wavefront level under the automatic split (2^27:
+// an OEP table of at most 2^29 slots, 8 GiB)
+constexpr uint64_t kAutoChildren = 1ull << 27;
 // persistent grid for `units` work units of a scan launch (multi: several
 // items; levels: state levels per wave), or of the other persistent kernels
 // (levels 0: the one-item scan figure at 3 levels)

@@ -327,24 +327,52 @@ def labelled_states(task, depth_opt, solutions, negatives_per_node=4, seed=0):
 
 
 # ---------------------------------------------------------------- search
-def guided_search(task, max_depth, net=None, budget=2000, seed=0):
+def guided_search(task, max_depth, net=None, budget=2000, seed=0, r_exhaustive=0, solver=None, stats=None):
     """Best-first search over straight-line kernel prefixes (no CMP / BR),
     children ordered by the network's logit (net None: by depth, then
     grammar order -- breadth-first, the order without the network).  A
     child whose state has a live register equal to the target on every test
     ends the search; the program is returned with the number of expansions
-    (nodes whose children were generated).  Returns (None, expansions) when
-    the budget runs out."""
-    from ._lib import lib as _l
+    (nodes popped within the budget).  Returns (None, expansions) when the
+    budget runs out.
+
+    r_exhaustive > 0: a node within r_exhaustive instructions of max_depth
+    is not expanded but completed by the GPU's exhaustive leaf completion
+    (solver: a pca_amd.Solver of the task), as the reference's loop does
+    (python/bootstrap.py:85-113); a completion ends the search, a failed one
+    counts as the node's expansion.  stats (a dict) receives the completion
+    calls and their time."""
+    from . import search_state as ss
+    import time as _time
     live0 = sum(1 << (i + 1) for i in range(task.num_inputs))
     tgt = np.array([task.tests[t].out[0] for t in range(task.num_tests)], dtype=np.uint16)
     heap = [(0.0, 0, ())]
     counter = 1
     seen = set()
     expansions = 0
+    if stats is not None:
+        stats.update(completion_calls=0, completion_s=0.0)
+    if r_exhaustive > 0 and solver is None:
+        raise ValueError("r_exhaustive needs the task's Solver")
     while heap and expansions < budget:
         _, _, prog = heapq.heappop(heap)
         if len(prog) >= max_depth:
+            continue
+        if r_exhaustive > 0 and len(prog) >= max_depth - r_exhaustive:
+            expansions += 1
+            st = ss.after(task, list(prog))
+            if st is None:     # a DIV/MOD of the prefix divides by zero on some test
+                continue
+            t0 = _time.perf_counter()
+            found, n, ins = solver.exhaustive_complete(st, max_depth)
+            if stats is not None:
+                stats["completion_calls"] += 1
+                stats["completion_s"] += _time.perf_counter() - t0
+            if found:
+                end = ss.after(task, list(prog) + ins)
+                mask = ss.leaf_check(task, end)
+                out_reg = (mask & -mask).bit_length() - 1
+                return (list(prog) + ins, out_reg), expansions
             continue
         live = live0
         for c in prog:

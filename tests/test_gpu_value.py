@@ -129,31 +129,27 @@ def test_forward_backward_match_oracle(pca):
         off += n
 
 
-def test_training_reduces_loss_and_guides_search(pca):
-    """Train on labelled states of exhaustively solved tasks, then compare
-    guided best-first search with the same search ordered breadth-first
-    (no network) on held-out tasks, both with a budget of 500 expansions:
-    the network must find more verified optimal programs, and fewer
-    expansions on the tasks both find; the exhaustive solver's optimum
-    bounds every length found."""
+def _solved(pca, seed, n, length, lo=2):
     from pca_amd import taskgen
-    from pca_amd.value import ValueNet, labelled_states, state_records, guided_search, verify_program
+    out = []
+    for i in range(n):
+        t = taskgen.chain_task(seed, i, length)
+        with pca.Solver(t) as sv:
+            r = sv.solve(length, cap=4096)
+        if r.optimal_depth >= lo:
+            out.append((t, r.optimal_depth, [list(s[0]) for s in r.solutions]))
+    return out
 
-    def solved(seed, n, length):
-        out = []
-        for i in range(n):
-            t = taskgen.chain_task(seed, i, length)
-            with pca.Solver(t) as sv:
-                r = sv.solve(length, cap=4096)
-            if r.optimal_depth >= 2:
-                out.append((t, r.optimal_depth, [list(s[0]) for s in r.solutions]))
-        return out
 
+@pytest.fixture(scope="module")
+def trained(pca):
+    """The network trained on labelled states of exhaustively solved tasks
+    (dependency chains of 3 instructions, optimum >= 2): (net, losses)."""
     import time
+    from pca_amd.value import ValueNet, labelled_states, state_records
     t0 = time.time()
-    train = solved(101, 1200, 3)
-    progs, depths, labels, tasks_idx = [], [], [], []
-    recs, nvalid = [], []
+    train = _solved(pca, 101, 1200, 3)
+    depths, labels, recs, nvalid = [], [], [], []
     for t, d, sols in train:
         # solutions come back as full programs; keep the kernel part
         kernels = [s[t.num_inputs:t.num_inputs + d] for s in sols]
@@ -173,9 +169,22 @@ def test_training_reduces_loss_and_guides_search(pca):
     t0 = time.time()
     losses = net.train(rec, nvalid, depths, labels, steps=1500, batch=512, lr=1e-3)
     print(f"[train] loss {losses[0]:.3f} -> {losses[-1]:.3f} in {time.time() - t0:.1f}s", flush=True)
+    return net, losses
+
+
+def test_training_reduces_loss_and_guides_search(pca, trained):
+    """Train on labelled states of exhaustively solved tasks, then compare
+    guided best-first search with the same search ordered breadth-first
+    (no network) on held-out tasks, both with a budget of 500 expansions:
+    the network must find more verified optimal programs, and fewer
+    expansions on the tasks both find; the exhaustive solver's optimum
+    bounds every length found."""
+    import time
+    from pca_amd.value import guided_search, verify_program
+    net, losses = trained
     assert losses[-1] < 0.7 * losses[0]
 
-    held = solved(202, 40, 3)
+    held = _solved(pca, 202, 40, 3)
     exp_guided, exp_plain, found_g, found_p = 0, 0, 0, 0
     t0 = time.time()
     for t, d, _ in held:
@@ -197,3 +206,38 @@ def test_training_reduces_loss_and_guides_search(pca):
           flush=True)
     assert found_g > found_p
     assert exp_guided * 2 < exp_plain
+
+
+def test_guided_search_with_gpu_leaf_completion(pca, trained):
+    """The reference loop's shape (python/bootstrap.py:85-113): nodes within
+    3 instructions of the budget are handed to the GPU's exhaustive leaf
+    completion instead of being expanded.  On held-out tasks whose optimum is
+    4 (beyond what the search alone finds with 50 expansions), completion
+    finds optimal programs, more of them when the network orders the nodes
+    than in breadth-first order; every program passes the CPU VM and none
+    is shorter than the exhaustive optimum."""
+    import time
+    from pca_amd.value import guided_search, verify_program
+    net, _ = trained
+    held = _solved(pca, 408, 40, 4, lo=4)
+    assert len(held) >= 5
+    found = {}
+    t0 = time.time()
+    for name, use_net, r in (("guided+R3", True, 3), ("plain+R3", False, 3), ("guided", True, 0)):
+        n_opt, calls = 0, 0
+        for t, d, _ in held:
+            st = {}
+            with pca.Solver(t) as sv:
+                res, _ = guided_search(t, d, net=net if use_net else None, budget=50, r_exhaustive=r, solver=sv,
+                                       stats=st)
+            calls += st["completion_calls"]
+            if res is not None:
+                prog, reg = res
+                assert verify_program(t, prog, reg)
+                assert len(prog) >= d
+                n_opt += len(prog) == d
+        found[name] = n_opt
+        print(f"[completion] {name}: optimal {n_opt}/{len(held)}, completion calls {calls}", flush=True)
+    print(f"[completion] {time.time() - t0:.1f}s", flush=True)
+    assert found["guided+R3"] > found["plain+R3"]
+    assert found["guided+R3"] > found["guided"]
