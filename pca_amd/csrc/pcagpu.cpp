@@ -285,9 +285,9 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile 
         l.counters = h->d_counters.p;
         l.work = h->d_work.p;
         l.tstat = adapt ? h->d_tstat.p : nullptr;
-        l.grab = 1;
-        l.pad = 0;
         const int grid = grid_for(h, units, false, l.levels);
+        l.grab = scan_grab(h->k, units, grid);
+        l.pad = 0;
         int rc = h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks) : graph_scan(h, l, it, grid, n, n_blocks);
         if (rc) return rc;
         HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
@@ -412,6 +412,29 @@ int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_
         h->stats.prefixes++;
     }
     return 0;
+}
+
+uint32_t prefixes_fitting(const pcagpu *h, const pca_prefix_t *p, uint32_t n, ScanTile tile)
+{
+    if (h->k <= 1) return n;
+    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
+    const int bk = tile.block_kids;
+    auto blocks = [bk](int nc) { return (uint64_t)((nc + bk - 1) / bk); };
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < n; i++) {
+        const uint32_t live = p[i].live;
+        const int ns = h->ct.count(live, mf_s);
+        uint64_t b = 0;
+        if (h->k == 2) {
+            b = blocks(ns);
+        } else {
+            for (int js = 0; js < ns; js++)
+                b += blocks(h->ct.count(live_after(live, h->ct.at(live, mf_s, js)), mf_s1));
+        }
+        if (total + b > kBlockMax) return i;
+        total += b;
+    }
+    return n;
 }
 
 int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units)

@@ -172,9 +172,21 @@ int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_
 // scan blocks of the staged batch into h_blocks (k >= 2); returns the block
 // count and the unit count, or PCAGPU_EOVERFLOW if they do not fit one launch
 int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units);
+// how many of the n prefixes p (not yet staged) have scan blocks that fit one
+// launch's block buffer together (n when k <= 1)
+uint32_t prefixes_fitting(const pcagpu *h, const pca_prefix_t *p, uint32_t n, ScanTile tile);
 // the scan tile for the handle's current suffix length: fixed by the
 // options, the tuner's winner, or (*cand >= 0) the candidate to try next
 ScanTile current_tile(const pcagpu *h, int *cand);
 // state levels per wave a scan at k suffix positions keeps in LDS
 inline uint32_t scan_levels(int k) { return k < 1 ? 1u : (uint32_t)k; }
+// units a wave takes per atomic on the scan's work counter: runs of about an
+// eighth of a wave's share when units are single prefixes (k <= 1, cheap),
+// one at a time when they are blocks of up to 64 parents each
+inline uint32_t scan_grab(int k, uint64_t units, int grid)
+{
+    if (k >= 2) return 1;
+    const uint64_t waves = (uint64_t)grid * SCAN_WAVES;
+    return (uint32_t)std::min<uint64_t>(std::max<uint64_t>(units / (waves * 8), 1), 32);
+}
 }  // namespace pca

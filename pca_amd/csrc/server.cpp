@@ -101,13 +101,16 @@ int retask(pcagpu *h, const pca_task_t *task)
     return 0;
 }
 
-// The split-depth prefixes of depth D (split D-1): a CPU DFS for splits of
-// at most one level (at most a few hundred prefixes), the device wavefront
-// beyond (the host thread also schedules every other task).
+// The split-depth prefixes of depth D (split D-1, as pcagpu_solve's
+// automatic split: the wavefront stops at the deepest level whose child
+// lists fit kAutoChildren): a CPU DFS for splits of at most one level (at
+// most a few hundred prefixes), the device wavefront beyond (the host
+// thread also schedules every other task).
 int start_depth(Slot &sl)
 {
     pcagpu *h = sl.h;
-    const int D = sl.D, s = D > 0 ? D - 1 : 0;
+    const int D = sl.D;
+    int s = D > 0 ? D - 1 : 0;
     sl.frontier.clear();
     sl.next = 0;
     if (s <= 1) {
@@ -121,13 +124,14 @@ int start_depth(Slot &sl)
         }
         sl.frontier.resize((size_t)n);
     } else {
-        long n = pcagpu_frontier(h, D, s, nullptr, 0);
+        int cur = 0, s_done = s;
+        const long n = gpu_frontier(h, D, s, kAutoChildren, &s_done, &cur);
         if (n < 0) return (int)n;
+        s = s_done;
         sl.frontier.resize((size_t)n);
-        if (n > 0) {
-            long m = pcagpu_frontier(h, D, s, sl.frontier.data(), n);
-            if (m < 0) return (int)m;
-        }
+        if (n > 0)
+            HIP_TRY(hipMemcpy(sl.frontier.data(), h->d_front[cur].p, (size_t)n * sizeof(pca_prefix_t),
+                              hipMemcpyDeviceToHost));
     }
     return pcagpu_begin_depth(h, D, s);
 }
@@ -317,12 +321,13 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     for (Slot &sl : s->slots) {
         if (!sl.busy || sl.next >= sl.frontier.size()) continue;
         pcagpu *h = sl.h;
-        const uint32_t n = (uint32_t)std::min<size_t>(sl.frontier.size() - sl.next, h->batch_cap);
+        int cand;
+        const ScanTile tile = current_tile(h, &cand);   // no tuning inside shared launches
+        uint32_t n = (uint32_t)std::min<size_t>(sl.frontier.size() - sl.next, h->batch_cap);
+        n = std::max<uint32_t>(prefixes_fitting(h, sl.frontier.data() + sl.next, n, tile), 1);
         if ((rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return rc;
         uint32_t nb = 0;
         uint64_t nu = 0;
-        int cand;
-        const ScanTile tile = current_tile(h, &cand);   // no tuning inside shared launches
         if ((rc = build_blocks(h, tile, &nb, &nu)) < 0) return rc;
         HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
                                s->stream));
@@ -352,37 +357,76 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         sl.next += n;
         in_launch.push_back(&sl);
     }
-    // ---- one launch for every task's batch
+    // ---- the launches: an item whose units alone fill the chip several
+    // times over runs in a launch of its own (the one-item kernel, its task
+    // record in the kernel arguments); the others share one persistent
+    // multi-item launch.  A unit of k >= 2 is a block of up to 64 parent
+    // states, of k <= 1 a single one: sizes are compared in blocks.
     if (!s->items.empty()) {
-        if ((rc = s->d_items.ensure(s->items.size()))) return rc;
-        HIP_TRY(hipMemcpyAsync(s->d_items.p, s->items.data(), s->items.size() * sizeof(ScanItem),
-                               hipMemcpyHostToDevice, s->stream));
+        const uint64_t resident = (uint64_t)grid_for(in_launch[0]->h, UINT64_MAX / 2, true, levels) * SCAN_WAVES;
+        std::vector<ScanItem> shared;
+        std::vector<size_t> alone, shared_idx;
+        uint64_t shared_units = 0;
+        uint32_t shared_levels = 1;
+        int shared_k = 0;
+        for (size_t i = 0; i < s->items.size(); i++) {
+            const uint64_t blocks = s->items[i].k >= 2 ? s->items[i].n_units : s->items[i].n_units / 64;
+            if (blocks >= 4 * resident) {
+                alone.push_back(i);
+            } else {
+                ScanItem it = s->items[i];
+                it.unit_begin = shared_units;
+                shared_units += it.n_units;
+                shared_levels = std::max(shared_levels, scan_levels(it.k));
+                shared_k = std::max(shared_k, it.k);
+                shared.push_back(it);
+                shared_idx.push_back(i);
+            }
+        }
         HIP_TRY(hipMemsetAsync(s->d_counters.p, 0, 4 * sizeof(unsigned long long), s->stream));
-        HIP_TRY(hipMemsetAsync(s->d_work.p, 0, sizeof(unsigned long long), s->stream));
-        ScanLaunch l;
-        l.items = s->d_items.p;
-        l.n_items = (uint32_t)s->items.size();
-        l.levels = levels;
-        l.n_units = units;
-        l.cand = cand_dev(in_launch[0]->h);
-        l.counters = s->d_counters.p;
-        l.work = s->d_work.p;
-        l.tstat = nullptr;
-        l.grab = 1;
-        l.pad = 0;
-        HIP_TRY(hipEventRecord(s->ev0, s->stream));
-        launch_scan(l, s->items.data(), in_launch[0]->h->dt,
-                    grid_for(in_launch[0]->h, units, s->items.size() > 1, levels), s->stream);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipEventRecord(s->ev1, s->stream));
-        HIP_TRY(hipStreamSynchronize(s->stream));
-        float ms = 0.f;
-        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        auto run = [&](const ScanItem *items, uint32_t n_items, uint64_t n_units, uint32_t lv, int k,
+                       const DevTask &task0) -> int {
+            ScanLaunch l;
+            l.items = s->d_items.p;
+            l.n_items = n_items;
+            l.levels = lv;
+            l.n_units = n_units;
+            l.cand = cand_dev(in_launch[0]->h);
+            l.counters = s->d_counters.p;
+            l.work = s->d_work.p;
+            l.tstat = nullptr;
+            const int grid = grid_for(in_launch[0]->h, n_units, n_items > 1, lv);
+            l.grab = scan_grab(k, n_units, grid);   // runs of units when they are single prefixes
+            l.pad = 0;
+            if (n_items > 1)
+                HIP_TRY(hipMemcpyAsync(s->d_items.p, items, n_items * sizeof(ScanItem), hipMemcpyHostToDevice,
+                                       s->stream));
+            HIP_TRY(hipMemsetAsync(s->d_work.p, 0, sizeof(unsigned long long), s->stream));
+            HIP_TRY(hipEventRecord(s->ev0, s->stream));
+            launch_scan(l, items, task0, grid, s->stream);
+            HIP_TRY(hipGetLastError());
+            HIP_TRY(hipEventRecord(s->ev1, s->stream));
+            HIP_TRY(hipStreamSynchronize(s->stream));
+            float ms = 0.f;
+            HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+            s->st.launches++;
+            s->st.items += n_items;
+            s->st.kernel_ns += (uint64_t)((double)ms * 1e6);
+            return 0;
+        };
+        if ((rc = s->d_items.ensure(std::max<size_t>(shared.size(), 1)))) return rc;
+        for (size_t i : alone) {
+            ScanItem it = s->items[i];
+            it.unit_begin = 0;
+            if ((rc = run(&it, 1, it.n_units, scan_levels(it.k), it.k, in_launch[i]->h->dt))) return rc;
+        }
+        // (a single shared item also takes the one-item kernel, with its own
+        // task record)
+        if (!shared.empty() && (rc = run(shared.data(), (uint32_t)shared.size(), shared_units, shared_levels,
+                                         shared_k, in_launch[shared_idx[0]]->h->dt)))
+            return rc;
         unsigned long long c[4];
         HIP_TRY(hipMemcpy(c, s->d_counters.p, sizeof(c), hipMemcpyDeviceToHost));
-        s->st.launches++;
-        s->st.items += s->items.size();
-        s->st.kernel_ns += (uint64_t)((double)ms * 1e6);
         s->st.candidates += c[0];
         s->st.suffixes += c[1];
         // ---- collect each task's hits
