@@ -363,6 +363,9 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     const uint8_t *GM = a.cand.g->vg_rdmask[m];
     const uint16_t *GJ = &a.cand.g->vg_j[m][0][0];
     const int t0 = t;
+    // survivor-phase reach / end counts on one parent in eight are enough to
+    // order the tests, at an eighth of their cost
+    const bool tst8 = tst && ((prefix_idx + (a.k >= 2 ? jpath[0] : 0u)) & 7u) == 0u;
     auto drain = [&](uint32_t cnt) {
         const bool valid = (uint32_t)lane < cnt;
         const uint32_t e = valid ? queue[lane] : 0u;
@@ -411,7 +414,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
                 }
             }
             if (!match) ok = 0;
-            if (tst) {
+            if (tst8) {
                 const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
                 if (lane == 0) {
                     tst[tt][0] += before;
