@@ -66,3 +66,18 @@ def test_server_rejects_bad_task_without_stalling(pca):
         g = sv.submit(good, 2)
         recs = {r["id"]: r for r in sv.drain()}
     assert recs[b]["status"] == FAILED and recs[g]["status"] == DONE
+
+
+def test_suite_budget4_server_equals_one_at_a_time(pca):
+    """Config-3 generator at a depth budget of 4: the tasks that need depth 4
+    give frontiers whose scans fill the chip alone (their own launches,
+    capped automatic split) next to many small shared items; every task's
+    optimum and solution count equal the one-at-a-time solver's."""
+    from pca_amd import taskgen, serving
+    tasks = [taskgen.random_task(11, i) for i in range(150)]
+    a, wa, st = serving.suite_run(tasks, 4, max_slots=256, prefixes_per_item=1 << 20)
+    b, wb = serving.sequential_suite(tasks, 4)
+    print(f"[suite] solved {sum(x[0] >= 0 for x in a)}/150 at budget 4, depth 4: {sum(x[0] == 4 for x in a)}; "
+          f"server {wa:.2f}s in {st['launches']} launches, one at a time {wb:.2f}s", flush=True)
+    assert a == b
+    assert sum(x[0] == 4 for x in a) >= 1
