@@ -127,6 +127,7 @@ int scan_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_
         if (hits <= h->hit_cap) break;
         const uint32_t want = hits + hits / 2 + 1024;
         h->d_hits.release();
+        h->hit_cap = 0;   // until the larger buffer exists: launches write no records
         int rc;
         if ((rc = h->d_hits.ensure(want))) return rc;
         h->hit_cap = want;

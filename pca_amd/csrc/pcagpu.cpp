@@ -296,6 +296,7 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile 
         if (o->hits <= h->hit_cap) break;
         const uint32_t want = o->hits + o->hits / 2 + 1024;
         h->d_hits.release();
+        h->hit_cap = 0;   // until the larger buffer exists: launches write no records
         if ((rc = h->d_hits.ensure(want))) return rc;
         h->hit_cap = want;
     }
