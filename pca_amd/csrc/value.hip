@@ -9,8 +9,10 @@
 // whichever dimension is contiguous in memory (the transposed storage modes
 // only change that).  Out-of-range rows, columns and k are zero-filled, so
 // any M, N, K works.  Few output tiles with a long K (weight gradients) are
-// split along K across workgroups with atomic FP32 adds.  The epilogue adds
-// the bias, applies ReLU and stores FP32 and/or BF16.
+// split along K across workgroups with atomic FP32 adds.  Workgroups that
+// read the same rows of A run on one XCD (its L2 serves the slab to all of
+// them).  The epilogue adds the bias, applies ReLU and stores FP32 and/or
+// BF16, four adjacent columns per store.
 //
 // Operand layout of v_mfma_f32_32x32x16_bf16 (lane l of 64): A: row l % 32,
 // k = 8 * (l / 32) .. +7; B: column l % 32, the same k; the 16 accumulators
@@ -73,8 +75,12 @@ __device__ __forceinline__ void stage(uint16_t (*S)[LDS_K], const uint16_t *X, i
             const int gr = r0 + r, gk = k0 + kc;
             if (gr < R && gk < K) load4(X + (int64_t)gr * ld + gk, K - gk, v);
             else v[0] = v[1] = v[2] = v[3] = 0;
-#pragma unroll
-            for (int j = 0; j < 4; j++) S[r][kc + j] = v[j];
+            // the four values are adjacent in the LDS row: one 8-byte store
+            // (rows are 80 bytes, kc a multiple of 4: 8-byte aligned)
+            uint2 q;
+            q.x = (uint32_t)v[0] | ((uint32_t)v[1] << 16);
+            q.y = (uint32_t)v[2] | ((uint32_t)v[3] << 16);
+            *reinterpret_cast<uint2 *>(&S[r][kc]) = q;
         } else {
             const int k = c >> 5, rc = (c & 31) * 4;
             const int gr = r0 + rc, gk = k0 + k;
@@ -86,10 +92,21 @@ __device__ __forceinline__ void stage(uint16_t (*S)[LDS_K], const uint16_t *X, i
     }
 }
 
-// C tile of 128 x 128 per workgroup, 64 x 64 per wave (2 x 2 MFMA blocks);
-// blockIdx.z selects a K range (split K: partial sums added atomically).
+// C tile of 128 x 128 per workgroup, 64 x 64 per wave (2 x 2 MFMA blocks).
+// One-dimensional grid: workgroup b runs on XCD b % 8 (the hardware deals
+// workgroups out round-robin), so b is mapped to (split z, row tile m,
+// column tile n) with every column tile of one row tile -- the workgroups
+// that read the same 128-row slab of A -- on one XCD at consecutive slots,
+// and the slab comes from HBM once into that XCD's L2 instead of once per
+// column tile.  Split K: z selects a K range (partial sums added
+// atomically).
+//
+// The MFMA takes B's slab as its first operand and A's as its second, so it
+// produces C^T blocks: lane l holds, for each group g of four accumulators,
+// C[m0 + l % 32][n0 + 8 g + 4 (l / 32) + 0..3] -- four adjacent columns of
+// one row, stored as one 8-byte (BF16) or 16-byte (FP32) vector.
 template <bool AT, bool BT>
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_per_split,
+__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_per_split, int tiles_m, int tiles_n,
                                                    const uint16_t *__restrict__ A, int lda,
                                                    const uint16_t *__restrict__ B, int ldb,
                                                    const float *__restrict__ bias, int relu, float *C32, int ldc,
@@ -99,10 +116,17 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
     __shared__ __attribute__((aligned(16))) uint16_t Bs[GN][LDS_K];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
-    const int m0 = blockIdx.y * GM, n0 = blockIdx.x * GN;
-    const int kz0 = blockIdx.z * k_per_split;
+    // XCD-aware tile of this workgroup
+    const int groups_m = (tiles_m + 7) / 8;
+    const int per_z = groups_m * 8 * tiles_n;
+    const int bz = (int)blockIdx.x / per_z, b = (int)blockIdx.x - bz * per_z;
+    const int xcd = b & 7, slot = b >> 3;
+    const int tm = (slot / tiles_n) * 8 + xcd, tn = slot % tiles_n;
+    if (tm >= tiles_m) return;
+    const int m0 = tm * GM, n0 = tn * GN;
+    const int kz0 = bz * k_per_split;
     const int kz1 = min(K, kz0 + k_per_split);
-    const bool split = gridDim.z > 1;
+    const bool split = k_per_split < K;
     f32x16 acc[2][2] = {};
     for (int k0 = kz0; k0 < kz1; k0 += GK) {
         stage<!AT>(As, A, lda, m0, M, k0, kz1, tid);   // A: M x K, k contiguous unless transposed
@@ -121,33 +145,65 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
             for (int i = 0; i < 2; i++)
 #pragma unroll
                 for (int j = 0; j < 2; j++)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b[j], a[i], acc[i][j], 0, 0, 0);
         }
         __syncthreads();
     }
+    const bool vec32 = C32 && (ldc & 3) == 0 && (reinterpret_cast<uintptr_t>(C32) & 15u) == 0;
+    const bool vec16 = C16 && (ldc16 & 3) == 0 && (reinterpret_cast<uintptr_t>(C16) & 7u) == 0;
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int col = n0 + wn * 64 + j * 32 + (lane & 31);
-        if (col >= N) continue;
-        const float bv = bias ? bias[col] : 0.f;
+    for (int i = 0; i < 2; i++) {
+        const int row = m0 + wm * 64 + i * 32 + (lane & 31);
+        if (row >= M) continue;
 #pragma unroll
-        for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++)
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int row = m0 + wm * 64 + i * 32 + 8 * (r >> 2) + 4 * (lane >> 5) + (r & 3);
-                if (row >= M) continue;
-                float v = acc[i][j][r];
+            for (int g = 0; g < 4; g++) {
+                const int col = n0 + wn * 64 + j * 32 + 8 * g + 4 * (lane >> 5);
+                if (col >= N) continue;
+                float v[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) v[e] = acc[i][j][4 * g + e];
+                const bool full = col + 3 < N;
                 if (split) {
-                    atomicAdd(C32 + (int64_t)row * ldc + col, v);
+                    for (int e = 0; e < 4; e++)
+                        if (col + e < N) atomicAdd(C32 + (int64_t)row * ldc + col + e, v[e]);
                     continue;
                 }
-                v += bv;
-                if (relu) v = v > 0.f ? v : 0.f;
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    if (bias) v[e] += (col + e < N) ? bias[col + e] : 0.f;
+                    if (relu) v[e] = v[e] > 0.f ? v[e] : 0.f;
+                }
                 if (C32) {
                     float *p = C32 + (int64_t)row * ldc + col;
-                    *p = accumulate ? *p + v : v;
+                    if (full && vec32) {
+                        float4 o = make_float4(v[0], v[1], v[2], v[3]);
+                        if (accumulate) {
+                            const float4 q = *reinterpret_cast<const float4 *>(p);
+                            o.x += q.x;
+                            o.y += q.y;
+                            o.z += q.z;
+                            o.w += q.w;
+                        }
+                        *reinterpret_cast<float4 *>(p) = o;
+                    } else {
+                        for (int e = 0; e < 4; e++)
+                            if (col + e < N) p[e] = accumulate ? p[e] + v[e] : v[e];
+                    }
                 }
-                if (C16) C16[(int64_t)row * ldc16 + col] = to_bf16(v);
+                if (C16) {
+                    uint16_t *p = C16 + (int64_t)row * ldc16 + col;
+                    if (full && vec16) {
+                        uint2 o;
+                        o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                        o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                        *reinterpret_cast<uint2 *>(p) = o;
+                    } else {
+                        for (int e = 0; e < 4; e++)
+                            if (col + e < N) p[e] = to_bf16(v[e]);
+                    }
+                }
             }
     }
 }
@@ -290,7 +346,6 @@ int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, cons
     if (M < 0 || N < 0 || K < 0 || !A || !B || (!C32 && !C16)) return PCAGPU_EINVAL;
     if (M == 0 || N == 0) return PCAGPU_OK;
     const int tiles_m = (M + GM - 1) / GM, tiles_n = (N + GN - 1) / GN;
-    if (tiles_m > 65535) return PCAGPU_EINVAL;
     // split K when few output tiles face a long K (weight gradients): the
     // partial sums go to C32 atomically, so only a plain FP32 output can
     // take it
@@ -298,7 +353,7 @@ int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, cons
     const int tiles = tiles_m * tiles_n;
     if (C32 && !C16 && !bias && !relu && tiles < 512 && K >= 2048)
         splits = std::min(std::min((K + 1023) / 1024, (1024 + tiles - 1) / tiles), 64);
-    const int kps = ((K + splits - 1) / splits + GK - 1) / GK * GK;
+    const int kps = std::max(((K + splits - 1) / splits + GK - 1) / GK * GK, GK);   // K = 0: one empty range
     splits = std::max(1, (K + kps - 1) / kps);
     clear_error();
     if (splits > 1 && !accumulate) {
@@ -306,10 +361,13 @@ int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, cons
         hipLaunchKernelGGL(fill_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st(stream), M, N, ldc,
                            C32);
     }
-    const dim3 grid((unsigned)tiles_n, (unsigned)tiles_m, (unsigned)splits);
-#define PCAV_GEMM(AT, BT)                                                                                       \
-    hipLaunchKernelGGL((gemm_kernel<AT, BT>), grid, dim3(256), 0, st(stream), M, N, K, kps, A, lda, B, ldb, bias, \
-                       relu, C32, ldc, accumulate, C16, ldc16)
+    // one-dimensional grid, XCD-aware tile order inside the kernel
+    const int64_t blocks = (int64_t)splits * ((tiles_m + 7) / 8) * 8 * tiles_n;
+    if (blocks > 0x7FFFFFFF) return PCAGPU_EINVAL;
+    const dim3 grid((unsigned)blocks);
+#define PCAV_GEMM(AT, BT)                                                                                            \
+    hipLaunchKernelGGL((gemm_kernel<AT, BT>), grid, dim3(256), 0, st(stream), M, N, K, kps, tiles_m, tiles_n, A, lda, \
+                       B, ldb, bias, relu, C32, ldc, accumulate, C16, ldc16)
     if (a_trans && b_trans) PCAV_GEMM(true, true);
     else if (a_trans) PCAV_GEMM(true, false);
     else if (b_trans) PCAV_GEMM(false, true);
