@@ -146,6 +146,29 @@ def value_bench(batch=512, steps=30, seed=5):
             "params": net.n_params, "timing": "host clock around device-synchronised repetitions"}
 
 
+def wavefront_bench(task, D=8, s=3):
+    """The GPU wavefront producer (exact-OEP level expansion, the expand_*
+    kernels) of the bench task down to split s of depth D: device time from
+    the handle's HIP events, children and survivors per level summed, and
+    the algorithmic HBM bytes of the table and frontier (per child a 16-byte
+    slot probed in each of two passes plus an 8-byte atomicMin, per
+    survivor its 18-byte record) against the 8 TB/s peak."""
+    from pca_amd import Solver
+    from pca_amd._lib import lib
+    with Solver(task) as sv:
+        lib().pcagpu_frontier(sv._h, D, s, None, 0)   # first call: buffers
+        sv.reset_stats()
+        n = lib().pcagpu_frontier(sv._h, D, s, None, 0)
+        st = sv.stats()
+    secs = st["expand_ns"] * 1e-9
+    children, kept = st["frontier_nodes"], st["frontier_kept"]
+    nbytes = children * 40 + kept * 18
+    return {"workload": f"is_power2_32 wavefront, depth {D}, levels 0..{s}", "prefixes": n, "children": children,
+            "kept": kept, "expand_ms": secs * 1e3, "children_per_s": children / secs,
+            "hbm": {"algorithmic_bytes": nbytes, "achieved_gbs": nbytes / secs / 1e9, "peak_gbs": 8000.0,
+                    "frac": nbytes / secs / 8e12}}
+
+
 def completion_bench(reps=3):
     """Leaf completion (search_exhaustive_complete) on the GPU: the
     two-input fixture states with a budget of 3 instructions (the reference
@@ -318,6 +341,8 @@ def main():
     if not args.no_value:
         line["value_net"] = value_bench()
         print("value net done", file=sys.stderr, flush=True)
+    line["wavefront"] = wavefront_bench(task)
+    print("wavefront done", file=sys.stderr, flush=True)
     if not args.no_complete:
         line["leaf_completion"] = completion_bench()
         print("leaf completion done", file=sys.stderr, flush=True)
