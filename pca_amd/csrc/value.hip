@@ -11,8 +11,11 @@
 // any M, N, K works.  Few output tiles with a long K (weight gradients) are
 // split along K across workgroups with atomic FP32 adds.  Workgroups that
 // read the same rows of A run on one XCD (its L2 serves the slab to all of
-// them).  The epilogue adds the bias, applies ReLU and stores FP32 and/or
-// BF16, four adjacent columns per store.
+// them).  Interior slabs issue all their global loads before the first LDS
+// store waits on one; a row-contiguous slab is transposed in registers in
+// 4 x 4 blocks.  The epilogue adds the bias (staged in LDS), applies ReLU
+// and stores FP32 and/or BF16, four adjacent columns per store; a BF16-only
+// tile goes through LDS and out in 16-byte row runs.
 //
 // Operand layout of v_mfma_f32_32x32x16_bf16 (lane l of 64): A: row l % 32,
 // k = 8 * (l / 32) .. +7; B: column l % 32, the same k; the 16 accumulators
@@ -66,12 +69,72 @@ template <bool KCONTIG>
 __device__ __forceinline__ void stage(uint16_t (*S)[LDS_K], const uint16_t *X, int ld, int r0, int R, int k0, int K,
                                       int tid)
 {
+    constexpr int CHUNKS = GM * GK / 4;   // chunks of 4 elements
+    constexpr int PER_ROW = GK / 4;
+    constexpr int PER_THREAD = CHUNKS / 256;
+    // interior slab, 8-byte aligned rows: every load is issued before the
+    // first LDS store waits on one (the per-chunk bounds checks below would
+    // serialise the loads, one memory round trip each)
+    if (!KCONTIG && r0 + GM <= R && k0 + GK <= K && (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 7u) == 0) {
+        // rows contiguous in memory: each thread loads a 4 (k) x 4 (row)
+        // block as four 8-byte reads along rows, transposes it in registers
+        // and stores four 8-byte k-runs, one per row
+        constexpr int BLOCKS = (GK / 4) * (GM / 4) / 256;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        const int c = tid + i * 256;   // 1024 chunks of 4 elements
+        for (int i = 0; i < BLOCKS; i++) {
+            const int blk = tid + i * 256;
+            const int kb = blk / (GM / 4), rb = blk % (GM / 4);
+            uint2 q[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                q[j] = *reinterpret_cast<const uint2 *>(X + (int64_t)(k0 + 4 * kb + j) * ld + r0 + 4 * rb);
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                // element r of each k-row j: low / high half of word r / 2
+                uint32_t e[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    const uint32_t wv = (r < 2) ? q[j].x : q[j].y;
+                    e[j] = (r & 1) ? (wv >> 16) : (wv & 0xFFFFu);
+                }
+                uint2 o;
+                o.x = e[0] | (e[1] << 16);
+                o.y = e[2] | (e[3] << 16);
+                *reinterpret_cast<uint2 *>(&S[4 * rb + r][4 * kb]) = o;
+            }
+        }
+        return;
+    }
+    if (r0 + GM <= R && k0 + GK <= K && (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 7u) == 0) {
+        uint2 q[PER_THREAD];
+#pragma unroll
+        for (int i = 0; i < PER_THREAD; i++) {
+            const int c = tid + i * 256;
+            const int64_t off = KCONTIG ? (int64_t)(r0 + c / PER_ROW) * ld + k0 + (c % PER_ROW) * 4
+                                        : (int64_t)(k0 + (c >> 5)) * ld + r0 + (c & 31) * 4;
+            q[i] = *reinterpret_cast<const uint2 *>(X + off);
+        }
+#pragma unroll
+        for (int i = 0; i < PER_THREAD; i++) {
+            const int c = tid + i * 256;
+            if (KCONTIG) {
+                *reinterpret_cast<uint2 *>(&S[c / PER_ROW][(c % PER_ROW) * 4]) = q[i];
+            } else {
+                const int k = c >> 5, rc = (c & 31) * 4;
+                S[rc][k] = (uint16_t)q[i].x;
+                S[rc + 1][k] = (uint16_t)(q[i].x >> 16);
+                S[rc + 2][k] = (uint16_t)q[i].y;
+                S[rc + 3][k] = (uint16_t)(q[i].y >> 16);
+            }
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < PER_THREAD; i++) {
+        const int c = tid + i * 256;
         uint16_t v[4];
         if (KCONTIG) {
-            const int r = c >> 3, kc = (c & 7) * 4;
+            const int r = c / PER_ROW, kc = (c % PER_ROW) * 4;
             const int gr = r0 + r, gk = k0 + kc;
             if (gr < R && gk < K) load4(X + (int64_t)gr * ld + gk, K - gk, v);
             else v[0] = v[1] = v[2] = v[3] = 0;
@@ -112,8 +175,13 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
                                                    const float *__restrict__ bias, int relu, float *C32, int ldc,
                                                    int accumulate, uint16_t *C16, int ldc16)
 {
-    __shared__ __attribute__((aligned(16))) uint16_t As[GM][LDS_K];
-    __shared__ __attribute__((aligned(16))) uint16_t Bs[GN][LDS_K];
+    // the K loop's A and B slabs, then (BF16-only outputs) the C tile
+    constexpr int CS_LD = GN + 8;   // C tile row stride in halves: 272 bytes, 16-byte aligned
+    constexpr int SMEM = (2 * GM * LDS_K > GM * CS_LD) ? 2 * GM * LDS_K : GM * CS_LD;
+    __shared__ __attribute__((aligned(16))) uint16_t smem[SMEM];
+    uint16_t (*As)[LDS_K] = reinterpret_cast<uint16_t (*)[LDS_K]>(smem);
+    uint16_t (*Bs)[LDS_K] = reinterpret_cast<uint16_t (*)[LDS_K]>(smem + GM * LDS_K);
+    __shared__ float Bias[GN];   // the tile's bias, read once (0 past N or without bias)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
     // XCD-aware tile of this workgroup
@@ -124,6 +192,8 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
     const int tm = (slot / tiles_n) * 8 + xcd, tn = slot % tiles_n;
     if (tm >= tiles_m) return;
     const int m0 = tm * GM, n0 = tn * GN;
+    if (tid < GN) Bias[tid] = (bias && n0 + tid < N) ? bias[n0 + tid] : 0.f;
+    __syncthreads();
     const int kz0 = bz * k_per_split;
     const int kz1 = min(K, kz0 + k_per_split);
     const bool split = k_per_split < K;
@@ -149,6 +219,47 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
         }
         __syncthreads();
     }
+    if (!split && !C32) {
+        // BF16 output only (the forward activations): the tile goes through
+        // LDS so that global stores are whole 16-byte runs of one row
+        uint16_t (*Cs)[CS_LD] = reinterpret_cast<uint16_t (*)[CS_LD]>(smem);   // the K loop ended on a barrier
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int g = 0; g < 4; g++) {
+                    const int rl = wm * 64 + i * 32 + (lane & 31);
+                    const int cl = wn * 64 + j * 32 + 8 * g + 4 * (lane >> 5);
+                    float v[4];
+#pragma unroll
+                    for (int e = 0; e < 4; e++) {
+                        v[e] = acc[i][j][4 * g + e] + Bias[cl + e];
+                        if (relu) v[e] = v[e] > 0.f ? v[e] : 0.f;
+                    }
+                    uint2 o;
+                    o.x = (uint32_t)to_bf16(v[0]) | ((uint32_t)to_bf16(v[1]) << 16);
+                    o.y = (uint32_t)to_bf16(v[2]) | ((uint32_t)to_bf16(v[3]) << 16);
+                    *reinterpret_cast<uint2 *>(&Cs[rl][cl]) = o;
+                }
+        __syncthreads();
+        const bool vec = (ldc16 & 7) == 0 && (reinterpret_cast<uintptr_t>(C16) & 15u) == 0 && n0 + GN <= N;
+#pragma unroll
+        for (int i = 0; i < GM * GN / 8 / 256; i++) {
+            const int c = tid + i * 256;
+            const int rl = c / (GN / 8), cl = (c % (GN / 8)) * 8;
+            const int row = m0 + rl;
+            if (row >= M) continue;
+            uint16_t *p = C16 + (int64_t)row * ldc16 + n0 + cl;
+            if (vec) {
+                *reinterpret_cast<uint4 *>(p) = *reinterpret_cast<const uint4 *>(&Cs[rl][cl]);
+            } else {
+                for (int e = 0; e < 8; e++)
+                    if (n0 + cl + e < N) p[e] = Cs[rl][cl + e];
+            }
+        }
+        return;
+    }
     const bool vec32 = C32 && (ldc & 3) == 0 && (reinterpret_cast<uintptr_t>(C32) & 15u) == 0;
     const bool vec16 = C16 && (ldc16 & 3) == 0 && (reinterpret_cast<uintptr_t>(C16) & 7u) == 0;
 #pragma unroll
@@ -172,7 +283,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
                 }
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
-                    if (bias) v[e] += (col + e < N) ? bias[col + e] : 0.f;
+                    v[e] += Bias[col - n0 + e];
                     if (relu) v[e] = v[e] > 0.f ? v[e] : 0.f;
                 }
                 if (C32) {
