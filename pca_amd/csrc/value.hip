@@ -155,6 +155,62 @@ __device__ __forceinline__ void stage(uint16_t (*S)[LDS_K], const uint16_t *X, i
     }
 }
 
+// The interior-slab path of stage() split in two, so that a slab's loads
+// can be issued a K step ahead (into registers) and stored after the MFMAs
+// of the current step: slab_interior says whether the fast path applies,
+// load_slab issues the four 8-byte loads of this thread, store_slab writes
+// them to LDS (row-contiguous slabs transposed in registers first).
+template <bool KCONTIG>
+__device__ __forceinline__ bool slab_interior(const uint16_t *X, int ld, int r0, int R, int k0, int K)
+{
+    static_assert(GM * GK / 4 / 256 == 4 && (GK / 4) * (GM / 4) / 256 == 1, "one 4-load unit per thread");
+    return r0 + GM <= R && k0 + GK <= K && (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 7u) == 0;
+}
+
+template <bool KCONTIG>
+__device__ __forceinline__ void load_slab(uint2 q[4], const uint16_t *X, int ld, int r0, int k0, int tid)
+{
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        int64_t off;
+        if (KCONTIG) {
+            const int c = tid + i * 256;
+            off = (int64_t)(r0 + c / (GK / 4)) * ld + k0 + (c % (GK / 4)) * 4;
+        } else {
+            const int kb = tid / (GM / 4), rb = tid % (GM / 4);
+            off = (int64_t)(k0 + 4 * kb + i) * ld + r0 + 4 * rb;
+        }
+        q[i] = *reinterpret_cast<const uint2 *>(X + off);
+    }
+}
+
+template <bool KCONTIG>
+__device__ __forceinline__ void store_slab(uint16_t (*S)[LDS_K], const uint2 q[4], int tid)
+{
+    if (KCONTIG) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int c = tid + i * 256;
+            *reinterpret_cast<uint2 *>(&S[c / (GK / 4)][(c % (GK / 4)) * 4]) = q[i];
+        }
+    } else {
+        const int kb = tid / (GM / 4), rb = tid % (GM / 4);
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            uint32_t e[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const uint32_t wv = (r < 2) ? q[j].x : q[j].y;
+                e[j] = (r & 1) ? (wv >> 16) : (wv & 0xFFFFu);
+            }
+            uint2 o;
+            o.x = e[0] | (e[1] << 16);
+            o.y = e[2] | (e[3] << 16);
+            *reinterpret_cast<uint2 *>(&S[4 * rb + r][4 * kb]) = o;
+        }
+    }
+}
+
 // C tile of 128 x 128 per workgroup, 64 x 64 per wave (2 x 2 MFMA blocks).
 // One-dimensional grid: workgroup b runs on XCD b % 8 (the hardware deals
 // workgroups out round-robin), so b is mapped to (split z, row tile m,
@@ -198,10 +254,32 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
     const int kz1 = min(K, kz0 + k_per_split);
     const bool split = k_per_split < K;
     f32x16 acc[2][2] = {};
+    // A: M x K, k contiguous unless transposed; B: K x N, n contiguous
+    // unless transposed.  qa / qb hold the next interior slab, loaded one K
+    // step ahead; boundary slabs go through stage() when their step comes.
+    uint2 qa[4], qb[4];
+    auto interior = [&](int k) {
+        return k < kz1 && slab_interior<!AT>(A, lda, m0, M, k, kz1) && slab_interior<BT>(B, ldb, n0, N, k, kz1);
+    };
+    bool ahead = interior(kz0);
+    if (ahead) {
+        load_slab<!AT>(qa, A, lda, m0, kz0, tid);
+        load_slab<BT>(qb, B, ldb, n0, kz0, tid);
+    }
     for (int k0 = kz0; k0 < kz1; k0 += GK) {
-        stage<!AT>(As, A, lda, m0, M, k0, kz1, tid);   // A: M x K, k contiguous unless transposed
-        stage<BT>(Bs, B, ldb, n0, N, k0, kz1, tid);    // B: K x N, n contiguous unless transposed
+        if (ahead) {
+            store_slab<!AT>(As, qa, tid);
+            store_slab<BT>(Bs, qb, tid);
+        } else {
+            stage<!AT>(As, A, lda, m0, M, k0, kz1, tid);
+            stage<BT>(Bs, B, ldb, n0, N, k0, kz1, tid);
+        }
         __syncthreads();
+        ahead = interior(k0 + GK);
+        if (ahead) {   // in flight during this step's MFMAs
+            load_slab<!AT>(qa, A, lda, m0, k0 + GK, tid);
+            load_slab<BT>(qb, B, ldb, n0, k0 + GK, tid);
+        }
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
             const int ko = kk + 8 * (lane >> 5);
