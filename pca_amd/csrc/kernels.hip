@@ -864,11 +864,12 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
     }
 }
 
-// One-item launches: register budget of 5 waves per SIMD (at most 96
-// VGPRs), measured faster than the 4 waves of the unconstrained allocation
-// despite a few spills.  Multi-item launches (more live context per unit):
+// One-item launches: register budget of 6 waves per SIMD (at most 80
+// VGPRs), measured on the is_power2_32 depth-8 window faster than 5 waves
+// (96 VGPRs), 7 and 8 despite the spills (10.7 ms per launch against 11.1,
+// 10.8 and 15.3).  Multi-item launches (more live context per unit):
 // 4 waves per SIMD.
-__global__ __launch_bounds__(SCAN_WG, 5) void scan_kernel_one(ScanLaunch lch, ScanItem one, DevTask task1)
+__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanItem one, DevTask task1)
 {
     scan_body<false>(lch, one, task1);
 }
