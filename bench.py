@@ -220,6 +220,52 @@ def completion_cpu_baseline():
             "cores": 1, "kind": "port"}
 
 
+def window(task, D, s, B, warmup, steps, progress=False):
+    """The fixed-depth window: DFS-ordered split-s prefixes of the depth-D
+    tree (host DFS, exact OEP), B per step, each step one flush through the
+    scan kernel (prefixes resident before the timed region).  Returns the
+    wall time of the timed steps, the handle's counters over them and the
+    device name."""
+    import ctypes as C
+    import pca_amd
+    need = B * (warmup + steps)
+    prefixes, _, _ = pca_amd.host_frontier(task, D, s, cap=need)
+    if len(prefixes) < need:
+        raise RuntimeError("not enough prefixes")
+    with pca_amd.Solver(task) as sv:
+        lib = pca_amd.lib()
+        arr = (pca_amd.Prefix * need)()
+        for i, (ins, live) in enumerate(prefixes):
+            for q, v in enumerate(ins):
+                arr[i].insns[q] = v
+            arr[i].num_insns = s
+            arr[i].live = live
+
+        def step(i):
+            rc = lib.pcagpu_begin_depth(sv._h, D, s)
+            assert rc == 0, rc
+            ptr = C.cast(C.addressof(arr) + i * B * C.sizeof(pca_amd.Prefix), C.POINTER(pca_amd.Prefix))
+            rc = lib.pcagpu_submit(sv._h, ptr, B, i * B)
+            assert rc == 0, rc
+            rc = lib.pcagpu_flush(sv._h)
+            assert rc >= 0, rc
+            n = lib.pcagpu_end_depth(sv._h, None, 0)
+            assert n >= 0, n
+
+        for i in range(warmup):
+            step(i)
+            if progress:
+                print(f"warmup step {i} done", file=sys.stderr, flush=True)
+        sv.reset_stats()
+        t0 = time.perf_counter()
+        for i in range(warmup, warmup + steps):
+            step(i)
+            if progress:
+                print(f"step {i - warmup} done", file=sys.stderr, flush=True)
+        t1 = time.perf_counter()
+        return t1 - t0, sv.stats(), sv.device_name
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,45 +294,7 @@ def main():
     task = taskgen.is_power2_32(42)
     task_path = os.path.join(ROOT, "tests", "golden", "tasks", "is_power2_32.json")
     D, s, B = args.depth, args.split, args.batch
-    need = B * (args.warmup + args.steps)
-    prefixes, _, _ = pca_amd.host_frontier(task, D, s, cap=need)
-    if len(prefixes) < need:
-        raise RuntimeError("not enough prefixes")
-
-    with pca_amd.Solver(task) as sv:
-        lib = pca_amd.lib()
-        import ctypes as C
-        arr = (pca_amd.Prefix * need)()
-        for i, (ins, live) in enumerate(prefixes):
-            for q, v in enumerate(ins):
-                arr[i].insns[q] = v
-            arr[i].num_insns = s
-            arr[i].live = live
-
-        def step(i):
-            rc = lib.pcagpu_begin_depth(sv._h, D, s)
-            assert rc == 0, rc
-            ptr = C.cast(C.addressof(arr) + i * B * C.sizeof(pca_amd.Prefix), C.POINTER(pca_amd.Prefix))
-            rc = lib.pcagpu_submit(sv._h, ptr, B, i * B)
-            assert rc == 0, rc
-            rc = lib.pcagpu_flush(sv._h)
-            assert rc >= 0, rc
-            n = lib.pcagpu_end_depth(sv._h, None, 0)
-            assert n >= 0, n
-
-        for i in range(args.warmup):
-            step(i)
-            print(f"warmup step {i} done", file=sys.stderr, flush=True)
-        sv.reset_stats()
-        t0 = time.perf_counter()
-        for i in range(args.warmup, args.warmup + args.steps):
-            step(i)
-            print(f"step {i - args.warmup} done", file=sys.stderr, flush=True)
-        t1 = time.perf_counter()
-        st = sv.stats()
-        dev = sv.device_name
-
-    wall = t1 - t0
+    wall, st, dev = window(task, D, s, B, args.warmup, args.steps, progress=True)
     cand = st["candidates"]
     launches = max(st["launches"], 1)
     kernel_s = st["kernel_ns"] * 1e-9
