@@ -9,7 +9,8 @@
 // whichever dimension is contiguous in memory (the transposed storage modes
 // only change that).  Out-of-range rows, columns and k are zero-filled, so
 // any M, N, K works.  Few output tiles with a long K (weight gradients) are
-// split along K across workgroups with atomic FP32 adds.  Workgroups that
+// split along K across workgroups, each range's partial tile stored to a
+// workspace slice and the slices summed by a second kernel.  Workgroups that
 // read the same rows of A run on one XCD (its L2 serves the slab to all of
 // them).  Interior slabs issue all their global loads before the first LDS
 // store waits on one; a row-contiguous slab is transposed in registers in
@@ -24,6 +25,9 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <utility>
 
 #include "../../include/pcagpu.h"
 #include "../../include/pcavalue.h"
@@ -155,32 +159,37 @@ __device__ __forceinline__ void stage(uint16_t (*S)[LDS_K], const uint16_t *X, i
     }
 }
 
-// The interior-slab path of stage() split in two, so that a slab's loads
-// can be issued a K step ahead (into registers) and stored after the MFMAs
-// of the current step: slab_interior says whether the fast path applies,
-// load_slab issues the four 8-byte loads of this thread, store_slab writes
-// them to LDS (row-contiguous slabs transposed in registers first).
+// The fast path of stage() split in two, so that a slab's loads can be
+// issued a K step ahead (into registers) and stored after the MFMAs of the
+// current step.  slab_fast: 8-byte aligned rows and row / k bounds that are
+// multiples of 4, so that every 4-element piece is wholly inside or wholly
+// outside the matrix; load_slab issues this thread's four 8-byte loads
+// (zeros for pieces outside), store_slab writes them to LDS (row-contiguous
+// slabs transposed in registers first).
 template <bool KCONTIG>
-__device__ __forceinline__ bool slab_interior(const uint16_t *X, int ld, int r0, int R, int k0, int K)
+__device__ __forceinline__ bool slab_fast(const uint16_t *X, int ld, int R, int k0, int K)
 {
     static_assert(GM * GK / 4 / 256 == 4 && (GK / 4) * (GM / 4) / 256 == 1, "one 4-load unit per thread");
-    return r0 + GM <= R && k0 + GK <= K && (ld & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 7u) == 0;
+    return k0 < K && (ld & 3) == 0 && (R & 3) == 0 && (K & 3) == 0 && (reinterpret_cast<uintptr_t>(X) & 7u) == 0;
 }
 
 template <bool KCONTIG>
-__device__ __forceinline__ void load_slab(uint2 q[4], const uint16_t *X, int ld, int r0, int k0, int tid)
+__device__ __forceinline__ void load_slab(uint2 q[4], const uint16_t *X, int ld, int r0, int R, int k0, int K,
+                                          int tid)
 {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        int64_t off;
+        int r, k;
         if (KCONTIG) {
             const int c = tid + i * 256;
-            off = (int64_t)(r0 + c / (GK / 4)) * ld + k0 + (c % (GK / 4)) * 4;
+            r = r0 + c / (GK / 4);
+            k = k0 + (c % (GK / 4)) * 4;
         } else {
-            const int kb = tid / (GM / 4), rb = tid % (GM / 4);
-            off = (int64_t)(k0 + 4 * kb + i) * ld + r0 + 4 * rb;
+            r = r0 + 4 * (tid % (GM / 4));
+            k = k0 + 4 * (tid / (GM / 4)) + i;
         }
-        q[i] = *reinterpret_cast<const uint2 *>(X + off);
+        const int64_t off = KCONTIG ? (int64_t)r * ld + k : (int64_t)k * ld + r;
+        q[i] = (r < R && k < K) ? *reinterpret_cast<const uint2 *>(X + off) : make_uint2(0u, 0u);
     }
 }
 
@@ -229,7 +238,7 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
                                                    const uint16_t *__restrict__ A, int lda,
                                                    const uint16_t *__restrict__ B, int ldb,
                                                    const float *__restrict__ bias, int relu, float *C32, int ldc,
-                                                   int accumulate, uint16_t *C16, int ldc16)
+                                                   int accumulate, uint16_t *C16, int ldc16, int64_t split_stride)
 {
     // the K loop's A and B slabs, then (BF16-only outputs) the C tile
     constexpr int CS_LD = GN + 8;   // C tile row stride in halves: 272 bytes, 16-byte aligned
@@ -240,13 +249,14 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
     __shared__ float Bias[GN];   // the tile's bias, read once (0 past N or without bias)
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int wm = w >> 1, wn = w & 1;
-    // XCD-aware tile of this workgroup
-    const int groups_m = (tiles_m + 7) / 8;
-    const int per_z = groups_m * 8 * tiles_n;
-    const int bz = (int)blockIdx.x / per_z, b = (int)blockIdx.x - bz * per_z;
-    const int xcd = b & 7, slot = b >> 3;
-    const int tm = (slot / tiles_n) * 8 + xcd, tn = slot % tiles_n;
-    if (tm >= tiles_m) return;
+    // XCD-aware tile of this workgroup: the row slabs (split z, row tile
+    // tm), numbered z * tiles_m + tm, are dealt to the XCDs round-robin, and
+    // all column tiles of one row slab sit on its XCD at consecutive slots
+    const int xcd = (int)blockIdx.x & 7, slot = (int)blockIdx.x >> 3;
+    const int mz = (slot / tiles_n) * 8 + xcd, tn = slot % tiles_n;
+    const int n_slabs = tiles_m * (K > k_per_split ? (K + k_per_split - 1) / k_per_split : 1);
+    if (mz >= n_slabs) return;
+    const int bz = mz / tiles_m, tm = mz - bz * tiles_m;
     const int m0 = tm * GM, n0 = tn * GN;
     if (tid < GN) Bias[tid] = (bias && n0 + tid < N) ? bias[n0 + tid] : 0.f;
     __syncthreads();
@@ -255,16 +265,15 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
     const bool split = k_per_split < K;
     f32x16 acc[2][2] = {};
     // A: M x K, k contiguous unless transposed; B: K x N, n contiguous
-    // unless transposed.  qa / qb hold the next interior slab, loaded one K
-    // step ahead; boundary slabs go through stage() when their step comes.
+    // unless transposed.  qa / qb hold the next slab, loaded one K step
+    // ahead when both operands take the fast path; other slabs go through
+    // stage() when their step comes.
     uint2 qa[4], qb[4];
-    auto interior = [&](int k) {
-        return k < kz1 && slab_interior<!AT>(A, lda, m0, M, k, kz1) && slab_interior<BT>(B, ldb, n0, N, k, kz1);
-    };
-    bool ahead = interior(kz0);
+    auto fast = [&](int k) { return slab_fast<!AT>(A, lda, M, k, kz1) && slab_fast<BT>(B, ldb, N, k, kz1); };
+    bool ahead = fast(kz0);
     if (ahead) {
-        load_slab<!AT>(qa, A, lda, m0, kz0, tid);
-        load_slab<BT>(qb, B, ldb, n0, kz0, tid);
+        load_slab<!AT>(qa, A, lda, m0, M, kz0, kz1, tid);
+        load_slab<BT>(qb, B, ldb, n0, N, kz0, kz1, tid);
     }
     for (int k0 = kz0; k0 < kz1; k0 += GK) {
         if (ahead) {
@@ -275,10 +284,10 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
             stage<BT>(Bs, B, ldb, n0, N, k0, kz1, tid);
         }
         __syncthreads();
-        ahead = interior(k0 + GK);
+        ahead = fast(k0 + GK);
         if (ahead) {   // in flight during this step's MFMAs
-            load_slab<!AT>(qa, A, lda, m0, k0 + GK, tid);
-            load_slab<BT>(qb, B, ldb, n0, k0 + GK, tid);
+            load_slab<!AT>(qa, A, lda, m0, M, k0 + GK, kz1, tid);
+            load_slab<BT>(qb, B, ldb, n0, N, k0 + GK, kz1, tid);
         }
 #pragma unroll
         for (int kk = 0; kk < GK; kk += 16) {
@@ -338,6 +347,12 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
         }
         return;
     }
+    // split K: this range's partial tile goes to its own slice of the
+    // workspace (plain stores), summed by a second kernel
+    if (split) {
+        C32 += bz * split_stride;
+        accumulate = 0;
+    }
     const bool vec32 = C32 && (ldc & 3) == 0 && (reinterpret_cast<uintptr_t>(C32) & 15u) == 0;
     const bool vec16 = C16 && (ldc16 & 3) == 0 && (reinterpret_cast<uintptr_t>(C16) & 7u) == 0;
 #pragma unroll
@@ -354,11 +369,6 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
 #pragma unroll
                 for (int e = 0; e < 4; e++) v[e] = acc[i][j][4 * g + e];
                 const bool full = col + 3 < N;
-                if (split) {
-                    for (int e = 0; e < 4; e++)
-                        if (col + e < N) atomicAdd(C32 + (int64_t)row * ldc + col + e, v[e]);
-                    continue;
-                }
 #pragma unroll
                 for (int e = 0; e < 4; e++) {
                     v[e] += Bias[col - n0 + e];
@@ -397,11 +407,26 @@ __global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_pe
     }
 }
 
-// zeroes rows x cols of a row-major FP32 matrix with leading dimension ld
-__global__ void fill_rows_kernel(int rows, int cols, int ld, float *p)
+// C[m][n] (+)= sum over s of W[s][m][n] (split-K partial tiles, ld N)
+__global__ void reduce_splits_kernel(int M, int N, int splits, const float *__restrict__ W, float *C, int ldc,
+                                     int accumulate)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < (int64_t)rows * cols) p[(i / cols) * ld + (i % cols)] = 0.f;
+    const int64_t MN = (int64_t)M * N;
+    if (i >= MN) return;
+    // eight slices per round: their loads are all in flight together
+    float v = 0.f;
+    int z = 0;
+    for (; z + 8 <= splits; z += 8) {
+        float t[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) t[u] = W[(z + u) * MN + i];
+#pragma unroll
+        for (int u = 0; u < 8; u++) v += t[u];
+    }
+    for (; z < splits; z++) v += W[z * MN + i];
+    float *p = C + (i / N) * ldc + (i % N);
+    *p = accumulate ? *p + v : v;
 }
 
 __global__ void relu_backward_kernel(int64_t n, const uint16_t *act, float *grad, uint16_t *out)
@@ -518,6 +543,29 @@ __global__ void to_bf16_kernel(int64_t n, const float *x, uint16_t *y)
 }
 
 inline hipStream_t st(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Split-K workspace of the current device, grown on demand and kept until
+// the process ends.  Growing waits for the device first: launches still
+// queued on any stream may read the old buffer.
+float *split_workspace(size_t floats)
+{
+    static std::mutex mu;
+    static std::map<int, std::pair<float *, size_t>> ws;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto &e = ws[dev];
+    if (e.second >= floats) return e.first;
+    if (e.first) {
+        if (hipDeviceSynchronize() != hipSuccess) return nullptr;
+        (void)hipFree(e.first);
+        e = {nullptr, 0};
+    }
+    float *p = nullptr;
+    if (hipMalloc((void **)&p, floats * sizeof(float)) != hipSuccess) return nullptr;
+    e = {p, floats};
+    return p;
+}
 inline int grid1d(int64_t n) { return (int)std::min<int64_t>((n + 255) / 256, 65535); }
 // error state of the launches just made; an entry point first clears any
 // error a previous, unrelated runtime call of this thread left behind
@@ -535,33 +583,39 @@ int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, cons
     if (M < 0 || N < 0 || K < 0 || !A || !B || (!C32 && !C16)) return PCAGPU_EINVAL;
     if (M == 0 || N == 0) return PCAGPU_OK;
     const int tiles_m = (M + GM - 1) / GM, tiles_n = (N + GN - 1) / GN;
-    // split K when few output tiles face a long K (weight gradients): the
-    // partial sums go to C32 atomically, so only a plain FP32 output can
-    // take it
+    // split K when few output tiles face a long K (weight gradients): about
+    // 512 workgroups, each K range at least 256 long; the partial tiles go
+    // to a workspace and are summed into C32, so only a plain FP32 output
+    // can take it
     int splits = 1;
     const int tiles = tiles_m * tiles_n;
-    if (C32 && !C16 && !bias && !relu && tiles < 512 && K >= 2048)
-        splits = std::min(std::min((K + 1023) / 1024, (1024 + tiles - 1) / tiles), 64);
+    if (C32 && !C16 && !bias && !relu && tiles < 256 && K >= 1024)
+        splits = std::min(std::min((512 + tiles - 1) / tiles, K / 256), 64);
     const int kps = std::max(((K + splits - 1) / splits + GK - 1) / GK * GK, GK);   // K = 0: one empty range
     splits = std::max(1, (K + kps - 1) / kps);
+    float *ws = nullptr;
+    if (splits > 1 && !(ws = split_workspace((size_t)splits * M * N))) return PCAGPU_ENOMEM;
     clear_error();
-    if (splits > 1 && !accumulate) {
-        const int64_t n = (int64_t)M * N;
-        hipLaunchKernelGGL(fill_rows_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st(stream), M, N, ldc,
-                           C32);
-    }
     // one-dimensional grid, XCD-aware tile order inside the kernel
-    const int64_t blocks = (int64_t)splits * ((tiles_m + 7) / 8) * 8 * tiles_n;
+    const int64_t blocks = ((int64_t)splits * tiles_m + 7) / 8 * 8 * tiles_n;
     if (blocks > 0x7FFFFFFF) return PCAGPU_EINVAL;
     const dim3 grid((unsigned)blocks);
+    float *Cout = ws ? ws : C32;
+    const int ldo = ws ? N : ldc;
+    const int64_t stride = ws ? (int64_t)M * N : 0;
 #define PCAV_GEMM(AT, BT)                                                                                            \
     hipLaunchKernelGGL((gemm_kernel<AT, BT>), grid, dim3(256), 0, st(stream), M, N, K, kps, tiles_m, tiles_n, A, lda, \
-                       B, ldb, bias, relu, C32, ldc, accumulate, C16, ldc16)
+                       B, ldb, bias, relu, Cout, ldo, accumulate, C16, ldc16, stride)
     if (a_trans && b_trans) PCAV_GEMM(true, true);
     else if (a_trans) PCAV_GEMM(true, false);
     else if (b_trans) PCAV_GEMM(false, true);
     else PCAV_GEMM(false, false);
 #undef PCAV_GEMM
+    if (ws) {
+        const int64_t n = (int64_t)M * N;
+        hipLaunchKernelGGL(reduce_splits_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st(stream), M, N,
+                           splits, ws, C32, ldc, accumulate);
+    }
     return status();
 }
 
