@@ -302,7 +302,13 @@ typedef struct {
     uint32_t max_slots;          /* tasks resident at once (default 64)        */
     uint64_t hbm_budget;         /* device bytes the slots may hold (default 200 GiB) */
     uint32_t prefixes_per_item;  /* prefixes a task adds to one launch (default 16384) */
-    uint32_t reserved[4];
+    uint32_t warm_slots;         /* slots opened by server_open, so the first
+                                    requests pay no allocation (default 4;
+                                    0xFFFFFFFF: none)                          */
+    uint32_t step_prefixes;      /* prefixes one step scans over all tasks: a
+                                    task adds max(prefixes_per_item,
+                                    step_prefixes / active tasks) (default 262144) */
+    uint32_t reserved[2];
 } pcagpu_server_opts_t;
 
 typedef struct {

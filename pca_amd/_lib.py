@@ -160,7 +160,8 @@ class SolveRes(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("max_slots", C.c_uint32), ("hbm_budget", C.c_uint64),
-                ("prefixes_per_item", C.c_uint32), ("reserved", C.c_uint32 * 4)]
+                ("prefixes_per_item", C.c_uint32), ("warm_slots", C.c_uint32), ("step_prefixes", C.c_uint32),
+                ("reserved", C.c_uint32 * 2)]
 
 
 class Request(C.Structure):

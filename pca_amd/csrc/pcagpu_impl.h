@@ -145,6 +145,10 @@ namespace pca {
 int fill_devtask(pcagpu *h);
 // device test order: h->dt's per-test fields and h->order from `order`
 void set_test_order(pcagpu *h, const std::vector<int> &order);
+// Folds one launch's per-position test counts (2 per position) into the
+// task's statistics and re-sorts its test order; uploads the task record
+// if the order changed.
+int adapt_test_order(pcagpu *h, const unsigned long long *ts);
 // the grammar tables of the current device, uploaded on first use and kept
 // until the process ends (read-only, shared by every handle on the device)
 const GrammarDev *shared_grammar_dev(int device, bool branchless = false);

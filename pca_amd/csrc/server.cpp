@@ -7,11 +7,18 @@
 // against an HBM budget) and runs a scheduling loop:
 //
 //   admit queued requests into free slots while slots and budget allow
-//   every active task contributes its next batch of split-depth prefixes
+//   every active task contributes its next batch of split-depth prefixes,
+//   its share of the step (prefixes_per_item, or more when few tasks are
+//   active: step_prefixes split over them)
 //   ONE persistent scan launch processes the batches of all tasks
 //   tasks whose depth is complete are resolved on the host (exact OEP,
 //   canonical order, CPU re-verification); solved ones free their slot,
 //   the others move to the next depth
+//
+// Slots are opened ahead (warm_slots) and re-tasked per request, so a
+// request pays no device allocation; a step zeroes and reads back one
+// control block (counters, hit counts, test statistics) for all its
+// launches, one host wait per step.
 //
 // Requests arriving while others are on the GPU join at the next step, so
 // uneven tasks overlap in the same launches.  Each task's result is exactly
@@ -67,9 +74,21 @@ struct pcagpu_server {
     uint64_t next_id = 1;
     double t0 = 0;
     DevBuf<ScanItem> d_items;
-    DevBuf<unsigned long long> d_counters, d_work;
+    // one step's control block, zeroed once and read back once: scan
+    // counters [0,4), a work counter per launch, a 32-bit hit count per
+    // item, then per launch the per-test counts of a one-item launch
+    DevBuf<unsigned long long> d_ctl;
+    unsigned long long *h_ctl = nullptr;   // pinned mirror
     std::vector<ScanItem> items;
     pcagpu_server_stats_t st;
+
+    size_t max_launches() const { return (size_t)opts.max_slots + 1; }
+    size_t hit_word() const { return 4 + max_launches(); }
+    size_t tstat_word(size_t launch) const { return hit_word() + (opts.max_slots + 1) / 2 + launch * 2 * MAXT; }
+    size_t ctl_words() const { return tstat_word(max_launches()); }
+    unsigned long long *work_dev(size_t launch) { return d_ctl.p + 4 + launch; }
+    uint32_t *hit_count_dev(size_t item) { return reinterpret_cast<uint32_t *>(d_ctl.p + hit_word()) + item; }
+    uint32_t hit_count_host(size_t item) const { return reinterpret_cast<const uint32_t *>(h_ctl + hit_word())[item]; }
 };
 
 namespace {
@@ -185,6 +204,35 @@ int advance(pcagpu_server *s, Slot &sl)
     }
 }
 
+// A new resident slot on s's device, holding `task` (the caller re-tasks
+// it per request).  Its prefix batch holds one step's largest share.
+int open_slot(pcagpu_server *s, const pca_task_t *task, Slot **out)
+{
+    pcagpu_opts_t o;
+    std::memset(&o, 0, sizeof(o));
+    o.device = s->opts.device;
+    o.batch_capacity = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes);
+    o.hit_capacity = 1u << 16;
+    pcagpu *h = nullptr;
+    const int rc = pcagpu_open(task, &o, &h);
+    if (rc < 0) return rc;
+    s->slots.push_back(Slot());
+    s->slots.back().h = h;
+    s->st.slots_opened++;
+    *out = &s->slots.back();
+    return 0;
+}
+
+// True if one more slot fits max_slots and the HBM budget (the first slot
+// always does, so a small budget serialises instead of stalling).
+bool slot_fits(const pcagpu_server *s)
+{
+    if (s->slots.size() >= s->opts.max_slots) return false;
+    if (s->slots.empty()) return true;
+    const size_t per = slots_bytes(s) / s->slots.size();
+    return slots_bytes(s) + per <= s->opts.hbm_budget;
+}
+
 int admit(pcagpu_server *s)
 {
     while (!s->queue.empty()) {
@@ -192,19 +240,9 @@ int admit(pcagpu_server *s)
         for (Slot &sl : s->slots)
             if (!sl.busy && sl.h) { free_slot = &sl; break; }
         if (!free_slot) {
-            if (s->slots.size() >= s->opts.max_slots) return 0;
-            // a new resident slot must fit the HBM budget (the first one is
-            // always allowed, so a small budget serialises instead of stalling)
-            const size_t per = s->slots.empty() ? (size_t)64 << 20 : slots_bytes(s) / s->slots.size();
-            if (!s->slots.empty() && slots_bytes(s) + per > s->opts.hbm_budget) return 0;
+            if (!slot_fits(s)) return 0;
             Request &r = s->reqs[s->queue.front()];
-            pcagpu_opts_t o;
-            std::memset(&o, 0, sizeof(o));
-            o.device = s->opts.device;
-            o.batch_capacity = s->opts.prefixes_per_item;
-            o.hit_capacity = 1u << 16;
-            pcagpu *h = nullptr;
-            int rc = pcagpu_open(&r.task, &o, &h);
+            int rc = open_slot(s, &r.task, &free_slot);
             if (rc == PCAGPU_EINVAL) {
                 // the task itself is unsupported: fail this request only
                 const uint64_t bad = s->queue.front();
@@ -217,10 +255,6 @@ int admit(pcagpu_server *s)
                 continue;
             }
             if (rc < 0) return rc;
-            s->slots.push_back(Slot());
-            s->slots.back().h = h;
-            free_slot = &s->slots.back();
-            s->st.slots_opened++;
         }
         const uint64_t id = s->queue.front();
         s->queue.pop_front();
@@ -260,6 +294,9 @@ extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_serve
     if (!s->opts.max_slots) s->opts.max_slots = 64;
     if (!s->opts.hbm_budget) s->opts.hbm_budget = (uint64_t)200 << 30;
     if (!s->opts.prefixes_per_item) s->opts.prefixes_per_item = 1u << 14;
+    if (!s->opts.step_prefixes) s->opts.step_prefixes = 1u << 18;
+    const uint32_t warm = s->opts.warm_slots == 0xFFFFFFFFu ? 0u
+                          : std::min(s->opts.warm_slots ? s->opts.warm_slots : 4u, s->opts.max_slots);
     std::memset(&s->st, 0, sizeof(s->st));
     s->t0 = now_s();
     if (hipSetDevice(s->opts.device) != hipSuccess ||
@@ -268,9 +305,27 @@ extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_serve
         pcagpu_server_close(s);
         return PCAGPU_EHIP;
     }
-    if (s->d_items.ensure(s->opts.max_slots) || s->d_counters.ensure(8) || s->d_work.ensure(1)) {
+    if (s->d_items.ensure(s->opts.max_slots) || s->d_ctl.ensure(s->ctl_words()) ||
+        hipHostMalloc((void **)&s->h_ctl, s->ctl_words() * sizeof(unsigned long long)) != hipSuccess) {
         pcagpu_server_close(s);
         return PCAGPU_ENOMEM;
+    }
+    // warm slots: opened now on a placeholder task (one input, one test) so
+    // the first requests are only re-tasked
+    pca_task_t ph;
+    std::memset(&ph, 0, sizeof(ph));
+    ph.num_inputs = 1;
+    ph.num_outputs = 1;
+    ph.input_ports[0] = 0;
+    ph.output_ports[0] = 0;
+    ph.num_tests = 1;
+    for (uint32_t i = 0; i < warm && slot_fits(s); i++) {
+        Slot *sl = nullptr;
+        const int rc = open_slot(s, &ph, &sl);
+        if (rc < 0) {
+            pcagpu_server_close(s);
+            return rc;
+        }
     }
     *out = s;
     return 0;
@@ -281,8 +336,8 @@ extern "C" void pcagpu_server_close(pcagpu_server_t *s)
     if (!s) return;
     for (Slot &sl : s->slots) pcagpu_close(sl.h);
     s->d_items.release();
-    s->d_counters.release();
-    s->d_work.release();
+    s->d_ctl.release();
+    if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -313,17 +368,22 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     (void)hipSetDevice(s->opts.device);
     int rc = admit(s);
     if (rc < 0) return rc;
-    // ---- every active task stages its next batch
+    // ---- every active task stages its next batch: its share of the step
+    // (at least prefixes_per_item, more when few tasks are active, so a task
+    // running alone needs few steps)
     s->items.clear();
     uint64_t units = 0;
     uint32_t levels = 1;
     std::vector<Slot *> in_launch;
+    uint32_t active = 0;
+    for (const Slot &sl : s->slots) active += sl.busy && sl.next < sl.frontier.size();
+    const uint32_t share = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes / std::max(active, 1u));
     for (Slot &sl : s->slots) {
         if (!sl.busy || sl.next >= sl.frontier.size()) continue;
         pcagpu *h = sl.h;
         int cand;
         const ScanTile tile = current_tile(h, &cand);   // no tuning inside shared launches
-        uint32_t n = (uint32_t)std::min<size_t>(sl.frontier.size() - sl.next, h->batch_cap);
+        uint32_t n = (uint32_t)std::min<size_t>({sl.frontier.size() - sl.next, (size_t)h->batch_cap, (size_t)share});
         n = std::max<uint32_t>(prefixes_fitting(h, sl.frontier.data() + sl.next, n, tile), 1);
         if ((rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return rc;
         uint32_t nb = 0;
@@ -335,7 +395,6 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)nb * sizeof(ScanBlock),
                                    hipMemcpyHostToDevice, s->stream));
         levels = std::max(levels, scan_levels(h->k));
-        HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), s->stream));
         ScanItem it;
         std::memset(&it, 0, sizeof(it));
         it.task = h->d_task.p;
@@ -349,7 +408,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         it.k = h->k;
         it.tile = tile;
         it.hits = h->d_hits.p;
-        it.hit_count = h->d_hit_count.p;
+        it.hit_count = s->hit_count_dev(in_launch.size());
         it.hit_cap = h->hit_cap;
         s->items.push_back(it);
         units += nu;
@@ -361,7 +420,9 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     // times over runs in a launch of its own (the one-item kernel, its task
     // record in the kernel arguments); the others share one persistent
     // multi-item launch.  A unit of k >= 2 is a block of up to 64 parent
-    // states, of k <= 1 a single one: sizes are compared in blocks.
+    // states, of k <= 1 a single one: sizes are compared in blocks.  The
+    // launches go back to back on the stream; the host waits once, for the
+    // control block's copy.
     if (!s->items.empty()) {
         const uint64_t resident = (uint64_t)grid_for(in_launch[0]->h, UINT64_MAX / 2, true, levels) * SCAN_WAVES;
         std::vector<ScanItem> shared;
@@ -383,57 +444,67 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
                 shared_idx.push_back(i);
             }
         }
-        HIP_TRY(hipMemsetAsync(s->d_counters.p, 0, 4 * sizeof(unsigned long long), s->stream));
+        if ((rc = s->d_items.ensure(std::max<size_t>(shared.size(), 1)))) return rc;
+        const size_t ctl_bytes = s->ctl_words() * sizeof(unsigned long long);
+        HIP_TRY(hipMemsetAsync(s->d_ctl.p, 0, ctl_bytes, s->stream));
+        HIP_TRY(hipEventRecord(s->ev0, s->stream));
+        uint32_t n_launch = 0;
+        // one-item launches count test outcomes per launch; the task's test
+        // order adapts to them as in pcagpu_solve (multi-item launches mix
+        // tasks and count nothing)
+        std::vector<std::pair<uint32_t, Slot *>> counted;
         auto run = [&](const ScanItem *items, uint32_t n_items, uint64_t n_units, uint32_t lv, int k,
-                       const DevTask &task0) -> int {
+                       const DevTask &task0, Slot *one) -> int {
+            if (n_items == 1) counted.emplace_back(n_launch, one);
             ScanLaunch l;
             l.items = s->d_items.p;
             l.n_items = n_items;
             l.levels = lv;
             l.n_units = n_units;
             l.cand = cand_dev(in_launch[0]->h);
-            l.counters = s->d_counters.p;
-            l.work = s->d_work.p;
-            l.tstat = nullptr;
+            l.counters = s->d_ctl.p;
+            l.tstat = n_items == 1 ? s->d_ctl.p + s->tstat_word(n_launch) : nullptr;
+            l.work = s->work_dev(n_launch++);
             const int grid = grid_for(in_launch[0]->h, n_units, n_items > 1, lv);
             l.grab = scan_grab(k, n_units, grid);   // runs of units when they are single prefixes
             l.pad = 0;
             if (n_items > 1)
                 HIP_TRY(hipMemcpyAsync(s->d_items.p, items, n_items * sizeof(ScanItem), hipMemcpyHostToDevice,
                                        s->stream));
-            HIP_TRY(hipMemsetAsync(s->d_work.p, 0, sizeof(unsigned long long), s->stream));
-            HIP_TRY(hipEventRecord(s->ev0, s->stream));
             launch_scan(l, items, task0, grid, s->stream);
             HIP_TRY(hipGetLastError());
-            HIP_TRY(hipEventRecord(s->ev1, s->stream));
-            HIP_TRY(hipStreamSynchronize(s->stream));
-            float ms = 0.f;
-            HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
             s->st.launches++;
             s->st.items += n_items;
-            s->st.kernel_ns += (uint64_t)((double)ms * 1e6);
             return 0;
         };
-        if ((rc = s->d_items.ensure(std::max<size_t>(shared.size(), 1)))) return rc;
         for (size_t i : alone) {
             ScanItem it = s->items[i];
             it.unit_begin = 0;
-            if ((rc = run(&it, 1, it.n_units, scan_levels(it.k), it.k, in_launch[i]->h->dt))) return rc;
+            if ((rc = run(&it, 1, it.n_units, scan_levels(it.k), it.k, in_launch[i]->h->dt, in_launch[i])))
+                return rc;
         }
         // (a single shared item also takes the one-item kernel, with its own
         // task record)
         if (!shared.empty() && (rc = run(shared.data(), (uint32_t)shared.size(), shared_units, shared_levels,
-                                         shared_k, in_launch[shared_idx[0]]->h->dt)))
+                                         shared_k, in_launch[shared_idx[0]]->h->dt, in_launch[shared_idx[0]])))
             return rc;
-        unsigned long long c[4];
-        HIP_TRY(hipMemcpy(c, s->d_counters.p, sizeof(c), hipMemcpyDeviceToHost));
-        s->st.candidates += c[0];
-        s->st.suffixes += c[1];
+        HIP_TRY(hipEventRecord(s->ev1, s->stream));
+        HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl.p, s->tstat_word(n_launch) * sizeof(unsigned long long),
+                               hipMemcpyDeviceToHost, s->stream));
+        HIP_TRY(hipStreamSynchronize(s->stream));
+        float ms = 0.f;
+        HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
+        s->st.kernel_ns += (uint64_t)((double)ms * 1e6);
+        s->st.candidates += s->h_ctl[0];
+        s->st.suffixes += s->h_ctl[1];
+        for (const auto &c : counted)
+            if (!c.second->h->opts.fixed_test_order &&
+                (rc = adapt_test_order(c.second->h, s->h_ctl + s->tstat_word(c.first))) < 0)
+                return rc;
         // ---- collect each task's hits
-        for (Slot *sp : in_launch) {
-            pcagpu *h = sp->h;
-            uint32_t hits = 0;
-            HIP_TRY(hipMemcpy(&hits, h->d_hit_count.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < in_launch.size(); i++) {
+            pcagpu *h = in_launch[i]->h;
+            const uint32_t hits = s->hit_count_host(i);
             if (hits > h->hit_cap) {
                 // overflow: rescan this batch alone with a larger buffer
                 rc = pcagpu_flush(h);

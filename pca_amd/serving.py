@@ -21,9 +21,11 @@ QUEUED, ACTIVE, DONE, FAILED = 0, 1, 2, 3
 
 
 class Server:
-    def __init__(self, device=0, max_slots=64, hbm_budget=200 << 30, prefixes_per_item=1 << 14):
+    def __init__(self, device=0, max_slots=64, hbm_budget=200 << 30, prefixes_per_item=1 << 14, warm_slots=4,
+                 step_prefixes=1 << 18):
         o = ServerOpts(device=device, max_slots=max_slots, hbm_budget=hbm_budget,
-                       prefixes_per_item=prefixes_per_item)
+                       prefixes_per_item=prefixes_per_item,
+                       warm_slots=0xFFFFFFFF if warm_slots == 0 else warm_slots, step_prefixes=step_prefixes)
         h = C.c_void_p()
         rc = lib().pcagpu_server_open(C.byref(o), C.byref(h))
         if rc < 0:

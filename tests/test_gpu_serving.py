@@ -22,12 +22,23 @@ def _solo(pca, task, budget):
     return r.optimal_depth, r.num_solutions, [x[0] for x in r.solutions]
 
 
-def test_server_mix_equals_solo(pca):
-    from pca_amd.serving import Server, DONE
+@pytest.fixture(scope="module")
+def solo(pca):
     tasks = _tasks(pca)
-    print(f"[serving] {len(tasks)} tasks", flush=True)
-    want = [_solo(pca, t, d) for t, d in tasks]
-    with Server(max_slots=8) as sv:
+    return tasks, [_solo(pca, t, d) for t, d in tasks]
+
+
+@pytest.mark.parametrize("kw", [{}, {"warm_slots": 0, "prefixes_per_item": 2048, "step_prefixes": 4096},
+                                {"warm_slots": 8, "step_prefixes": 1}],
+                         ids=["default", "small-shares", "warm-no-step-share"])
+def test_server_mix_equals_solo(pca, solo, kw):
+    """Every request's answer equals pcagpu_solve's on the task alone, with
+    the default shares, with small shares (many steps per depth, the test
+    order adapting across them) and with the step share switched off."""
+    from pca_amd.serving import Server, DONE
+    tasks, want = solo
+    print(f"[serving] {len(tasks)} tasks {kw}", flush=True)
+    with Server(max_slots=8, **kw) as sv:
         ids = [sv.submit(t, d) for t, d in tasks[:10]]
         for _ in range(3):
             sv.step()                      # the rest arrive while these run
