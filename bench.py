@@ -100,7 +100,7 @@ def value_bench(batch=512, steps=30, seed=5):
     throughput in search states per second on synthetic states (random
     register files of is_power2_32's tests), and the layer-1 forward GEMM
     (M = 32 tests x batch, N = 512, K = 180) against the dense BF16 MFMA
-    peak."""
+    peak and against HBM, which bounds it at this K."""
     import numpy as np
     from pca_amd import taskgen
     from pca_amd._lib import lib
@@ -138,11 +138,21 @@ def value_bench(batch=512, steps=30, seed=5):
     h1 = net._buf("h1", (big * MAX_T, H1), np.uint16)
     t_gemm = timed(lambda: net._gemm(big * MAX_T, H1, PER_TEST, Xb, PER_TEST, 0, net.v16["W1"], H1, 0,
                                      bias=net.views["b1"], relu=True, C16=h1, ldc16=H1), 50)
-    flops = 2.0 * big * MAX_T * H1 * PER_TEST
+    M = big * MAX_T
+    flops = 2.0 * M * H1 * PER_TEST
+    # algorithmic HBM bytes: A (M x K BF16) and W1 read once, C (M x N BF16)
+    # written once; at K = 180 that is ~133 flop per byte, under the 312
+    # flop/byte balance of 2.5 PFLOP/s over 8 TB/s, so HBM bounds it
+    nbytes = 2.0 * (M * PER_TEST + PER_TEST * H1 + M * H1)
+    bound_tflops = min(2500.0, flops / nbytes * 8.0)
     return {"batch_states": batch, "train_states_per_s": batch / t_train, "infer_states_per_s": batch / t_infer,
-            "gemm_layer1": {"M": big * MAX_T, "N": H1, "K": PER_TEST, "us": t_gemm * 1e6,
+            "gemm_layer1": {"M": M, "N": H1, "K": PER_TEST, "us": t_gemm * 1e6,
                             "tflops": flops / t_gemm / 1e12, "peak_tflops": 2500.0,
-                            "frac": flops / t_gemm / 2.5e15},
+                            "frac": flops / t_gemm / 2.5e15,
+                            "roofline": {"bound": "hbm", "flop_per_byte": flops / nbytes,
+                                         "algorithmic_bytes": nbytes, "achieved_gbs": nbytes / t_gemm / 1e9,
+                                         "peak_gbs": 8000.0, "bound_tflops": bound_tflops,
+                                         "frac": nbytes / t_gemm / 8e12}},
             "params": net.n_params, "timing": "host clock around device-synchronised repetitions"}
 
 
