@@ -152,6 +152,7 @@ struct ExpandArgs {
     int D, pos;                      // children are placed at kernel position pos
     CandDev cand;
     unsigned long long *table;       // [2 * cap]: key, min child index
+    uint64_t *child_hash;            // [children] each non-BR child's key, written by insert
     uint64_t table_mask;             // cap - 1
     uint32_t *keep_bits;             // one bit per child
     uint64_t *keep_count;            // [n_parents] survivors per parent

@@ -991,6 +991,7 @@ __global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, Ex
             }
             h |= 1ull;
             const unsigned long long idx = base + (unsigned long long)j;
+            a.child_hash[idx] = h;   // survive reads it back instead of recomputing
             uint64_t slot = (h >> 1) & a.table_mask;
             for (uint64_t probe = 0;; probe++) {
                 if (probe > a.table_mask) { atomicAdd(&a.counters[1], 1ull); break; }
@@ -1007,7 +1008,9 @@ __global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, Ex
     }
 }
 
-// Pass 2: a child survives if it is a BR or the first visitor of its key.
+// Pass 2: a child survives if it is a BR or the first visitor of its key
+// (the key pass 1 stored per child, read back coalesced instead of stepping
+// the child again on every test).
 // A later visitor is confirmed against the winner's full state vector before
 // it is dropped; a mismatch is a 64-bit collision and is counted (the host
 // refuses the level).  Survivor bits are packed with a wave ballot.
@@ -1037,14 +1040,8 @@ __global__ __launch_bounds__(SCAN_WG) void expand_survive_kernel(DevTask task, E
                 if (insn_op(c) == OP_BR) {
                     keep = true;
                 } else {
-                    uint64_t h = 0x243F6A8885A308D3ull;
-                    for (int t = 0; t < task.T; t++) {
-                        TestState cs;
-                        child_row(s_st[w][t], c, (uint32_t)a.pos, cs);
-                        h = row_words_hash(h, cs);
-                    }
-                    h |= 1ull;
                     const unsigned long long idx = base + (unsigned long long)j;
+                    const uint64_t h = a.child_hash[idx];   // insert's key of this child
                     uint64_t slot = (h >> 1) & a.table_mask;
                     unsigned long long win = ~0ull;
                     for (uint64_t probe = 0; probe <= a.table_mask; probe++) {

@@ -122,6 +122,7 @@ struct pcagpu {
     DevBuf<uint64_t> d_child_off, d_keep_cnt, d_scan_tmp;
     DevBuf<unsigned long long> d_table;
     DevBuf<uint32_t> d_keep_bits;
+    DevBuf<uint64_t> d_child_hash;
 
     // depth state
     bool in_depth = false;
