@@ -153,6 +153,7 @@ struct ExpandArgs {
     CandDev cand;
     unsigned long long *table;       // [2 * cap]: key, min child index
     uint64_t *child_hash;            // [children] each non-BR child's key, written by insert
+    TestState *parent_rows;          // [n_parents * MAXT] each parent's test rows, written by insert
     uint64_t table_mask;             // cap - 1
     uint32_t *keep_bits;             // one bit per child
     uint64_t *keep_count;            // [n_parents] survivors per parent

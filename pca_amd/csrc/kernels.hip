@@ -974,6 +974,8 @@ __global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, Ex
         const uint32_t live = uni(P->live);
         replay_prefix(task, P, a.pos, s_st[w], lane);
         wave_sync();
+        // the parent's rows, for pass 2's confirmations (one slice writes them)
+        if (slice == 0 && lane < task.T) a.parent_rows[u * MAXT + lane] = s_st[w][lane];
         const int n = cand_count(a.cand, live, mf);
         int jb, je;
         units.range(n, slice, &jb, &je);
@@ -1061,14 +1063,14 @@ __global__ __launch_bounds__(SCAN_WG) void expand_survive_kernel(DevTask task, E
                     if (win == idx) {
                         keep = true;
                     } else if (win != ~0ull) {
-                        // confirm: rebuild the winner's state test by test
+                        // confirm: the winner's state test by test, from
+                        // its parent's rows (stored by pass 1)
                         const uint64_t wp = upper_parent(a.child_off, a.n_parents, win);
-                        const pca_prefix_t *WP = &a.parents[wp];
-                        const uint32_t wc = cand_at(a.cand, WP->live, mf, (int)(win - a.child_off[wp]));
+                        const uint32_t wc = cand_at(a.cand, a.parents[wp].live, mf, (int)(win - a.child_off[wp]));
+                        const TestState *WR = &a.parent_rows[wp * MAXT];
                         bool same = true;
                         for (int t = 0; t < task.T && same; t++) {
-                            TestState ws = task.root[t];
-                            for (int q = 0; q < a.pos; q++) step_test(ws, WP->insns[q], (uint32_t)q);
+                            TestState ws = WR[t];
                             step_test(ws, wc, (uint32_t)a.pos);
                             TestState cs;
                             child_row(s_st[w][t], c, (uint32_t)a.pos, cs);

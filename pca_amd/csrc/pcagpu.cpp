@@ -396,7 +396,7 @@ size_t handle_device_bytes(const pcagpu *h)
     return h->d_batch.bytes() + h->d_blocks.bytes() + h->d_groups.bytes() + h->d_goff.bytes() + h->d_task.bytes() + h->d_items.bytes() +
            h->d_hits.bytes() + h->d_hit_count.bytes() + h->d_counters.bytes() + h->d_work.bytes() + h->d_tstat.bytes() +
            h->d_front[0].bytes() + h->d_front[1].bytes() + h->d_child_off.bytes() + h->d_keep_cnt.bytes() +
-           h->d_scan_tmp.bytes() + h->d_table.bytes() + h->d_keep_bits.bytes() + h->d_child_hash.bytes() + h->d_q0.bytes() + h->d_qf.bytes() +
+           h->d_scan_tmp.bytes() + h->d_table.bytes() + h->d_keep_bits.bytes() + h->d_child_hash.bytes() + h->d_parent_rows.bytes() + h->d_q0.bytes() + h->d_qf.bytes() +
            h->d_match.bytes() + h->d_match_count.bytes();
 }
 
@@ -583,6 +583,7 @@ void pcagpu_close(pcagpu_t *h)
     h->d_table.release();
     h->d_keep_bits.release();
     h->d_child_hash.release();
+    h->d_parent_rows.release();
     h->d_q0.release();
     h->d_qf.release();
     h->d_match.release();
@@ -1151,7 +1152,7 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     uint64_t tcap = 1024;
     while (tcap < 2 * children) tcap <<= 1;
     if ((rc = h->d_table.ensure(2 * tcap)) || (rc = h->d_keep_bits.ensure(children / 32 + 8)) ||
-        (rc = h->d_child_hash.ensure(children ? children : 1)))
+        (rc = h->d_child_hash.ensure(children ? children : 1)) || (rc = h->d_parent_rows.ensure(n * MAXT)))
         return rc;
     // keys 0 (empty), min slots all-ones so atomicMin keeps the smallest index
     launch_table_init(h->d_table.p, tcap, h->stream);
@@ -1161,6 +1162,7 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     a.table = h->d_table.p;
     a.table_mask = tcap - 1;
     a.child_hash = h->d_child_hash.p;
+    a.parent_rows = h->d_parent_rows.p;
     a.keep_bits = h->d_keep_bits.p;
     a.keep_count = h->d_keep_cnt.p;
     a.counters = h->d_counters.p + 4;

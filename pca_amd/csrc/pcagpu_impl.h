@@ -123,6 +123,7 @@ struct pcagpu {
     DevBuf<unsigned long long> d_table;
     DevBuf<uint32_t> d_keep_bits;
     DevBuf<uint64_t> d_child_hash;
+    DevBuf<pca::TestState> d_parent_rows;
 
     // depth state
     bool in_depth = false;
