@@ -161,8 +161,9 @@ def wavefront_bench(task, D=8, s=3):
     kernels) of the bench task down to split s of depth D: device time from
     the handle's HIP events, children and survivors per level summed, and
     the algorithmic HBM bytes of the table and frontier (per child a 16-byte
-    slot probed in each of two passes plus an 8-byte atomicMin, per
-    survivor its 18-byte record) against the 8 TB/s peak."""
+    slot probed in each of two passes, an 8-byte atomicMin and its 8-byte
+    key written and read back; per survivor its 18-byte record) against the
+    8 TB/s peak."""
     from pca_amd import Solver
     from pca_amd._lib import lib
     with Solver(task) as sv:
@@ -172,7 +173,7 @@ def wavefront_bench(task, D=8, s=3):
         st = sv.stats()
     secs = st["expand_ns"] * 1e-9
     children, kept = st["frontier_nodes"], st["frontier_kept"]
-    nbytes = children * 40 + kept * 18
+    nbytes = children * 56 + kept * 18
     return {"workload": f"is_power2_32 wavefront, depth {D}, levels 0..{s}", "prefixes": n, "children": children,
             "kept": kept, "expand_ms": secs * 1e3, "children_per_s": children / secs,
             "hbm": {"algorithmic_bytes": nbytes, "achieved_gbs": nbytes / secs / 1e9, "peak_gbs": 8000.0,
