@@ -326,6 +326,10 @@ typedef struct {
     uint64_t launches, items;       /* scan launches, task batches in them        */
     uint64_t kernel_ns, candidates, suffixes;
     uint64_t slots, slots_opened, device_bytes;
+    /* host time of the steps, by phase: admission (re-tasking, the first
+     * depth), staging the batches, waiting for the launches, and finished
+     * depths (resolution, the next depth's prefixes) */
+    uint64_t admit_ns, stage_ns, wait_ns, resolve_ns;
 } pcagpu_server_stats_t;
 
 int    pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_server_t **out);

@@ -173,7 +173,7 @@ class Request(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "submitted", "admitted", "completed", "failed", "launches", "items", "kernel_ns", "candidates",
-        "suffixes", "slots", "slots_opened", "device_bytes")]
+        "suffixes", "slots", "slots_opened", "device_bytes", "admit_ns", "stage_ns", "wait_ns", "resolve_ns")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

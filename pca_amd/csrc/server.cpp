@@ -25,10 +25,12 @@
 // what pcagpu_solve gives for it alone: the same prefixes, kernel and
 // resolution, only scheduled together.
 #include <algorithm>
+#include <atomic>
 #include <cstring>
 #include <deque>
 #include <map>
 #include <new>
+#include <thread>
 #include <vector>
 
 #include "host_dfs.h"
@@ -180,28 +182,81 @@ void fail_request(pcagpu_server *s, Slot &sl)
     sl.frontier.clear();
 }
 
-// Depth finished: resolve; either complete the request or go one deeper.
-int advance(pcagpu_server *s, Slot &sl)
+// What resolving a slot's finished depth led to.
+struct Outcome {
+    int rc = 0;          // < 0: the request fails
+    bool done = false;   // optimum proven, or none up to the budget
+    int optimal = -1;
+    long nsol = 0;
+    std::vector<pca_solution_t> sols;
+};
+
+// Depth finished: resolve it and, while no program is found and budget is
+// left, start the next depth.  Touches only the slot (its handle, stream and
+// frontier), so finished slots are resolved on several host threads.
+Outcome resolve_depths(Slot &sl)
 {
+    Outcome o;
     std::vector<pca_solution_t> buf(4096);
     for (;;) {
         const long n = pcagpu_end_depth(sl.h, buf.data(), (long)buf.size());
-        if (n < 0) return (int)n;
+        if (n < 0) {
+            o.rc = (int)n;
+            return o;
+        }
         if (n > 0) {
             buf.resize((size_t)std::min<long>(n, (long)buf.size()));
-            finish(s, sl, sl.D, n, buf);
-            return 0;
+            o.done = true;
+            o.optimal = sl.D;
+            o.nsol = n;
+            o.sols = std::move(buf);
+            return o;
         }
         if (sl.D >= sl.max_depth) {
-            finish(s, sl, -1, 0, {});
-            return 0;
+            o.done = true;
+            return o;
         }
         sl.D++;
-        int rc = start_depth(sl);
-        if (rc < 0) return rc;
-        if (!sl.frontier.empty()) return 0;
+        const int rc = start_depth(sl);
+        if (rc < 0) {
+            o.rc = rc;
+            return o;
+        }
+        if (!sl.frontier.empty()) return o;
         // an empty frontier closes the depth at once
     }
+}
+
+// The server-side effect of an outcome (single-threaded).
+int apply(pcagpu_server *s, Slot &sl, const Outcome &o)
+{
+    if (o.rc < 0) return o.rc;
+    if (o.done) finish(s, sl, o.optimal, o.nsol, o.sols);
+    return 0;
+}
+
+int advance(pcagpu_server *s, Slot &sl) { return apply(s, sl, resolve_depths(sl)); }
+
+// f(0..n-1) on up to 8 host threads (the calling one included), each with
+// the server's device current.
+template <class F>
+void run_parallel(const pcagpu_server *s, size_t n, F &&f)
+{
+    const size_t nt = std::min<size_t>({n, (size_t)std::max(1u, std::thread::hardware_concurrency()), 8});
+    if (nt <= 1) {
+        for (size_t i = 0; i < n; i++) f(i);
+        return;
+    }
+    std::atomic<size_t> next{0};
+    auto work = [&] {
+        (void)hipSetDevice(s->opts.device);
+        for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
+    };
+    std::vector<std::thread> pool;
+    pool.reserve(nt - 1);
+    for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
+    work();
+    for (std::thread &th : pool) th.join();
 }
 
 // A new resident slot on s's device, holding `task` (the caller re-tasks
@@ -366,7 +421,14 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
 {
     if (!s) return PCAGPU_EINVAL;
     (void)hipSetDevice(s->opts.device);
+    double tp = now_s();
+    auto lap = [&](uint64_t &acc) {
+        const double t = now_s();
+        acc += (uint64_t)((t - tp) * 1e9);
+        tp = t;
+    };
     int rc = admit(s);
+    lap(s->st.admit_ns);
     if (rc < 0) return rc;
     // ---- every active task stages its next batch: its share of the step
     // (at least prefixes_per_item, more when few tasks are active, so a task
@@ -445,6 +507,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             }
         }
         if ((rc = s->d_items.ensure(std::max<size_t>(shared.size(), 1)))) return rc;
+        lap(s->st.stage_ns);
         const size_t ctl_bytes = s->ctl_words() * sizeof(unsigned long long);
         HIP_TRY(hipMemsetAsync(s->d_ctl.p, 0, ctl_bytes, s->stream));
         HIP_TRY(hipEventRecord(s->ev0, s->stream));
@@ -492,6 +555,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         HIP_TRY(hipMemcpyAsync(s->h_ctl, s->d_ctl.p, s->tstat_word(n_launch) * sizeof(unsigned long long),
                                hipMemcpyDeviceToHost, s->stream));
         HIP_TRY(hipStreamSynchronize(s->stream));
+        lap(s->st.wait_ns);
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, s->ev0, s->ev1));
         s->st.kernel_ns += (uint64_t)((double)ms * 1e6);
@@ -527,11 +591,17 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             h->batch_n = 0;
         }
     }
-    // ---- tasks whose depth is complete
-    for (Slot &sl : s->slots) {
-        if (!sl.busy || sl.next < sl.frontier.size()) continue;
-        if ((rc = advance(s, sl)) < 0) fail_request(s, sl);
-    }
+    lap(s->st.stage_ns);   // hit collection
+    // ---- tasks whose depth is complete: resolved in parallel, applied in
+    // slot order
+    std::vector<Slot *> fin;
+    for (Slot &sl : s->slots)
+        if (sl.busy && sl.next >= sl.frontier.size()) fin.push_back(&sl);
+    std::vector<Outcome> res(fin.size());
+    run_parallel(s, fin.size(), [&](size_t i) { res[i] = resolve_depths(*fin[i]); });
+    for (size_t i = 0; i < fin.size(); i++)
+        if (apply(s, *fin[i], res[i]) < 0) fail_request(s, *fin[i]);
+    lap(s->st.resolve_ns);
     int pending = (int)s->queue.size();
     for (const Slot &sl : s->slots) pending += sl.busy;
     return pending;
