@@ -40,6 +40,32 @@ using namespace pca;
 
 namespace {
 
+// Growable pinned host buffer (contents are not kept when it grows).
+struct PinnedBuf {
+    unsigned char *p = nullptr;
+    size_t n = 0;
+    int ensure(size_t want)
+    {
+        if (want <= n) return 0;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+        const size_t cap = std::max(want, (size_t)1 << 20);
+        if (hipHostMalloc((void **)&p, cap) != hipSuccess) {
+            p = nullptr;
+            return PCAGPU_ENOMEM;
+        }
+        n = cap;
+        return 0;
+    }
+    void release()
+    {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
+
 struct Slot {
     pcagpu *h = nullptr;
     bool busy = false;
@@ -81,6 +107,10 @@ struct pcagpu_server {
     // item, then per launch the per-test counts of a one-item launch
     DevBuf<unsigned long long> d_ctl;
     unsigned long long *h_ctl = nullptr;   // pinned mirror
+    // one step's prefixes and scan blocks of every item, copied in one go
+    // (pinned host side), and the items' hit records, read back in one go
+    PinnedBuf h_stage, h_hits;
+    DevBuf<unsigned char> d_stage;
     std::vector<ScanItem> items;
     pcagpu_server_stats_t st;
 
@@ -393,6 +423,9 @@ extern "C" void pcagpu_server_close(pcagpu_server_t *s)
     s->d_items.release();
     s->d_ctl.release();
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
+    s->h_stage.release();
+    s->h_hits.release();
+    s->d_stage.release();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
     if (s->stream) (void)hipStreamDestroy(s->stream);
@@ -440,6 +473,9 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     uint32_t active = 0;
     for (const Slot &sl : s->slots) active += sl.busy && sl.next < sl.frontier.size();
     const uint32_t share = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes / std::max(active, 1u));
+    auto align16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    std::vector<uint32_t> staged_blocks;
+    size_t stage_bytes = 0;
     for (Slot &sl : s->slots) {
         if (!sl.busy || sl.next >= sl.frontier.size()) continue;
         pcagpu *h = sl.h;
@@ -451,17 +487,12 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         uint32_t nb = 0;
         uint64_t nu = 0;
         if ((rc = build_blocks(h, tile, &nb, &nu)) < 0) return rc;
-        HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
-                               s->stream));
-        if (nb)
-            HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)nb * sizeof(ScanBlock),
-                                   hipMemcpyHostToDevice, s->stream));
+        staged_blocks.push_back(nb);
+        stage_bytes += align16((size_t)n * sizeof(pca_prefix_t)) + align16((size_t)nb * sizeof(ScanBlock));
         levels = std::max(levels, scan_levels(h->k));
         ScanItem it;
         std::memset(&it, 0, sizeof(it));
         it.task = h->d_task.p;
-        it.prefixes = h->d_batch.p;
-        it.blocks = h->d_blocks.p;
         it.n_prefix = n;
         it.unit_begin = units;
         it.n_units = nu;
@@ -477,6 +508,23 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         sl.staged = n;
         sl.next += n;
         in_launch.push_back(&sl);
+    }
+    // ---- every item's prefixes and blocks into one pinned buffer, one copy
+    if (!s->items.empty()) {
+        if ((rc = s->h_stage.ensure(stage_bytes)) || (rc = s->d_stage.ensure(s->h_stage.n))) return rc;
+        size_t off = 0;
+        for (size_t i = 0; i < s->items.size(); i++) {
+            const pcagpu *h = in_launch[i]->h;
+            const size_t pb = (size_t)s->items[i].n_prefix * sizeof(pca_prefix_t);
+            const size_t bb = (size_t)staged_blocks[i] * sizeof(ScanBlock);
+            std::memcpy(s->h_stage.p + off, h->h_batch, pb);
+            s->items[i].prefixes = reinterpret_cast<const pca_prefix_t *>(s->d_stage.p + off);
+            off += align16(pb);
+            if (bb) std::memcpy(s->h_stage.p + off, h->h_blocks, bb);
+            s->items[i].blocks = reinterpret_cast<const ScanBlock *>(s->d_stage.p + off);
+            off += align16(bb);
+        }
+        HIP_TRY(hipMemcpyAsync(s->d_stage.p, s->h_stage.p, off, hipMemcpyHostToDevice, s->stream));
     }
     // ---- the launches: an item whose units alone fill the chip several
     // times over runs in a launch of its own (the one-item kernel, its task
@@ -565,20 +613,40 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             if (!c.second->h->opts.fixed_test_order &&
                 (rc = adapt_test_order(c.second->h, s->h_ctl + s->tstat_word(c.first))) < 0)
                 return rc;
-        // ---- collect each task's hits
+        // ---- collect each task's hits: every item's records copied into one
+        // pinned buffer, one wait; an overflowing item rescans its batch
+        // alone with a larger buffer
+        size_t hit_total = 0;
+        for (size_t i = 0; i < in_launch.size(); i++) {
+            const uint32_t hits = s->hit_count_host(i);
+            if (hits <= in_launch[i]->h->hit_cap) hit_total += hits;
+        }
+        if (hit_total) {
+            if ((rc = s->h_hits.ensure(hit_total * sizeof(HitRec)))) return rc;
+            size_t off = 0;
+            for (size_t i = 0; i < in_launch.size(); i++) {
+                const uint32_t hits = s->hit_count_host(i);
+                if (!hits || hits > in_launch[i]->h->hit_cap) continue;
+                HIP_TRY(hipMemcpyAsync(s->h_hits.p + off * sizeof(HitRec), in_launch[i]->h->d_hits.p,
+                                       (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost, s->stream));
+                off += hits;
+            }
+            HIP_TRY(hipStreamSynchronize(s->stream));
+        }
+        size_t hit_off = 0;
         for (size_t i = 0; i < in_launch.size(); i++) {
             pcagpu *h = in_launch[i]->h;
             const uint32_t hits = s->hit_count_host(i);
             if (hits > h->hit_cap) {
-                // overflow: rescan this batch alone with a larger buffer
                 rc = pcagpu_flush(h);
                 if (rc < 0) return rc;
                 continue;
             }
             if (hits) {
-                std::vector<HitRec> buf(hits);
-                HIP_TRY(hipMemcpy(buf.data(), h->d_hits.p, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost));
-                for (const HitRec &r : buf) {
+                const HitRec *buf = reinterpret_cast<const HitRec *>(s->h_hits.p) + hit_off;
+                hit_off += hits;
+                for (uint32_t q = 0; q < hits; q++) {
+                    const HitRec &r = buf[q];
                     HostHit x;
                     x.prefix = h->batch_first + r.prefix;
                     x.mask = r.mask;
