@@ -267,12 +267,13 @@ int apply(pcagpu_server *s, Slot &sl, const Outcome &o)
 
 int advance(pcagpu_server *s, Slot &sl) { return apply(s, sl, resolve_depths(sl)); }
 
-// f(0..n-1) on up to 8 host threads (the calling one included), each with
-// the server's device current.
+// f(0..n-1) on up to 8 host threads (the calling one included; only it when
+// `parallel` is false), each with the server's device current.
 template <class F>
-void run_parallel(const pcagpu_server *s, size_t n, F &&f)
+void run_parallel(const pcagpu_server *s, size_t n, F &&f, bool parallel = true)
 {
-    const size_t nt = std::min<size_t>({n, (size_t)std::max(1u, std::thread::hardware_concurrency()), 8});
+    const size_t nt =
+        parallel ? std::min<size_t>({n, (size_t)std::max(1u, std::thread::hardware_concurrency()), 8}) : 1;
     if (nt <= 1) {
         for (size_t i = 0; i < n; i++) f(i);
         return;
@@ -474,39 +475,62 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     for (const Slot &sl : s->slots) active += sl.busy && sl.next < sl.frontier.size();
     const uint32_t share = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes / std::max(active, 1u));
     auto align16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
+    // staging is host work per slot (tile, prefix fitting, the batch and its
+    // scan blocks): slots stage in parallel, items are assembled in slot order
+    struct Staged {
+        int rc = 0;
+        uint32_t n = 0, nb = 0;
+        uint64_t nu = 0;
+        ScanTile tile{};
+    };
+    std::vector<Slot *> act;
+    size_t left = 0;
+    for (Slot &sl : s->slots)
+        if (sl.busy && sl.next < sl.frontier.size()) {
+            act.push_back(&sl);
+            left += sl.frontier.size() - sl.next;
+        }
+    std::vector<Staged> stg(act.size());
+    // (threads only pay off with work to share: many slots or big batches)
+    run_parallel(s, act.size(), [&](size_t i) {
+        Slot &sl = *act[i];
+        pcagpu *h = sl.h;
+        Staged &g = stg[i];
+        int cand;
+        g.tile = current_tile(h, &cand);   // no tuning inside shared launches
+        uint32_t n = (uint32_t)std::min<size_t>({sl.frontier.size() - sl.next, (size_t)h->batch_cap, (size_t)share});
+        n = std::max<uint32_t>(prefixes_fitting(h, sl.frontier.data() + sl.next, n, g.tile), 1);
+        if ((g.rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return;
+        if ((g.rc = build_blocks(h, g.tile, &g.nb, &g.nu)) < 0) return;
+        g.n = n;
+    }, act.size() >= 8 || left >= (1u << 16));
     std::vector<uint32_t> staged_blocks;
     size_t stage_bytes = 0;
-    for (Slot &sl : s->slots) {
-        if (!sl.busy || sl.next >= sl.frontier.size()) continue;
+    for (size_t i = 0; i < act.size(); i++) {
+        Slot &sl = *act[i];
         pcagpu *h = sl.h;
-        int cand;
-        const ScanTile tile = current_tile(h, &cand);   // no tuning inside shared launches
-        uint32_t n = (uint32_t)std::min<size_t>({sl.frontier.size() - sl.next, (size_t)h->batch_cap, (size_t)share});
-        n = std::max<uint32_t>(prefixes_fitting(h, sl.frontier.data() + sl.next, n, tile), 1);
-        if ((rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return rc;
-        uint32_t nb = 0;
-        uint64_t nu = 0;
-        if ((rc = build_blocks(h, tile, &nb, &nu)) < 0) return rc;
-        staged_blocks.push_back(nb);
-        stage_bytes += align16((size_t)n * sizeof(pca_prefix_t)) + align16((size_t)nb * sizeof(ScanBlock));
+        const Staged &g = stg[i];
+        if (g.rc < 0) return g.rc;
+        staged_blocks.push_back(g.nb);
+        stage_bytes += align16((size_t)g.n * sizeof(pca_prefix_t)) + align16((size_t)g.nb * sizeof(ScanBlock));
         levels = std::max(levels, scan_levels(h->k));
         ScanItem it;
         std::memset(&it, 0, sizeof(it));
         it.task = h->d_task.p;
-        it.n_prefix = n;
+        it.n_prefix = g.n;
         it.unit_begin = units;
-        it.n_units = nu;
+        it.n_units = g.nu;
         it.D = h->D;
         it.s = h->s;
         it.k = h->k;
-        it.tile = tile;
+        it.tile = g.tile;
         it.hits = h->d_hits.p;
         it.hit_count = s->hit_count_dev(in_launch.size());
         it.hit_cap = h->hit_cap;
         s->items.push_back(it);
-        units += nu;
-        sl.staged = n;
-        sl.next += n;
+        units += g.nu;
+        sl.staged = g.n;
+        sl.next += g.n;
         in_launch.push_back(&sl);
     }
     // ---- every item's prefixes and blocks into one pinned buffer, one copy
