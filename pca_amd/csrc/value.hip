@@ -485,14 +485,64 @@ __global__ void pool_backward_kernel(int T, int C, int E, const float *dZ, const
 }
 
 // 64 columns x 256 row-chunks per workgroup, partial sums added atomically
-__global__ void colsum_kernel(int M, int N, const float *G, int ldg, float *out)
+// Column sums (bias gradients), two passes without atomics.  Pass 1: a
+// block sums CS_ROWS rows of up to 256 columns, each lane four adjacent
+// columns (one 16-byte load per row when aligned), its four waves on
+// interleaved rows; one partial row per block goes to the workspace.
+constexpr int CS_ROWS = 64;
+
+__global__ __launch_bounds__(256) void colsum_partial_kernel(int M, int N, const float *__restrict__ G, int ldg,
+                                                             int vec, float *__restrict__ P)
 {
-    const int c = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int r0 = blockIdx.y * 1024 + (threadIdx.x >> 6);
-    if (c >= N) return;
+    __shared__ float red[4][4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = ((int)blockIdx.x * 64 + lane) * 4;
+    const int r0 = (int)blockIdx.y * CS_ROWS, r1 = min(M, r0 + CS_ROWS);
+    float s0 = 0.f, s1 = 0.f, s2 = 0.f, s3 = 0.f;
+    if (c < N) {
+        const bool full = vec && c + 3 < N;
+#pragma unroll 4
+        for (int r = r0 + w; r < r1; r += 4) {
+            const float *row = G + (int64_t)r * ldg + c;
+            if (full) {
+                const float4 v = *reinterpret_cast<const float4 *>(row);
+                s0 += v.x;
+                s1 += v.y;
+                s2 += v.z;
+                s3 += v.w;
+            } else {
+                s0 += row[0];
+                if (c + 1 < N) s1 += row[1];
+                if (c + 2 < N) s2 += row[2];
+                if (c + 3 < N) s3 += row[3];
+            }
+        }
+    }
+    red[w][0][lane] = s0;
+    red[w][1][lane] = s1;
+    red[w][2][lane] = s2;
+    red[w][3][lane] = s3;
+    __syncthreads();
+    if (w == 0 && c < N) {
+        float *p = P + (int64_t)blockIdx.y * N + c;
+        for (int j = 0; j < 4 && c + j < N; j++) p[j] = red[0][j][lane] + red[1][j][lane] + red[2][j][lane] + red[3][j][lane];
+    }
+}
+
+// Pass 2: out[c] = sum of the partial rows (four waves on interleaved rows).
+__global__ __launch_bounds__(256) void colsum_reduce_kernel(int N, int parts, const float *__restrict__ P, float *out)
+{
+    __shared__ float red[4][64];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = (int)blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int r = r0; r < M && r < (int)(blockIdx.y + 1) * 1024; r += 4) s += G[(int64_t)r * ldg + c];
-    atomicAdd(&out[c], s);
+    if (c < N) {
+#pragma unroll 8
+        for (int q = w; q < parts; q += 4) s += P[(int64_t)q * N + c];
+    }
+    red[w][lane] = s;
+    __syncthreads();
+    if (w == 0 && c < N) out[c] = red[0][lane] + red[1][lane] + red[2][lane] + red[3][lane];
 }
 
 __global__ void fill_f32_kernel(int n, float *p)
@@ -666,14 +716,22 @@ int pcav_pool_backward(int B, int T, int C, int E, const float *dZ, const uint8_
 
 int pcav_colsum(int M, int N, const float *G, int ldg, float *out, void *stream)
 {
-    if (M < 0 || N <= 0 || !G || !out) return PCAGPU_EINVAL;
+    if (M < 0 || N <= 0 || !G || !out || ldg < N) return PCAGPU_EINVAL;
     clear_error();
-    hipLaunchKernelGGL(fill_f32_kernel, dim3((N + 255) / 256), dim3(256), 0, st(stream), N, out);
-    if (M == 0) return status();
-    const dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 1023) / 1024));
-    if (grid.y > 65535u) return PCAGPU_EINVAL;
+    if (M == 0) {
+        hipLaunchKernelGGL(fill_f32_kernel, dim3((N + 255) / 256), dim3(256), 0, st(stream), N, out);
+        return status();
+    }
+    const int parts = (M + CS_ROWS - 1) / CS_ROWS;
+    if (parts > 65535) return PCAGPU_EINVAL;
+    float *ws = split_workspace((size_t)parts * N);
+    if (!ws) return PCAGPU_ENOMEM;
+    const int vec = ((uintptr_t)G % 16 == 0 && ldg % 4 == 0) ? 1 : 0;
     clear_error();
-    hipLaunchKernelGGL(colsum_kernel, grid, dim3(256), 0, st(stream), M, N, G, ldg, out);
+    hipLaunchKernelGGL(colsum_partial_kernel, dim3((unsigned)((N + 255) / 256), (unsigned)parts), dim3(256), 0,
+                       st(stream), M, N, G, ldg, vec, ws);
+    hipLaunchKernelGGL(colsum_reduce_kernel, dim3((unsigned)((N + 63) / 64)), dim3(256), 0, st(stream), N, parts, ws,
+                       out);
     return status();
 }
 

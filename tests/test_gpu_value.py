@@ -76,6 +76,28 @@ def test_gemm_split_k_matches_numpy(pca, M, N, K, acc):
     assert np.allclose(C32.numpy(), ref, rtol=1e-4, atol=1e-3 * np.sqrt(K))
 
 
+@pytest.mark.parametrize("M,N,ldg,off", [(16384, 512, 512, 0), (16384, 256, 256, 0), (512, 1, 1, 0),
+                                         (1000, 37, 40, 1), (70, 300, 301, 0), (65, 3, 3, 0), (0, 5, 5, 0)])
+def test_colsum_matches_numpy(pca, M, N, ldg, off):
+    """Bias-gradient column sums: the 16-byte path (aligned rows), the scalar
+    path (a pointer off by one float, a stride not a multiple of 4), ragged
+    column tails and M = 0 (zeros); the output is overwritten."""
+    import ctypes as C
+    from pca_amd._lib import lib
+    from pca_amd.value import DeviceArray
+    rng = np.random.default_rng(M + N + ldg + off)
+    g = rng.normal(0, 1, (max(M, 1) * ldg + off,)).astype(np.float32)
+    G = DeviceArray(g.shape, np.float32)
+    G.upload(g)
+    out = DeviceArray(N, np.float32)
+    out.upload(np.full(N, 7.0, dtype=np.float32))
+    print(f"[colsum] M={M} N={N} ldg={ldg} off={off}", flush=True)
+    assert lib().pcav_colsum(M, N, C.c_void_p(G.ptr + 4 * off), ldg, out.ptr, None) == 0
+    assert lib().pcav_synchronize() == 0
+    rows = g[off:off + M * ldg].reshape(M, ldg)[:, :N].astype(np.float64)
+    assert np.allclose(out.numpy(), rows.sum(axis=0), rtol=1e-5, atol=1e-3)
+
+
 def _records(pca, names=("abs", "sign", "is_power2_32", "max")):
     from pca_amd.value import state_records
     from pca_amd._lib import gen_candidates
