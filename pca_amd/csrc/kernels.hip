@@ -997,18 +997,20 @@ __global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, Ex
             uint64_t slot = (h >> 1) & a.table_mask;
             for (uint64_t probe = 0;; probe++) {
                 if (probe > a.table_mask) { atomicAdd(&a.counters[1], 1ull); break; }
-                // Device-scope atomics are the cost of this pass, so plain
-                // loads go first.  A key is written once (0 -> key) and a
-                // slot's index only decreases, so any value a plain load
+                // Read-modify-write atomics are the cost of this pass, so
+                // relaxed loads go first.  A key is written once (0 -> key)
+                // and a slot's index only decreases, so any value a load
                 // returns, however stale, is safe to act on: a nonzero key
                 // is final (ours: found; another: next slot), and an index
                 // at or below ours makes the atomicMin a no-op.  Only a slot
                 // that reads empty needs the CAS.
                 unsigned long long *k = &a.table[2 * slot];
-                unsigned long long prev = *k;
+                unsigned long long prev = __hip_atomic_load(k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 if (prev == 0ull) prev = atomicCAS(k, 0ull, (unsigned long long)h);
                 if (prev == 0ull || prev == h) {
-                    if (prev == 0ull || a.table[2 * slot + 1] > idx) atomicMin(&a.table[2 * slot + 1], idx);
+                    unsigned long long *mi = &a.table[2 * slot + 1];
+                    if (prev == 0ull || __hip_atomic_load(mi, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) > idx)
+                        atomicMin(mi, idx);
                     break;
                 }
                 slot = (slot + 1) & a.table_mask;
