@@ -867,14 +867,16 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
 // One-item launches: register budget of 6 waves per SIMD (at most 80
 // VGPRs), measured on the is_power2_32 depth-8 window faster than 5 waves
 // (96 VGPRs), 7 and 8 despite the spills (10.7 ms per launch against 11.1,
-// 10.8 and 15.3).  Multi-item launches (more live context per unit):
-// 4 waves per SIMD.
+// 10.8 and 15.3).  Multi-item launches (more live context per unit): 3
+// waves per SIMD (168 VGPRs, 21 spilled), measured on the server's 1,000-task
+// suite at depth budget 4 against 4 waves (128 VGPRs, 95 spilled) and 2
+// (188, none): 4.70 s of kernel time against 4.97 and 4.80.
 __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanItem one, DevTask task1)
 {
     scan_body<false>(lch, one, task1);
 }
 
-__global__ __launch_bounds__(SCAN_WG, 4) void scan_kernel_multi(ScanLaunch lch, ScanItem one, DevTask task1)
+__global__ __launch_bounds__(SCAN_WG, 3) void scan_kernel_multi(ScanLaunch lch, ScanItem one, DevTask task1)
 {
     scan_body<true>(lch, one, task1);
 }
