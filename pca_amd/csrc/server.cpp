@@ -26,9 +26,13 @@
 // resolution, only scheduled together.
 #include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <new>
 #include <thread>
 #include <vector>
@@ -39,6 +43,79 @@
 using namespace pca;
 
 namespace {
+
+// Host worker threads of one server, created with it: run(n, f) calls
+// f(0..n-1) spread over the workers and the calling thread and returns when
+// all calls are done.  Workers make the server's device current once.
+class HostPool {
+public:
+    HostPool(int device, unsigned workers) : device_(device)
+    {
+        for (unsigned i = 0; i < workers; i++) threads_.emplace_back([this] { loop(); });
+    }
+    ~HostPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (std::thread &t : threads_) t.join();
+    }
+    template <class F>
+    void run(size_t n, F &&f, bool parallel = true)
+    {
+        if (!parallel || n <= 1 || threads_.empty()) {
+            for (size_t i = 0; i < n; i++) f(i);
+            return;
+        }
+        const std::function<void(size_t)> job(std::ref(f));
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            job_ = &job;
+            n_ = n;
+            next_.store(0);
+            pending_ = threads_.size();
+            gen_++;
+        }
+        wake_.notify_all();
+        for (size_t i; (i = next_.fetch_add(1)) < n;) job(i);
+        std::unique_lock<std::mutex> lk(mu_);
+        done_.wait(lk, [this] { return pending_ == 0; });   // every worker has left the job
+        job_ = nullptr;
+    }
+
+private:
+    void loop()
+    {
+        (void)hipSetDevice(device_);
+        uint64_t seen = 0;
+        for (;;) {
+            const std::function<void(size_t)> *job;
+            size_t n;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
+                if (stop_) return;
+                seen = gen_;
+                job = job_;
+                n = n_;
+            }
+            for (size_t i; (i = next_.fetch_add(1)) < n;) (*job)(i);
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    int device_;
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable wake_, done_;
+    const std::function<void(size_t)> *job_ = nullptr;
+    size_t n_ = 0, pending_ = 0;
+    uint64_t gen_ = 0;
+    std::atomic<size_t> next_{0};
+    bool stop_ = false;
+};
 
 // Growable pinned host buffer (contents are not kept when it grows).
 struct PinnedBuf {
@@ -111,6 +188,7 @@ struct pcagpu_server {
     // (pinned host side), and the items' hit records, read back in one go
     PinnedBuf h_stage, h_hits;
     DevBuf<unsigned char> d_stage;
+    std::unique_ptr<HostPool> pool;   // staging and resolution threads
     std::vector<ScanItem> items;
     pcagpu_server_stats_t st;
 
@@ -267,27 +345,12 @@ int apply(pcagpu_server *s, Slot &sl, const Outcome &o)
 
 int advance(pcagpu_server *s, Slot &sl) { return apply(s, sl, resolve_depths(sl)); }
 
-// f(0..n-1) on up to 8 host threads (the calling one included; only it when
-// `parallel` is false), each with the server's device current.
+// f(0..n-1) on the server's host threads (the calling one included; only
+// it when `parallel` is false).
 template <class F>
-void run_parallel(const pcagpu_server *s, size_t n, F &&f, bool parallel = true)
+void run_parallel(pcagpu_server *s, size_t n, F &&f, bool parallel = true)
 {
-    const size_t nt =
-        parallel ? std::min<size_t>({n, (size_t)std::max(1u, std::thread::hardware_concurrency()), 8}) : 1;
-    if (nt <= 1) {
-        for (size_t i = 0; i < n; i++) f(i);
-        return;
-    }
-    std::atomic<size_t> next{0};
-    auto work = [&] {
-        (void)hipSetDevice(s->opts.device);
-        for (size_t i; (i = next.fetch_add(1)) < n;) f(i);
-    };
-    std::vector<std::thread> pool;
-    pool.reserve(nt - 1);
-    for (size_t t = 1; t < nt; t++) pool.emplace_back(work);
-    work();
-    for (std::thread &th : pool) th.join();
+    s->pool->run(n, std::forward<F>(f), parallel);
 }
 
 // A new resident slot on s's device, holding `task` (the caller re-tasks
@@ -393,6 +456,13 @@ extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_serve
     }
     if (s->d_items.ensure(s->opts.max_slots) || s->d_ctl.ensure(s->ctl_words()) ||
         hipHostMalloc((void **)&s->h_ctl, s->ctl_words() * sizeof(unsigned long long)) != hipSuccess) {
+        pcagpu_server_close(s);
+        return PCAGPU_ENOMEM;
+    }
+    // up to 8 host threads in all (the caller is one of them)
+    s->pool.reset(new (std::nothrow) HostPool(
+        s->opts.device, (unsigned)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 8u) - 1u));
+    if (!s->pool) {
         pcagpu_server_close(s);
         return PCAGPU_ENOMEM;
     }
