@@ -53,6 +53,27 @@ def test_server_mix_equals_solo(pca, solo, kw):
             assert got == want[i], (i, got[:2], want[i][:2])
     assert st["items"] > st["launches"], "tasks did not share launches"
     assert st["slots"] <= 8
+    # host time by phase is accounted
+    assert st["stage_ns"] > 0 and st["wait_ns"] > 0 and st["resolve_ns"] > 0 and st["admit_ns"] > 0
+
+
+@pytest.mark.parametrize("warm,max_slots,want", [(None, 64, 4), (0, 64, 0), (8, 3, 3), (2, 64, 2)])
+def test_server_warm_slots(pca, warm, max_slots, want):
+    """Warm slots are opened by server_open (none when switched off, at most
+    max_slots); the first request is re-tasked into one instead of opening
+    a slot, and its answer is the solver's."""
+    from pca_amd.serving import Server, DONE
+    t, d = _tasks(pca)[0]
+    kw = {} if warm is None else {"warm_slots": warm}
+    with Server(max_slots=max_slots, **kw) as sv:
+        assert sv.stats()["slots_opened"] == want
+        rid = sv.submit(t, d)
+        recs = {r["id"]: r for r in sv.drain()}
+        st = sv.stats()
+        sols, n = sv.solutions(rid)
+    assert recs[rid]["status"] == DONE
+    assert (recs[rid]["optimal_depth"], n, [x[0] for x in sols]) == _solo(pca, t, d)
+    assert st["slots_opened"] == max(want, 1)
 
 
 @pytest.mark.parametrize("max_slots,budget", [(2, 200 << 30), (8, 1)])
