@@ -226,15 +226,20 @@ __device__ __forceinline__ void store_slab(uint16_t (*S)[LDS_K], const uint2 q[4
 // column tile n) with every column tile of one row tile -- the workgroups
 // that read the same 128-row slab of A -- on one XCD at consecutive slots,
 // and the slab comes from HBM once into that XCD's L2 instead of once per
-// column tile.  Split K: z selects a K range (partial sums added
-// atomically).
+// column tile.  Split K: z selects a K range; each range's partial tile
+// goes to a workspace and a second kernel adds them.
+//
+// Register budget: 4 waves per SIMD (128 VGPRs, the accumulators in VGPRs,
+// no spills), against 3 waves with the accumulators in AGPRs (152
+// registers): the layer-1 GEMM (M = 131,072, N = 512, K = 180) took 65
+// against 73 us.
 //
 // The MFMA takes B's slab as its first operand and A's as its second, so it
 // produces C^T blocks: lane l holds, for each group g of four accumulators,
 // C[m0 + l % 32][n0 + 8 g + 4 (l / 32) + 0..3] -- four adjacent columns of
 // one row, stored as one 8-byte (BF16) or 16-byte (FP32) vector.
 template <bool AT, bool BT>
-__global__ __launch_bounds__(256) void gemm_kernel(int M, int N, int K, int k_per_split, int tiles_m, int tiles_n,
+__global__ __launch_bounds__(256, 4) void gemm_kernel(int M, int N, int K, int k_per_split, int tiles_m, int tiles_n,
                                                    const uint16_t *__restrict__ A, int lda,
                                                    const uint16_t *__restrict__ B, int ldb,
                                                    const float *__restrict__ bias, int relu, float *C32, int ldc,
