@@ -21,6 +21,13 @@ QUEUED, ACTIVE, DONE, FAILED = 0, 1, 2, 3
 
 
 class Server:
+    """One continuous-batching server (``pcagpu_server_*``).
+
+    ``max_slots`` / ``hbm_budget`` bound the resident tasks; ``warm_slots``
+    slots are opened now (0: none) so early requests allocate nothing; a
+    task adds at least ``prefixes_per_item`` prefixes to a step, and
+    ``step_prefixes`` split over the active tasks when that is more."""
+
     def __init__(self, device=0, max_slots=64, hbm_budget=200 << 30, prefixes_per_item=1 << 14, warm_slots=4,
                  step_prefixes=1 << 18):
         o = ServerOpts(device=device, max_slots=max_slots, hbm_budget=hbm_budget,
