@@ -489,7 +489,6 @@ __global__ void pool_backward_kernel(int T, int C, int E, const float *dZ, const
     }
 }
 
-// 64 columns x 256 row-chunks per workgroup, partial sums added atomically
 // Column sums (bias gradients), two passes without atomics.  Pass 1: a
 // block sums CS_ROWS rows of up to 256 columns, each lane four adjacent
 // columns (one 16-byte load per row when aligned), its four waves on
