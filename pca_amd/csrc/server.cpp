@@ -38,84 +38,12 @@
 #include <vector>
 
 #include "host_dfs.h"
+#include "host_pool.h"
 #include "pcagpu_impl.h"
 
 using namespace pca;
 
 namespace {
-
-// Host worker threads of one server, created with it: run(n, f) calls
-// f(0..n-1) spread over the workers and the calling thread and returns when
-// all calls are done.  Workers make the server's device current once.
-class HostPool {
-public:
-    HostPool(int device, unsigned workers) : device_(device)
-    {
-        for (unsigned i = 0; i < workers; i++) threads_.emplace_back([this] { loop(); });
-    }
-    ~HostPool()
-    {
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            stop_ = true;
-        }
-        wake_.notify_all();
-        for (std::thread &t : threads_) t.join();
-    }
-    template <class F>
-    void run(size_t n, F &&f, bool parallel = true)
-    {
-        if (!parallel || n <= 1 || threads_.empty()) {
-            for (size_t i = 0; i < n; i++) f(i);
-            return;
-        }
-        const std::function<void(size_t)> job(std::ref(f));
-        {
-            std::lock_guard<std::mutex> lk(mu_);
-            job_ = &job;
-            n_ = n;
-            next_.store(0);
-            pending_ = threads_.size();
-            gen_++;
-        }
-        wake_.notify_all();
-        for (size_t i; (i = next_.fetch_add(1)) < n;) job(i);
-        std::unique_lock<std::mutex> lk(mu_);
-        done_.wait(lk, [this] { return pending_ == 0; });   // every worker has left the job
-        job_ = nullptr;
-    }
-
-private:
-    void loop()
-    {
-        (void)hipSetDevice(device_);
-        uint64_t seen = 0;
-        for (;;) {
-            const std::function<void(size_t)> *job;
-            size_t n;
-            {
-                std::unique_lock<std::mutex> lk(mu_);
-                wake_.wait(lk, [&] { return stop_ || gen_ != seen; });
-                if (stop_) return;
-                seen = gen_;
-                job = job_;
-                n = n_;
-            }
-            for (size_t i; (i = next_.fetch_add(1)) < n;) (*job)(i);
-            std::lock_guard<std::mutex> lk(mu_);
-            if (--pending_ == 0) done_.notify_one();
-        }
-    }
-    int device_;
-    std::vector<std::thread> threads_;
-    std::mutex mu_;
-    std::condition_variable wake_, done_;
-    const std::function<void(size_t)> *job_ = nullptr;
-    size_t n_ = 0, pending_ = 0;
-    uint64_t gen_ = 0;
-    std::atomic<size_t> next_{0};
-    bool stop_ = false;
-};
 
 // Growable pinned host buffer (contents are not kept when it grows).
 struct PinnedBuf {
@@ -460,8 +388,10 @@ extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_serve
         return PCAGPU_ENOMEM;
     }
     // up to 8 host threads in all (the caller is one of them)
+    const int dev = s->opts.device;
     s->pool.reset(new (std::nothrow) HostPool(
-        s->opts.device, (unsigned)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 8u) - 1u));
+        (unsigned)std::min<unsigned>(std::max(1u, std::thread::hardware_concurrency()), 8u) - 1u,
+        [dev] { (void)hipSetDevice(dev); }));
     if (!s->pool) {
         pcagpu_server_close(s);
         return PCAGPU_ENOMEM;

@@ -277,6 +277,20 @@ def test_packed_step_matches_step_test(tmp_path):
     assert out.stdout.startswith("ok ")
 
 
+def test_host_pool_under_thread_sanitizer(tmp_path):
+    """The server's host worker pool (pca_amd/csrc/host_pool.h): every
+    index of every run called exactly once, no data race reported by
+    ThreadSanitizer."""
+    exe = tmp_path / "pool_check"
+    src = os.path.join(ROOT, "tests", "native", "pool_check.cpp")
+    inc = os.path.join(ROOT, "pca_amd", "csrc")
+    subprocess.run(["g++", "-O1", "-g", "-std=c++17", "-fsanitize=thread", "-pthread", "-I" + inc, src, "-o",
+                    str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert out.stdout.startswith("ok ") and "ThreadSanitizer" not in out.stderr
+
+
 def test_value_oracle_gradients():
     """The value-network oracle's backward pass (float64, no bf16 rounding)
     against central finite differences of its own loss, on a small batch:
