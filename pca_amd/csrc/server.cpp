@@ -510,7 +510,13 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         Slot &sl = *act[i];
         pcagpu *h = sl.h;
         const Staged &g = stg[i];
-        if (g.rc < 0) return g.rc;
+        if (g.rc < 0) {
+            // this task's batch could not be staged (out of memory, a bad
+            // prefix): its request fails, the others go on; re-tasking the
+            // slot resets its handle's batch
+            fail_request(s, sl);
+            continue;
+        }
         staged_blocks.push_back(g.nb);
         stage_bytes += align16((size_t)g.n * sizeof(pca_prefix_t)) + align16((size_t)g.nb * sizeof(ScanBlock));
         levels = std::max(levels, scan_levels(h->k));
