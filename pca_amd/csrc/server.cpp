@@ -668,8 +668,8 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             pcagpu *h = in_launch[i]->h;
             const uint32_t hits = s->hit_count_host(i);
             if (hits > h->hit_cap) {
-                rc = pcagpu_flush(h);
-                if (rc < 0) return rc;
+                // the rescan with a larger buffer failed: this request fails
+                if (pcagpu_flush(h) < 0) fail_request(s, *in_launch[i]);
                 continue;
             }
             if (hits) {
