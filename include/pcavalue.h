@@ -35,7 +35,9 @@ extern "C" {
  * else M x K (A[m*lda+k]); b_trans: B is stored N x K (B[n*ldb+k]), else
  * K x N (B[k*ldb+n]).  Epilogue, in order: + bias[n] (bias may be NULL),
  * ReLU if relu; then C32[m*ldc+n] = v (or += v if accumulate) when C32 is
- * not NULL, and C16[m*ldc16+n] = bf16(v) when C16 is not NULL. */
+ * not NULL, and C16[m*ldc16+n] = bf16(v) when C16 is not NULL.  Long-K
+ * products (split K) and pcav_colsum use one workspace per device: such
+ * calls on different streams of one device must not overlap. */
 int pcav_gemm(int M, int N, int K, const uint16_t *A, int lda, int a_trans, const uint16_t *B, int ldb,
               int b_trans, const float *bias, int relu, float *C32, int ldc, int accumulate, uint16_t *C16,
               int ldc16, void *stream);
@@ -64,8 +66,9 @@ int pcav_pool_forward(int B, int T, int C, int E, const uint16_t *H, const uint8
 int pcav_pool_backward(int B, int T, int C, int E, const float *dZ, const uint8_t *nvalid, const int32_t *depth,
                        float *dH, float *demb, void *stream);
 
-/* out[n] = sum over m < M of G[m*ldg + n] (bias gradients), n < N; out is
- * overwritten. */
+/* out[n] = sum over m < M of G[m*ldg + n] (bias gradients), n < N, ldg >= N;
+ * out is overwritten.  Partial sums go through the device workspace
+ * pcav_gemm's split K uses (calls on different streams must not overlap). */
 int pcav_colsum(int M, int N, const float *G, int ldg, float *out, void *stream);
 
 /* Binary cross-entropy with logits over B states: loss[0] = mean of
