@@ -297,6 +297,8 @@ def main():
     ap.add_argument("--no-value", action="store_true")
     ap.add_argument("--no-complete", action="store_true")
     ap.add_argument("--suite-budget", type=int, default=3)
+    ap.add_argument("--suite-deep-budget", type=int, default=4,
+                    help="a second suite field at this depth budget (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=200)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -361,6 +363,9 @@ def main():
         line["serving"] = serving_bench(n_req=args.serving_requests)
     if args.suite:
         line["suite"] = suite_bench(args.suite, args.suite_budget)
+        if args.suite_deep_budget:
+            # the deeper budget: the tasks whose optimum needs it dominate
+            line["suite_deep"] = suite_bench(args.suite, args.suite_deep_budget)
     if not args.no_value:
         line["value_net"] = value_bench()
         print("value net done", file=sys.stderr, flush=True)
