@@ -88,6 +88,11 @@ def lib():
         L.or_replay_state.restype = None
         L.or_alu.argtypes = [C.c_int, C.c_uint16, C.c_uint16, C.c_uint8, C.POINTER(C.c_uint8), C.POINTER(C.c_int)]
         L.or_alu.restype = C.c_uint16
+        L.or_last_level.argtypes = [C.POINTER(OrTask), C.c_int, C.POINTER(C.c_uint16), C.POINTER(C.c_uint8),
+                                    C.c_long, C.c_int, C.POINTER(C.c_uint32), C.c_long, C.POINTER(C.c_uint64)]
+        L.or_last_level.restype = C.c_long
+        L.or_live_after.argtypes = [C.c_uint8, C.c_uint16]
+        L.or_live_after.restype = C.c_uint8
         _L = L
     return _L
 
@@ -174,6 +179,86 @@ def frontier(task, depth, split, cap=1 << 22):
     if n < 0:
         raise OverflowError("frontier larger than cap")
     return [(tuple(ins[i * 8:i * 8 + split]), live[i]) for i in range(n)]
+
+
+def frontier_arrays(task, depth, split, cap=1 << 18, max_cap=1 << 28):
+    """The frontier as flat ctypes arrays (8 kernel words per prefix, live
+    masks) and its length: the form last_level takes, for large levels.
+    The buffers start at cap prefixes and grow x8 (the DFS is re-run) up
+    to max_cap."""
+    while True:
+        ins = (C.c_uint16 * (8 * cap))()
+        live = (C.c_uint8 * cap)()
+        n = lib().or_frontier(C.byref(task), depth, split, ins, live, cap)
+        if n >= 0:
+            return ins, live, n
+        if cap >= max_cap:
+            raise OverflowError("frontier larger than max_cap")
+        cap *= 8
+
+
+def last_level(task, depth, prefixes=None, arrays=None, threads=None, cap=1 << 16):
+    """or_last_level: rows (prefix index, j, mask, cannot_pass, first) of the
+    children at kernel position depth-1 whose state has a register equal to
+    the target on every test, in DFS order, and the number of children.
+    Give either prefixes [(insns, live)] or arrays (ins, live, n)."""
+    if threads is None:
+        threads = max(1, min(8, os.cpu_count() or 1))
+    if arrays is None:
+        n = len(prefixes)
+        ins = (C.c_uint16 * (8 * max(n, 1)))()
+        live = (C.c_uint8 * max(n, 1))()
+        for i, (p, lv) in enumerate(prefixes):
+            for q, v in enumerate(p):
+                ins[i * 8 + q] = v
+            live[i] = lv
+    else:
+        ins, live, n = arrays
+    children = C.c_uint64()
+    while True:
+        rows = (C.c_uint32 * (4 * cap))()
+        m = lib().or_last_level(C.byref(task), depth, ins, live, n, threads, rows, cap, C.byref(children))
+        if m < 0:
+            raise MemoryError("or_last_level")
+        if m <= cap:
+            break
+        cap = m
+    out = [(rows[4 * i], rows[4 * i + 1], rows[4 * i + 2], rows[4 * i + 3] & 1, (rows[4 * i + 3] >> 1) & 1)
+           for i in range(m)]
+    return out, children.value
+
+
+def solve_at_depth(task, depth, threads=None):
+    """The exact-OEP solutions of kernel depth `depth` alone (tools/enumerate.c
+    IDDFS at target depth D, every solution kept), computed as the frontier
+    at D-1 (the DFS with exact keys) followed by the last position on
+    or_last_level: try_output on every first-visitor child, in DFS order.
+    Returns (programs, frontier size, children at the last position)."""
+    if depth == 0:
+        r = solve(task, 0, OEP_EXACT, max_solutions=0, start_depth=0)
+        return r["solutions"], 1, 0
+    ins, live, n = frontier_arrays(task, depth, depth - 1)
+    rows, children = last_level(task, depth, arrays=(ins, live, n), threads=threads)
+    L = lib()
+    nin = task.num_inputs
+    sols = []
+    cache = {}
+    for pidx, j, mask, dead, first in rows:
+        if not first or dead:
+            continue
+        if pidx not in cache:
+            cache = {pidx: gen_candidates(live[pidx], depth - 1, depth)}
+        c = cache[pidx][j]
+        kern = list(ins[pidx * 8:pidx * 8 + depth - 1]) + [c]
+        lv = L.or_live_after(live[pidx], c)
+        prog = [(23 << 11) | ((q + 1) << 8) | task.input_ports[q] for q in range(nin)] + kern
+        for r in range(1, 8):
+            if not (lv >> r & 1 and mask >> r & 1):
+                continue
+            full = prog + [(24 << 11) | (r << 8) | task.output_ports[0], 1 << 11]
+            if evaluate(task, full) == task.num_tests:
+                sols.append(full)
+    return sols, n, children
 
 
 def suffix_hits(task, depth, prefix, split, live, cap=1 << 20):

@@ -7,6 +7,7 @@
 #include "pca_oracle.h"
 
 #include <ctype.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -1152,6 +1153,192 @@ void or_replay_state(const or_task_t *task, const uint16_t *kernel, int len, uin
         for (int r = 0; r < 8; r++) out[k * 9 + r] = vm.r[r];
         out[k * 9 + 8] = vm.flags;
     }
+}
+
+/* ---- the last kernel position of a depth, brute force on the VM ----
+ * tools/enumerate.c:405-461 at pos == D-1 below every node of the frontier
+ * at D-1: each child's replay state (scaffold + kernel + HLT, 100 cycles,
+ * tools/enumerate.c:435-452), OEP on that state (first DFS visitor kept),
+ * then try_output (:271-290).  Only children with a register r1..r7 equal
+ * to the target on every test can pass, and two children with the same
+ * state agree on that, so only those children's states are kept and the
+ * first-visitor test runs among them alone.
+ *
+ * Each test's VM is run once per prefix up to the last slot and snapshotted
+ * there; a child is the snapshot with its word in the slot, run on by the
+ * same or_vm_run.  The part before the slot does not read the slot (jumps
+ * are forward and land at most on the HLT), so the result is the replay's.
+ */
+typedef struct {
+    uint32_t prefix, j, mask, flags;
+} lrow_t;
+
+typedef struct {
+    lrow_t *rows;
+    uint8_t *keys;
+    size_t n, cap;
+    uint64_t children;
+} lchunk_t;
+
+typedef struct {
+    const or_task_t *task;
+    int D;
+    const uint16_t *insns;
+    const uint8_t *live;
+    long n;
+    long chunk;
+    long nchunks;
+    lchunk_t *out;
+    long next;           /* chunk counter (atomic)                         */
+    int err;
+} ljob_t;
+
+static int lchunk_push(lchunk_t *c, const lrow_t *r, const uint8_t *key, int klen)
+{
+    if (c->n == c->cap) {
+        size_t nc = c->cap ? c->cap * 2 : 256;
+        lrow_t *nr = realloc(c->rows, nc * sizeof(lrow_t));
+        if (!nr) return -1;
+        c->rows = nr;
+        uint8_t *nk = realloc(c->keys, nc * (size_t)klen);
+        if (!nk) return -1;
+        c->keys = nk;
+        c->cap = nc;
+    }
+    c->rows[c->n] = *r;
+    memcpy(c->keys + c->n * (size_t)klen, key, (size_t)klen);
+    c->n++;
+    return 0;
+}
+
+static void last_level_chunk(ljob_t *jb, long ci)
+{
+    const or_task_t *t = jb->task;
+    const int T = t->num_tests, nin = t->num_inputs, D = jb->D, klen = T * 17;
+    const int slot = nin + D - 1;
+    lchunk_t *c = &jb->out[ci];
+    or_vm_t *snap = malloc((size_t)T * sizeof(or_vm_t));
+    if (!snap) { jb->err = 1; return; }
+    or_vm_t w;
+    uint8_t key[OR_MAX_TESTS * 17];
+    uint16_t cands[4096];
+    long i_end = (ci + 1) * jb->chunk < jb->n ? (ci + 1) * jb->chunk : jb->n;
+    for (long i = ci * jb->chunk; i < i_end; i++) {
+        uint16_t prog[OR_PROG_SIZE];
+        int n = 0;
+        for (int q = 0; q < nin; q++) prog[n++] = ENC_I(O_IN, q + 1, t->input_ports[q]);
+        for (int q = 0; q < D - 1; q++) prog[n++] = jb->insns[i * 8 + q];
+        prog[n++] = 0;                        /* the slot */
+        prog[n++] = ENC_R(O_HLT, 0, 0, 0);
+        for (int k = 0; k < T; k++) {
+            or_vm_t *v = &snap[k];
+            or_vm_init(v);
+            or_vm_load(v, prog, (size_t)n);
+            for (int p = 0; p < nin; p++) v->ports[t->input_ports[p]] = t->tests[k].in[p];
+            while (v->status == OR_RUNNING && v->cycles < 100 && v->pc < slot) or_vm_step(v);
+        }
+        const int nc = or_gen_candidates(cands, 4096, jb->live[i], D - 1, D, 1);
+        c->children += (uint64_t)nc;
+        for (int j = 0; j < nc; j++) {
+            uint32_t mask = 0xFE, dead = 0;
+            int k;
+            for (k = 0; k < T; k++) {
+                w = snap[k];
+                w.prog[slot] = cands[j];
+                or_vm_run(&w, 100);
+                uint32_t m = 0;
+                for (int r = 1; r < 8; r++)
+                    if (w.r[r] == t->tests[k].out[0]) m |= 1u << r;
+                mask &= m;
+                if (!mask) break;
+                for (int r = 0; r < OR_NUM_REGS; r++) {
+                    key[k * 17 + 2 * r] = (uint8_t)(w.r[r] & 0xFF);
+                    key[k * 17 + 2 * r + 1] = (uint8_t)(w.r[r] >> 8);
+                }
+                key[k * 17 + 16] = w.flags;
+            }
+            if (k < T) continue;
+            /* cannot pass: the whole program (OUT, HLT, the task's cycle
+             * limit) does not halt on some test, whatever is output */
+            prog[slot] = cands[j];
+            prog[slot + 1] = ENC_I(O_OUT, 1, t->output_ports[0]);
+            prog[slot + 2] = ENC_R(O_HLT, 0, 0, 0);
+            for (k = 0; k < T && !dead; k++) {
+                or_vm_t *v = &w;
+                or_vm_init(v);
+                or_vm_load(v, prog, (size_t)slot + 3);
+                for (int p = 0; p < nin; p++) v->ports[t->input_ports[p]] = t->tests[k].in[p];
+                if (or_vm_run(v, t->max_cycles) != OR_HALTED) dead = 1;
+            }
+            prog[slot] = 0;
+            prog[slot + 1] = ENC_R(O_HLT, 0, 0, 0);
+            lrow_t r = {(uint32_t)i, (uint32_t)j, mask, dead};
+            if (lchunk_push(c, &r, key, klen)) { jb->err = 1; break; }
+        }
+    }
+    free(snap);
+}
+
+static void *last_level_worker(void *arg)
+{
+    ljob_t *jb = arg;
+    for (;;) {
+        long ci = __atomic_fetch_add(&jb->next, 1, __ATOMIC_RELAXED);
+        if (ci >= jb->nchunks || jb->err) break;
+        last_level_chunk(jb, ci);
+    }
+    return NULL;
+}
+
+long or_last_level(const or_task_t *task, int D, const uint16_t *insns, const uint8_t *live, long n,
+                   int nthreads, uint32_t *rows, long cap, uint64_t *children_out)
+{
+    if (D < 1 || D > 8 || n < 0) return -1;
+    ljob_t jb;
+    memset(&jb, 0, sizeof(jb));
+    jb.task = task;
+    jb.D = D;
+    jb.insns = insns;
+    jb.live = live;
+    jb.n = n;
+    jb.chunk = 2048;
+    jb.nchunks = (n + jb.chunk - 1) / jb.chunk;
+    jb.out = calloc((size_t)(jb.nchunks ? jb.nchunks : 1), sizeof(lchunk_t));
+    if (!jb.out) return -1;
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    pthread_t th[64];
+    int started = 0;
+    for (int q = 1; q < nthreads; q++)
+        if (pthread_create(&th[started], NULL, last_level_worker, &jb) == 0) started++;
+    last_level_worker(&jb);
+    for (int q = 0; q < started; q++) pthread_join(th[q], NULL);
+    long total = 0;
+    uint64_t children = 0;
+    const int klen = task->num_tests * 17;
+    xset_t xs;
+    xset_init(&xs, klen);
+    for (long ci = 0; ci < jb.nchunks && !jb.err; ci++) {
+        lchunk_t *c = &jb.out[ci];
+        children += c->children;
+        for (size_t q = 0; q < c->n; q++) {
+            int ins = xset_insert(&xs, c->keys + q * (size_t)klen, 1);
+            if (ins < 0) { jb.err = 1; break; }
+            if (total < cap) {
+                uint32_t *o = rows + total * 4;
+                o[0] = c->rows[q].prefix;
+                o[1] = c->rows[q].j;
+                o[2] = c->rows[q].mask;
+                o[3] = c->rows[q].flags | (ins == 1 ? 2u : 0u);
+            }
+            total++;
+        }
+    }
+    for (long ci = 0; ci < jb.nchunks; ci++) { free(jb.out[ci].rows); free(jb.out[ci].keys); }
+    free(jb.out);
+    xset_free(&xs);
+    if (children_out) *children_out = children;
+    return jb.err ? -1 : total;
 }
 
 /* ---- exhaustive ALU digest (pins the VM's ALU semantics) ----

@@ -159,6 +159,17 @@ long or_suffix_hits(const or_task_t *task, int D, const uint16_t *prefix,
                     int s, uint8_t live, uint16_t *rows, long cap,
                     uint64_t *candidates_out);
 
+/* The last kernel position of depth D below n prefixes of D-1 instructions
+ * (kernel insns, 8 slots each, and live masks), brute force on the VM with
+ * nthreads threads: every child whose replay state has a register r1..r7
+ * equal to the target on every test, in DFS order, as rows of 4 uint32:
+ * prefix index, candidate index, that register mask, flags (bit 0: the
+ * program cannot pass on some test whatever it outputs; bit 1: the first
+ * child of the rows so far with this state, i.e. kept by the per-depth OEP
+ * when the prefixes are the whole frontier).  Returns the row count (only
+ * the first cap are written) or -1. */
+long or_last_level(const or_task_t *task, int D, const uint16_t *insns, const uint8_t *live, long n,
+                   int nthreads, uint32_t *rows, long cap, uint64_t *children_out);
 /* Exhaustive ALU digest over a in [0,2^16), b in [b_lo,b_hi): see the .c. */
 uint64_t or_alu_digest(int op, uint32_t b_lo, uint32_t b_hi);
 

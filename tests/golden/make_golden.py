@@ -12,13 +12,20 @@ container, where the reference checkout and oracle/_ref exist).
                           32-bit table events these equal the reference's
                           output (checked when they are written).
 
-Usage: python tests/golden/make_golden.py [ref|exact|all] [names...]
+  exact/<task>_d<D>.json  (deep / gen) depths the IDDFS cannot finish in
+                          minutes, solved one depth at a time: the exact-key
+                          DFS to D-1, then the last position brute force on
+                          the VM (or_last_level); gen_* tasks are generated
+                          ones with 24-32 tests and an optimum of 2-3.
+
+Usage: python tests/golden/make_golden.py [ref|exact|all|deep|gen|alu] [names...]
 """
 import json
 import os
 import re
 import subprocess
 import sys
+import time
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
@@ -89,6 +96,95 @@ def make_exact(names):
             json.dump(rec, f)
         print("exact", name, d, r["optimal_depth"], r["num_solutions"], "events", sum(rec["ref32_events"]),
               flush=True)
+
+
+# Deep fixtures: depths the IDDFS of the oracle cannot finish in minutes.
+# Every depth d below the optimum is solved alone (frontier at d-1, last
+# position on or_last_level) and must be empty; the optimum's programs come
+# from the same path.  Where the full IDDFS finishes (max at 3), it is run
+# as well and must agree.
+DEEP_RUNS = [("max", 3, True), ("is_power2", 4, False), ("is_power2_32", 4, False),
+             ("saturating_add", 4, False)]
+
+# Generated tasks of 24-32 tests whose optimum is 2 or 3 instructions: the
+# config-2 test count with solutions, drawn from the chain and suite
+# generators (pca_amd/taskgen.py), seed 2024.
+GEN_SEED = 2024
+GEN_WANT = 6
+
+
+def _deep_record(name, t, maxd, check_iddfs):
+    per_depth = []
+    t0 = time.time()
+    sols = []
+    for d in range(maxd + 1):
+        s0 = time.time()
+        sols, nf, ch = O.solve_at_depth(t, d)
+        per_depth.append({"depth": d, "frontier": nf, "last_children": ch, "solutions": len(sols),
+                          "seconds": round(time.time() - s0, 2)})
+        print(f"  {name} d{d}: frontier {nf}, children {ch}, {len(sols)} solutions, "
+              f"{time.time() - s0:.1f}s", flush=True)
+        if sols:
+            break
+    rec = {"task": name, "max_depth": maxd, "optimal_depth": d if sols else -1,
+           "num_solutions": len(sols), "solutions": sols, "per_depth": per_depth,
+           "method": "per depth: exact-key DFS to D-1 (or_frontier), last position by or_last_level",
+           "seconds": round(time.time() - t0, 1)}
+    if check_iddfs:
+        r = O.solve(t, maxd, O.OEP_EXACT, max_solutions=0)
+        assert r["optimal_depth"] == rec["optimal_depth"] and r["solutions"] == sols, name
+        rec["checked_against_iddfs"] = True
+        rec["tried"] = r["tried"][:maxd + 1]
+        rec["pruned"] = r["pruned"][:maxd + 1]
+    return rec
+
+
+def make_deep(names):
+    os.makedirs(os.path.join(HERE, "exact"), exist_ok=True)
+    for name, d, check in DEEP_RUNS:
+        if names and name not in names:
+            continue
+        rec = _deep_record(name, O.load_task(task_path(name)), d, check)
+        with open(os.path.join(HERE, "exact", f"{name}_d{d}.json"), "w") as f:
+            json.dump(rec, f)
+        print("deep", name, d, rec["optimal_depth"], rec["num_solutions"], flush=True)
+
+
+def make_generated(names):
+    """gen_*: tasks with 24-32 tests and an optimum of 2-3 instructions."""
+    from pca_amd import taskgen
+    found = 0
+    for i in range(400):
+        if found >= GEN_WANT:
+            break
+        task = taskgen.chain_task(GEN_SEED, i, 3) if i % 2 == 0 else taskgen.random_task(GEN_SEED, i)
+        if task.num_tests < 24:
+            continue
+        name = f"gen_{'chain' if i % 2 == 0 else 'suite'}_{i:03d}"
+        ot = O.task_from_bytes(bytes(task))
+        dstar = -1
+        for d in range(4):
+            sols, _, _ = O.solve_at_depth(ot, d)
+            if sols:
+                dstar = d
+                break
+        if dstar not in (2, 3):
+            continue
+        found += 1
+        if names and name not in names:
+            continue
+        with open(task_path(name), "w") as f:
+            f.write(task.to_json() + "\n")
+        rec = _deep_record(name, O.load_task(task_path(name)), dstar, dstar == 2)
+        with open(os.path.join(HERE, "exact", f"{name}_d{dstar}.json"), "w") as f:
+            json.dump(rec, f)
+        print("gen", name, "tests", task.num_tests, "inputs", task.num_inputs, "D*", dstar,
+              rec["num_solutions"], flush=True)
+
+
+if __name__ == "__main__" and len(sys.argv) > 1 and sys.argv[1] in ("deep", "gen"):
+    (make_deep if sys.argv[1] == "deep" else make_generated)(sys.argv[2:])
+    sys.exit(0)
 
 
 if __name__ == "__main__" and (len(sys.argv) < 2 or sys.argv[1] != "alu"):
