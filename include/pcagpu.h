@@ -150,6 +150,12 @@ int  pcagpu_state_records(const pca_task_t *task, const uint16_t *insns, const u
 int  pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **out);
 void pcagpu_close(pcagpu_t *h);
 const char *pcagpu_device_name(pcagpu_t *h);
+/* Re-points an open handle at another task: only the task record is
+ * uploaded (grammar tables, batch and hit buffers stay), and any depth in
+ * progress is dropped.  No reference counterpart (the reference loads one
+ * task per process): it lets one resident handle serve tasks one after
+ * another, as the server's slots do. */
+int  pcagpu_retask(pcagpu_t *h, const pca_task_t *task);
 
 /* Start kernel depth D with the CPU prefixes ending at split s
  * (gpu/hybrid.cu:699-745): resets the depth's hit list. 0 <= s <= D. */
@@ -308,7 +314,12 @@ typedef struct {
     uint32_t step_prefixes;      /* prefixes one step scans over all tasks: a
                                     task adds max(prefixes_per_item,
                                     step_prefixes / active tasks) (default 262144) */
-    uint32_t reserved[2];
+    uint32_t hit_capacity;       /* device hit records per slot before a batch
+                                    is rescanned alone with a larger buffer
+                                    (default 65536)                           */
+    uint32_t fail_step;          /* testing: the n-th step (1-based) fails
+                                    after staging, as a failed allocation
+                                    would (0: off)                            */
 } pcagpu_server_opts_t;
 
 typedef struct {
@@ -339,7 +350,10 @@ int    pcagpu_server_submit(pcagpu_server_t *s, const pca_task_t *task, int max_
 int    pcagpu_server_step(pcagpu_server_t *s);
 /* Completed requests since the last poll (each reported once). */
 int    pcagpu_server_poll(pcagpu_server_t *s, pcagpu_request_t *out, int cap);
-/* Canonical solutions of a completed request (up to 4096 kept). */
+/* Canonical solutions of a completed request: the first
+ * min(cap, PCAGPU_SERVER_KEEP, total) are written; returns the total count
+ * (which can exceed what is kept). */
+#define PCAGPU_SERVER_KEEP 4096
 long   pcagpu_server_solutions(pcagpu_server_t *s, uint64_t id, pca_solution_t *out, long cap);
 int    pcagpu_server_get_stats(pcagpu_server_t *s, pcagpu_server_stats_t *st);
 double pcagpu_server_clock(pcagpu_server_t *s);

@@ -14,6 +14,7 @@ MAX_TESTS = 32
 MAX_PORTS = 8
 MAX_KDEPTH = 8
 MAX_PROG = 16
+SERVER_KEEP = 4096   # PCAGPU_SERVER_KEEP: solutions a server keeps per request
 
 ERRORS = {
     -1: "EINVAL (bad argument or unsupported task)",
@@ -161,7 +162,7 @@ class SolveRes(C.Structure):
 class ServerOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("max_slots", C.c_uint32), ("hbm_budget", C.c_uint64),
                 ("prefixes_per_item", C.c_uint32), ("warm_slots", C.c_uint32), ("step_prefixes", C.c_uint32),
-                ("reserved", C.c_uint32 * 2)]
+                ("hit_capacity", C.c_uint32), ("fail_step", C.c_uint32)]
 
 
 class Request(C.Structure):
@@ -192,6 +193,7 @@ SIGNATURES = [
                                        C.POINTER(C.c_uint16), C.POINTER(C.c_uint8)]),
     ("pcagpu_open", C.c_int, [C.POINTER(Task), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     ("pcagpu_close", None, [C.c_void_p]),
+    ("pcagpu_retask", C.c_int, [C.c_void_p, C.POINTER(Task)]),
     ("pcagpu_device_name", C.c_char_p, [C.c_void_p]),
     ("pcagpu_begin_depth", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("pcagpu_submit", C.c_int, [C.c_void_p, C.POINTER(Prefix), C.c_uint32, C.c_uint64]),

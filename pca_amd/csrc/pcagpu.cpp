@@ -559,6 +559,24 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     return 0;
 }
 
+int pcagpu_retask(pcagpu_t *h, const pca_task_t *task)
+{
+    if (!h || !task) return PCAGPU_EINVAL;
+    if (task->num_outputs != 1 || task->num_inputs < 0 || task->num_inputs > 7 || task->num_tests < 0 ||
+        task->num_tests > PCAGPU_MAX_TESTS || task->output_ports[0] > 7)
+        return PCAGPU_EINVAL;
+    for (int i = 0; i < task->num_inputs; i++)
+        if (task->input_ports[i] > 7) return PCAGPU_EINVAL;
+    (void)hipSetDevice(h->device);
+    h->task = *task;
+    h->T = task->num_tests;
+    fill_devtask(h);
+    HIP_TRY(hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice));
+    h->in_depth = false;
+    h->batch_n = 0;
+    return 0;
+}
+
 void pcagpu_close(pcagpu_t *h)
 {
     if (!h) return;

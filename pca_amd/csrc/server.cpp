@@ -105,6 +105,7 @@ struct pcagpu_server {
     std::map<uint64_t, Request> reqs;
     std::vector<uint64_t> done_unpolled;
     uint64_t next_id = 1;
+    uint32_t step_no = 0;   // steps that staged work (for opts.fail_step)
     double t0 = 0;
     DevBuf<ScanItem> d_items;
     // one step's control block, zeroed once and read back once: scan
@@ -139,23 +140,6 @@ size_t slots_bytes(const pcagpu_server *s)
     for (const Slot &sl : s->slots)
         if (sl.h) b += handle_device_bytes(sl.h);
     return b;
-}
-
-// Re-points a resident handle at another task: only the task record changes
-// (the grammar tables are task independent).
-int retask(pcagpu *h, const pca_task_t *task)
-{
-    if (task->num_outputs != 1 || task->num_inputs < 0 || task->num_inputs > 7 || task->num_tests < 0 ||
-        task->num_tests > PCAGPU_MAX_TESTS || task->output_ports[0] > 7)
-        return PCAGPU_EINVAL;
-    for (int i = 0; i < task->num_inputs; i++)
-        if (task->input_ports[i] > 7) return PCAGPU_EINVAL;
-    h->task = *task;
-    h->T = task->num_tests;
-    fill_devtask(h);
-    HIP_TRY(hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice));
-    h->in_depth = false;
-    return 0;
 }
 
 // The split-depth prefixes of depth D (split D-1, as pcagpu_solve's
@@ -233,7 +217,7 @@ struct Outcome {
 Outcome resolve_depths(Slot &sl)
 {
     Outcome o;
-    std::vector<pca_solution_t> buf(4096);
+    std::vector<pca_solution_t> buf(PCAGPU_SERVER_KEEP);
     for (;;) {
         const long n = pcagpu_end_depth(sl.h, buf.data(), (long)buf.size());
         if (n < 0) {
@@ -289,7 +273,7 @@ int open_slot(pcagpu_server *s, const pca_task_t *task, Slot **out)
     std::memset(&o, 0, sizeof(o));
     o.device = s->opts.device;
     o.batch_capacity = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes);
-    o.hit_capacity = 1u << 16;
+    o.hit_capacity = s->opts.hit_capacity ? s->opts.hit_capacity : 1u << 16;
     pcagpu *h = nullptr;
     const int rc = pcagpu_open(task, &o, &h);
     if (rc < 0) return rc;
@@ -337,7 +321,7 @@ int admit(pcagpu_server *s)
         s->queue.pop_front();
         Request &r = s->reqs[id];
         Slot &sl = *free_slot;
-        if (retask(sl.h, &r.task) < 0) {
+        if (pcagpu_retask(sl.h, &r.task) < 0) {
             sl.req = id;
             fail_request(s, sl);
             continue;
@@ -539,6 +523,13 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         sl.next += g.n;
         in_launch.push_back(&sl);
     }
+    // From here on the staged requests are in the launch: a failure fails
+    // every one of them (their handles' batches are dropped with them, and
+    // the stream is drained so that no copy of this step is still running
+    // when the next step stages), and the server goes on with the rest.
+    auto launch_and_collect = [&]() -> int {
+    s->step_no++;
+    if (s->opts.fail_step && s->step_no == s->opts.fail_step) return PCAGPU_ENOMEM;
     // ---- every item's prefixes and blocks into one pinned buffer, one copy
     if (!s->items.empty()) {
         if ((rc = s->h_stage.ensure(stage_bytes)) || (rc = s->d_stage.ensure(s->h_stage.n))) return rc;
@@ -688,6 +679,17 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             h->batch_first += h->batch_n;
             h->batch_n = 0;
         }
+    }
+    return 0;
+    };
+    if ((rc = launch_and_collect()) < 0) {
+        (void)hipStreamSynchronize(s->stream);
+        for (Slot *sl : in_launch)
+            if (sl->busy) {
+                sl->h->batch_n = 0;
+                fail_request(s, *sl);
+            }
+        return rc;
     }
     lap(s->st.stage_ns);   // hit collection
     // ---- tasks whose depth is complete: resolved in parallel, applied in

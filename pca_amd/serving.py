@@ -14,7 +14,7 @@ import ctypes as C
 import random
 import time
 
-from ._lib import (PcaGpuError, Task, Solution, lib, ServerOpts, Request, ServerStats)
+from ._lib import (PcaGpuError, Task, Solution, lib, ServerOpts, Request, ServerStats, SERVER_KEEP)
 from .solver import Solver, _sol_tuple
 
 QUEUED, ACTIVE, DONE, FAILED = 0, 1, 2, 3
@@ -29,10 +29,14 @@ class Server:
     ``step_prefixes`` split over the active tasks when that is more."""
 
     def __init__(self, device=0, max_slots=64, hbm_budget=200 << 30, prefixes_per_item=1 << 14, warm_slots=4,
-                 step_prefixes=1 << 18):
+                 step_prefixes=1 << 18, hit_capacity=0, fail_step=0):
+        """hit_capacity: device hit records per slot (0: 65536) before a batch
+        is rescanned alone with a larger buffer; fail_step (testing): the
+        n-th step fails after staging, as a failed allocation would."""
         o = ServerOpts(device=device, max_slots=max_slots, hbm_budget=hbm_budget,
                        prefixes_per_item=prefixes_per_item,
-                       warm_slots=0xFFFFFFFF if warm_slots == 0 else warm_slots, step_prefixes=step_prefixes)
+                       warm_slots=0xFFFFFFFF if warm_slots == 0 else warm_slots, step_prefixes=step_prefixes,
+                       hit_capacity=hit_capacity, fail_step=fail_step)
         h = C.c_void_p()
         rc = lib().pcagpu_server_open(C.byref(o), C.byref(h))
         if rc < 0:
@@ -80,7 +84,7 @@ class Server:
         n = lib().pcagpu_server_solutions(self._h, rid, buf, cap)
         if n < 0:
             raise PcaGpuError(n, "server_solutions")
-        return [_sol_tuple(buf[i]) for i in range(min(n, cap))], n
+        return [_sol_tuple(buf[i]) for i in range(min(n, cap, SERVER_KEEP))], n
 
     def stats(self):
         st = ServerStats()
@@ -110,9 +114,15 @@ def poisson_schedule(n, rate, seed=0):
     return out
 
 
+def _programs(sols):
+    return [x[0] for x in sols]
+
+
 def poisson_run(tasks, budgets, arrivals, **server_kw):
     """Replays arrivals (seconds) against a Server; returns per-request
-    (arrival, done, optimal_depth, num_solutions) and the server stats."""
+    (arrival, done, optimal_depth, num_solutions, programs) -- programs: the
+    canonical program list, the first SERVER_KEEP of them -- and the server
+    stats.  Programs are read back after the run, off the clock."""
     with Server(**server_kw) as sv:
         ids = {}
         recs = {}
@@ -126,58 +136,88 @@ def poisson_run(tasks, budgets, arrivals, **server_kw):
             pending = sv.step()
             now = time.perf_counter() - t0
             for r in sv.poll():
-                recs[ids[r["id"]]] = (arrivals[ids[r["id"]]], now, r["optimal_depth"], r["num_solutions"])
+                recs[ids[r["id"]]] = [arrivals[ids[r["id"]]], now, r["optimal_depth"], r["num_solutions"], r["id"],
+                                      r["status"]]
             if pending == 0 and i < len(tasks):
                 time.sleep(max(0.0, min(arrivals[i] - (time.perf_counter() - t0), 0.001)))
-        return [recs[k] for k in range(len(tasks))], sv.stats()
+        out = []
+        for k in range(len(tasks)):
+            arr, done, od, ns, rid, status = recs[k]
+            progs = _programs(sv.solutions(rid)[0]) if status == DONE else None
+            out.append((arr, done, od, ns, progs))
+        return out, sv.stats()
 
 
-def sequential_run(tasks, budgets, arrivals):
-    """The same arrivals through the plain one-task-at-a-time GPU path."""
+def sequential_run(tasks, budgets, arrivals, warm=True):
+    """The same arrivals through the plain one-task-at-a-time GPU path
+    (pcagpu_solve).  warm: one resident handle re-tasked per request, opened
+    before the first arrival (allocation off the clock, as the server's
+    warm slots); otherwise a handle opened per request."""
     recs = []
-    t0 = time.perf_counter()
-    for task, budget, arr in zip(tasks, budgets, arrivals):
-        now = time.perf_counter() - t0
-        if now < arr:
-            time.sleep(arr - now)
-        with Solver(task) as s:
-            r = s.solve(budget, cap=1)
-        recs.append((arr, time.perf_counter() - t0, r.optimal_depth, r.num_solutions))
+    sv = Solver(tasks[0]) if warm and tasks else None
+    try:
+        t0 = time.perf_counter()
+        for task, budget, arr in zip(tasks, budgets, arrivals):
+            now = time.perf_counter() - t0
+            if now < arr:
+                time.sleep(arr - now)
+            if sv is not None:
+                sv.retask(task)
+                r = sv.solve(budget, cap=SERVER_KEEP)
+            else:
+                with Solver(task) as s:
+                    r = s.solve(budget, cap=SERVER_KEEP)
+            recs.append((arr, time.perf_counter() - t0, r.optimal_depth, r.num_solutions, _programs(r.solutions)))
+    finally:
+        if sv is not None:
+            sv.close()
     return recs
 
 
 def suite_run(tasks, budget, timeout=1200.0, **server_kw):
     """A batched suite: every task submitted at once, each server step one
     persistent launch over all active tasks.  Returns per-task
-    (optimal_depth, num_solutions), the wall time from the first submit to
-    the last result, the server stats (with ``open_s``, the time
+    (optimal_depth, num_solutions, programs), the wall time from the first
+    submit to the last result, the server stats (with ``open_s``, the time
     ``pcagpu_server_open`` took, warm slots included)."""
     t_open = time.perf_counter()
     with Server(**server_kw) as sv:
         t0 = time.perf_counter()
         ids = {sv.submit(t, budget): i for i, t in enumerate(tasks)}
-        out = [None] * len(tasks)
+        done = [None] * len(tasks)
         for r in sv.drain(timeout):
-            out[ids[r["id"]]] = (r["optimal_depth"], r["num_solutions"])
+            done[ids[r["id"]]] = r
         wall = time.perf_counter() - t0
         st = sv.stats()
         st["open_s"] = t0 - t_open
+        out = [(r["optimal_depth"], r["num_solutions"],
+                _programs(sv.solutions(r["id"])[0]) if r["status"] == DONE else None) for r in done]
         return out, wall, st
 
 
-def sequential_suite(tasks, budget):
-    """The same suite one task at a time through Solver.solve."""
+def sequential_suite(tasks, budget, warm=True):
+    """The same suite one task at a time through Solver.solve (one handle
+    re-tasked per task when warm, opened before the clock starts)."""
     out = []
-    t0 = time.perf_counter()
-    for t in tasks:
-        with Solver(t) as s:
-            r = s.solve(budget, cap=1)
-        out.append((r.optimal_depth, r.num_solutions))
-    return out, time.perf_counter() - t0
+    sv = Solver(tasks[0]) if warm and tasks else None
+    try:
+        t0 = time.perf_counter()
+        for t in tasks:
+            if sv is not None:
+                sv.retask(t)
+                r = sv.solve(budget, cap=SERVER_KEEP)
+            else:
+                with Solver(t) as s:
+                    r = s.solve(budget, cap=SERVER_KEEP)
+            out.append((r.optimal_depth, r.num_solutions, _programs(r.solutions)))
+        return out, time.perf_counter() - t0
+    finally:
+        if sv is not None:
+            sv.close()
 
 
 def latency_stats(recs):
-    lat = sorted(done - arr for arr, done, _, _ in recs)
+    lat = sorted(r[1] - r[0] for r in recs)
     if not lat:
         return {}
 

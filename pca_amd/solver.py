@@ -79,6 +79,16 @@ class Solver:
     def reset_stats(self):
         lib().pcagpu_reset_stats(self._h)
 
+    def retask(self, task):
+        """Points this resident handle at another task (pcagpu_retask): no
+        device allocation, only the task record is uploaded."""
+        if not isinstance(task, Task):
+            raise TypeError("task must be a pca_amd.Task")
+        rc = lib().pcagpu_retask(self._h, C.byref(task))
+        if rc < 0:
+            raise PcaGpuError(rc, "pcagpu_retask")
+        self.task = task
+
     # whole solve -------------------------------------------------------
     def solve(self, max_depth, start_depth=0, split=-1, producer="gpu", time_limit=0.0,
               cap=4096, verbose=False):
