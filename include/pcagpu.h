@@ -233,6 +233,11 @@ typedef struct {
     double   seconds;
     uint64_t candidates[PCAGPU_MAX_KDEPTH + 1]; /* per depth                  */
     uint64_t prefixes[PCAGPU_MAX_KDEPTH + 1];
+    /* per depth, the prefix levels: nodes tried and OEP-pruned (host DFS,
+     * tools/enumerate.c counters), or children generated and dropped as
+     * duplicates (GPU wavefront) */
+    uint64_t prefix_nodes[PCAGPU_MAX_KDEPTH + 1];
+    uint64_t prefix_pruned[PCAGPU_MAX_KDEPTH + 1];
 } pcagpu_result_t;
 
 /* Runs IDDFS on an open handle.  Writes up to cap solutions of the optimal
@@ -341,6 +346,8 @@ typedef struct {
      * depth), staging the batches, waiting for the launches, and finished
      * depths (resolution, the next depth's prefixes) */
     uint64_t admit_ns, stage_ns, wait_ns, resolve_ns;
+    uint64_t rescans;               /* batches rescanned alone after their hits
+                                       overflowed the slot's buffer           */
 } pcagpu_server_stats_t;
 
 int    pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_server_t **out);

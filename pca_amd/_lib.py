@@ -156,7 +156,9 @@ class SolveRes(C.Structure):
     _fields_ = [("optimal_depth", C.c_int), ("num_solutions", C.c_long), ("depths_done", C.c_int),
                 ("timed_out", C.c_int), ("seconds", C.c_double),
                 ("candidates", C.c_uint64 * (MAX_KDEPTH + 1)),
-                ("prefixes", C.c_uint64 * (MAX_KDEPTH + 1))]
+                ("prefixes", C.c_uint64 * (MAX_KDEPTH + 1)),
+                ("prefix_nodes", C.c_uint64 * (MAX_KDEPTH + 1)),
+                ("prefix_pruned", C.c_uint64 * (MAX_KDEPTH + 1))]
 
 
 class ServerOpts(C.Structure):
@@ -174,7 +176,8 @@ class Request(C.Structure):
 class ServerStats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "submitted", "admitted", "completed", "failed", "launches", "items", "kernel_ns", "candidates",
-        "suffixes", "slots", "slots_opened", "device_bytes", "admit_ns", "stage_ns", "wait_ns", "resolve_ns")]
+        "suffixes", "slots", "slots_opened", "device_bytes", "admit_ns", "stage_ns", "wait_ns", "resolve_ns",
+        "rescans")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

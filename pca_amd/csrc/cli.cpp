@@ -127,33 +127,42 @@ int main(int argc, char **argv)
         const double ds = now_s();
         uint64_t nodes = 0, pruned = 0;
         long n;
+        long total;
         if (mode == MODE_HYBRID) {
-            size_t cap = 1 << 16;
-            for (;;) {
-                pre.resize(cap);
-                n = pcagpu_host_frontier(&task, D, split, pre.data(), (long)cap, &nodes, &pruned);
-                if (n < 0) { std::fprintf(stderr, "prefix producer failed: %ld\n", n); return 1; }
-                if ((size_t)n < cap) break;
-                cap *= 4;
+            // one DFS walk streams the prefixes into submit; full batches run
+            // on the device while the walk goes on
+            pcagpu_solve_opts_t so = {};
+            so.max_depth = D;
+            so.start_depth = D;
+            so.split = split;
+            so.producer = 1;
+            pcagpu_result_t res;
+            if ((rc = pcagpu_solve(h, &so, &res, sols.data(), max_solutions)) < 0) {
+                std::fprintf(stderr, "depth %d failed: %d\n", D, rc);
+                return 1;
             }
+            n = (long)res.prefixes[D];
+            nodes = res.prefix_nodes[D];
+            pruned = res.prefix_pruned[D];
+            total = res.optimal_depth == D ? res.num_solutions : 0;
         } else {
             n = pcagpu_frontier(h, D, split, nullptr, 0);
             if (n < 0) { std::fprintf(stderr, "wavefront failed: %ld\n", n); return 1; }
             pre.resize((size_t)(n > 0 ? n : 1));
             if (n > 0 && pcagpu_frontier(h, D, split, pre.data(), n) < 0) return 1;
-        }
-        if ((rc = pcagpu_begin_depth(h, D, split)) < 0) return 1;
-        for (long i = 0; i < n; i += batch) {
-            const uint32_t c = (uint32_t)((n - i) < batch ? (n - i) : batch);
-            if ((rc = pcagpu_submit(h, pre.data() + i, c, (uint64_t)i)) < 0) {
-                std::fprintf(stderr, "submit failed: %d\n", rc);
+            if ((rc = pcagpu_begin_depth(h, D, split)) < 0) return 1;
+            for (long i = 0; i < n; i += batch) {
+                const uint32_t c = (uint32_t)((n - i) < batch ? (n - i) : batch);
+                if ((rc = pcagpu_submit(h, pre.data() + i, c, (uint64_t)i)) < 0) {
+                    std::fprintf(stderr, "submit failed: %d\n", rc);
+                    return 1;
+                }
+            }
+            total = pcagpu_end_depth(h, sols.data(), max_solutions);
+            if (total < 0) {
+                std::fprintf(stderr, "depth %d failed: %ld\n", D, total);
                 return 1;
             }
-        }
-        const long total = pcagpu_end_depth(h, sols.data(), max_solutions);
-        if (total < 0) {
-            std::fprintf(stderr, "depth %d failed: %ld\n", D, total);
-            return 1;
         }
         pcagpu_stats_t st;
         pcagpu_get_stats(h, &st);

@@ -181,6 +181,9 @@ int pcagpu_complete(pcagpu_t *h, const pca_search_state_t *start, int max_depth,
 {
     if (!h || !start || !solution_insns || !solution_len) return PCAGPU_EINVAL;
     *solution_len = 0;
+    (void)hipSetDevice(h->device);
+    const int rc0 = settle_scan(h);   // a scan in flight shares the stream and buffers
+    if (rc0) return rc0;
     h->complete_explored = 0;
     if (leaf_outputs(&h->task, start)) return 1;   // the start state already solves
     int max_r = max_depth - (int)start->depth;

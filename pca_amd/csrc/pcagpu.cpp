@@ -159,10 +159,12 @@ ScanTile current_tile(const pcagpu *h, int *cand)
     return kTiles[*cand];
 }
 
-void tile_feedback(pcagpu *h, int cand, float ms, uint64_t parents)
+// (a launch queued before the previous one's feedback arrived may have been
+// given the same candidate: only the candidate being tried is credited)
+void tile_feedback(pcagpu *h, int cand, int k_launch, float ms, uint64_t parents)
 {
-    const int k = std::min(std::max(h->k, 0), (int)MAXLEV);
-    if (cand < 0 || parents < kTileMinParents) return;
+    const int k = std::min(std::max(k_launch, 0), (int)MAXLEV);
+    if (cand < 0 || cand != h->tile_tried[k] || parents < kTileMinParents) return;
     h->tile_cost[k][cand] = (double)ms * 1e6 / (double)parents;
     if (++h->tile_tried[k] < kNTiles) return;
     int best = 0;
@@ -181,9 +183,10 @@ struct LaunchOut {
 };
 
 // One scan launch through the handle's graph (built on the first call):
-// prefix and block upload, clears of the launch counters, the kernel
-// between two event records, read-back of the counters.  Per launch only
-// the upload sizes and the kernel parameters change.
+// prefix and block upload, an event marking the uploads done, clears of the
+// launch counters, the kernel between two event records, read-back of the
+// counters.  Per launch only the upload sizes and the kernel parameters
+// change.  Queued on the handle's stream; nothing waits here.
 int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uint32_t n, uint32_t n_blocks)
 {
     ScanNodeArgs ka;
@@ -193,7 +196,7 @@ int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
     const size_t bbytes = (size_t)std::max<uint32_t>(n_blocks, 1u) * sizeof(ScanBlock);
     LaunchOut *o = reinterpret_cast<LaunchOut *>(h->h_out);
     if (!h->g_exec) {
-        hipGraphNode_t zero, ev0, ev1, r0, r1, r2;
+        hipGraphNode_t zero, up, ev0, ev1, r0, r1, r2;
         uint32_t *zp_ptr[4] = {reinterpret_cast<uint32_t *>(h->d_work.p), reinterpret_cast<uint32_t *>(h->d_tstat.p),
                                reinterpret_cast<uint32_t *>(h->d_counters.p), h->d_hit_count.p};
         const uint32_t zp_n[4] = {2, 4 * MAXT, 8, 1};
@@ -205,7 +208,8 @@ int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
                                         hipMemcpyHostToDevice));
         HIP_TRY(hipGraphAddMemcpyNode1D(&h->g_blocks, h->g_graph, &h->g_prefixes, 1, h->d_blocks.p, h->h_blocks,
                                         bbytes, hipMemcpyHostToDevice));
-        HIP_TRY(hipGraphAddKernelNode(&zero, h->g_graph, &h->g_blocks, 1, &zp));
+        HIP_TRY(hipGraphAddEventRecordNode(&up, h->g_graph, &h->g_blocks, 1, h->ev_up));
+        HIP_TRY(hipGraphAddKernelNode(&zero, h->g_graph, &up, 1, &zp));
         HIP_TRY(hipGraphAddEventRecordNode(&ev0, h->g_graph, &zero, 1, h->ev0));
         HIP_TRY(hipGraphAddKernelNode(&h->g_kernel, h->g_graph, &ev0, 1, &kp));
         HIP_TRY(hipGraphAddEventRecordNode(&ev1, h->g_graph, &h->g_kernel, 1, h->ev1));
@@ -224,19 +228,24 @@ int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
         HIP_TRY(hipGraphExecKernelNodeSetParams(h->g_exec, h->g_kernel, &kp));
     }
     HIP_TRY(hipGraphLaunch(h->g_exec, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
     return 0;
 }
 
-// The same launch with plain stream calls.
-int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uint32_t n, uint32_t n_blocks)
+// The same launch with plain stream calls.  upload = false relaunches the
+// kernel over the batch already on the device (the redo of a launch whose
+// hits overflowed).
+int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uint32_t n, uint32_t n_blocks,
+               bool upload)
 {
     LaunchOut *o = reinterpret_cast<LaunchOut *>(h->h_out);
-    HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
-                           h->stream));
-    if (n_blocks)
-        HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
-                               hipMemcpyHostToDevice, h->stream));
+    if (upload) {
+        HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
+                               h->stream));
+        if (n_blocks)
+            HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
+                                   hipMemcpyHostToDevice, h->stream));
+        HIP_TRY(hipEventRecord(h->ev_up, h->stream));
+    }
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_tstat.p, 0, 2 * MAXT * sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_counters.p, 0, 4 * sizeof(unsigned long long), h->stream));
@@ -248,59 +257,102 @@ int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
     HIP_TRY(hipMemcpyAsync(o->cnt, h->d_counters.p, sizeof(o->cnt), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipMemcpyAsync(o->ts, h->d_tstat.p, sizeof(o->ts), hipMemcpyDeviceToHost, h->stream));
     HIP_TRY(hipMemcpyAsync(&o->hits, h->d_hit_count.p, sizeof(o->hits), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
     return 0;
 }
 
-// One scan launch over the staged batch (n prefixes) and n_blocks blocks,
-// its hits appended to h->raw.  A launch whose hits overflow the device
-// buffer is repeated with a larger one: no hit is dropped.
-int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile, int cand)
+// The kernel arguments of a one-item scan launch over the device batch.
+int scan_args(pcagpu *h, uint32_t n, uint64_t units, ScanTile tile, int k, ScanItem *it, ScanLaunch *l)
 {
-    const LaunchOut *o = reinterpret_cast<const LaunchOut *>(h->h_out);
-    const bool adapt = !h->opts.fixed_test_order;
+    std::memset(it, 0, sizeof(*it));
+    it->task = h->d_task.p;
+    it->prefixes = h->d_batch.p;
+    it->blocks = h->d_blocks.p;
+    it->n_prefix = n;
+    it->unit_begin = 0;
+    it->n_units = units;
+    it->D = h->D;
+    it->s = h->D - k;
+    it->k = k;
+    it->tile = tile;
+    it->hits = h->d_hits.p;
+    it->hit_count = h->d_hit_count.p;
+    it->hit_cap = h->hit_cap;
+    l->items = h->d_items.p;
+    l->n_items = 1;
+    l->levels = scan_levels(k);
+    l->n_units = units;
+    l->cand = cand_dev(h);
+    l->counters = h->d_counters.p;
+    l->work = h->d_work.p;
+    l->tstat = h->opts.fixed_test_order ? nullptr : h->d_tstat.p;
+    const int grid = grid_for(h, units, false, l->levels);
+    l->grab = scan_grab(k, units, grid);
+    l->pad = 0;
+    return grid;
+}
+
+// Queues one scan launch over h_batch[0, n) and h_blocks[0, n_blocks)
+// (the previous one must have been collected).  Returns at once; the host
+// may refill the staging buffers after wait_uploads.
+int start_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile, int cand)
+{
     ScanItem it;
     ScanLaunch l;
+    const int grid = scan_args(h, n, units, tile, h->k, &it, &l);
+    int rc = h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks, true)
+                                 : graph_scan(h, l, it, grid, n, n_blocks);
+    if (rc) return rc;
+    h->pend.on = true;
+    h->pend.n = n;
+    h->pend.n_blocks = n_blocks;
+    h->pend.units = units;
+    h->pend.tile = tile;
+    h->pend.cand = cand;
+    h->pend.k = h->k;
+    h->pend.batch_first = h->batch_first;
+    h->up_waited = false;
+    return 0;
+}
+
+// Blocks until the launch in flight has read its prefixes and blocks from
+// the pinned staging buffers (usually long done: the uploads come first).
+int wait_uploads(pcagpu *h)
+{
+    if (h->up_waited) return 0;
+    if (h->pend.on) HIP_TRY(hipEventSynchronize(h->ev_up));
+    h->up_waited = true;
+    return 0;
+}
+
+int settle_scan(pcagpu *h)
+{
+    if (!h->pend.on) return 0;
+    h->pend.on = false;
+    h->up_waited = true;
+    const LaunchOut *o = reinterpret_cast<const LaunchOut *>(h->h_out);
+    const auto p = h->pend;
+    HIP_TRY(hipStreamSynchronize(h->stream));
     float ms = 0.f;
     for (;;) {
-        std::memset(&it, 0, sizeof(it));
-        it.task = h->d_task.p;
-        it.prefixes = h->d_batch.p;
-        it.blocks = h->d_blocks.p;
-        it.n_prefix = n;
-        it.unit_begin = 0;
-        it.n_units = units;
-        it.D = h->D;
-        it.s = h->s;
-        it.k = h->k;
-        it.tile = tile;
-        it.hits = h->d_hits.p;
-        it.hit_count = h->d_hit_count.p;
-        it.hit_cap = h->hit_cap;
-        l.items = h->d_items.p;
-        l.n_items = 1;
-        l.levels = scan_levels(h->k);
-        l.n_units = units;
-        l.cand = cand_dev(h);
-        l.counters = h->d_counters.p;
-        l.work = h->d_work.p;
-        l.tstat = adapt ? h->d_tstat.p : nullptr;
-        const int grid = grid_for(h, units, false, l.levels);
-        l.grab = scan_grab(h->k, units, grid);
-        l.pad = 0;
-        int rc = h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks) : graph_scan(h, l, it, grid, n, n_blocks);
-        if (rc) return rc;
         HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);
         h->stats.launches++;
         if (o->hits <= h->hit_cap) break;
+        // overflow: a larger buffer, and the kernel again over the batch
+        // still on the device
         const uint32_t want = o->hits + o->hits / 2 + 1024;
         h->d_hits.release();
         h->hit_cap = 0;   // until the larger buffer exists: launches write no records
-        if ((rc = h->d_hits.ensure(want))) return rc;
+        int rc = h->d_hits.ensure(want);
+        if (rc) return rc;
         h->hit_cap = want;
+        ScanItem it;
+        ScanLaunch l;
+        const int grid = scan_args(h, p.n, p.units, p.tile, p.k, &it, &l);
+        if ((rc = eager_scan(h, l, it, grid, p.n, p.n_blocks, false))) return rc;
+        HIP_TRY(hipStreamSynchronize(h->stream));
     }
-    h->stats.scan_tile = (uint64_t)tile.block_kids | ((uint64_t)tile.mask_tests << 8);
+    h->stats.scan_tile = (uint64_t)p.tile.block_kids | ((uint64_t)p.tile.mask_tests << 8);
     h->stats.candidates += o->cnt[0];
     h->stats.suffixes += o->cnt[1];
     h->stats.tests_evaluated += o->cnt[2];
@@ -312,15 +364,15 @@ int run_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile 
         HIP_TRY(hipMemcpy(buf.data(), h->d_hits.p, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost));
         for (const HitRec &r : buf) {
             HostHit x;
-            x.prefix = h->batch_first + r.prefix;
+            x.prefix = p.batch_first + r.prefix;
             x.mask = r.mask;
             x.flags = r.flags;
             std::memcpy(x.j, r.j, sizeof(x.j));
             h->raw.push_back(x);
         }
     }
-    tile_feedback(h, cand, ms, o->cnt[3]);
-    if (!adapt) return 0;
+    tile_feedback(h, p.cand, p.k, ms, o->cnt[3]);
+    if (h->opts.fixed_test_order) return 0;
     unsigned long long ts[2 * MAXT];
     std::memcpy(ts, o->ts, sizeof(ts));
     return adapt_test_order(h, ts);
@@ -369,26 +421,35 @@ int ensure_block_buffers(pcagpu *h)
     return 0;
 }
 
-// All scan launches of one flush: k <= 1 in one launch (a unit per prefix);
-// k >= 2 in launches of at most kBlockMax blocks.
+// Queues the scan launches of one flush: k <= 1 in one launch (a unit per
+// prefix); k >= 2 in launches of at most kBlockMax blocks.  Each launch
+// first collects the one before it (they share the hit buffer and the
+// counters); the last one is left in flight.  For k >= 2 the blocks are
+// built while the previous launch still runs.
 int scan_batch(pcagpu *h, uint32_t n)
 {
-    int cand;
-    ScanTile tile = current_tile(h, &cand);
-    if (h->k <= 1) return run_scan(h, n, 0, n, tile, cand);
-    int rc0 = ensure_block_buffers(h);
-    if (rc0) return rc0;
+    int cand, rc;
+    ScanTile tile;
+    if (h->k <= 1) {
+        if ((rc = settle_scan(h))) return rc;
+        tile = current_tile(h, &cand);
+        return start_scan(h, n, 0, n, tile, cand);
+    }
+    if ((rc = wait_uploads(h)) || (rc = ensure_block_buffers(h))) return rc;
+    tile = current_tile(h, &cand);
     uint32_t nb = 0;
     for (uint32_t i = 0; i < n; i++) {
         if (add_prefix_blocks(h, i, &nb, tile.block_kids)) continue;
         if (nb == 0) return PCAGPU_EOVERFLOW;
-        int rc = run_scan(h, n, nb, nb, tile, cand);
-        if (rc) return rc;
-        tile = current_tile(h, &cand);   // the tuner may have moved on
+        if ((rc = settle_scan(h)) || (rc = start_scan(h, n, nb, nb, tile, cand)) || (rc = wait_uploads(h)))
+            return rc;
+        tile = current_tile(h, &cand);
         nb = 0;
         i--;
     }
-    return nb ? run_scan(h, n, nb, nb, tile, cand) : 0;
+    if (!nb) return 0;
+    if ((rc = settle_scan(h))) return rc;
+    return start_scan(h, n, nb, nb, tile, cand);
 }
 
 size_t handle_device_bytes(const pcagpu *h)
@@ -403,6 +464,8 @@ size_t handle_device_bytes(const pcagpu *h)
 int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank)
 {
     if (!h->in_depth) return PCAGPU_ESTATE;
+    int rc = wait_uploads(h);
+    if (rc) return rc;
     if (h->batch_n + n > h->batch_cap) return PCAGPU_EOVERFLOW;
     for (uint32_t i = 0; i < n; i++) {
         if (p[i].num_insns != h->s) return PCAGPU_EINVAL;
@@ -445,8 +508,8 @@ int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units
         *n_units = h->batch_n;
         return 0;
     }
-    int rc = ensure_block_buffers(h);
-    if (rc) return rc;
+    int rc;
+    if ((rc = wait_uploads(h)) || (rc = ensure_block_buffers(h))) return rc;
     uint32_t nb = 0;
     for (uint32_t i = 0; i < h->batch_n; i++)
         if (!add_prefix_blocks(h, i, &nb, tile.block_kids)) return PCAGPU_EOVERFLOW;
@@ -537,7 +600,9 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     h->devname = prop.name[0] ? prop.name : prop.gcnArchName;
     h->cu_count = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess) return fail(PCAGPU_EHIP);
-    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) return fail(PCAGPU_EHIP);
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess ||
+        hipEventCreateWithFlags(&h->ev_up, hipEventDisableTiming) != hipSuccess)
+        return fail(PCAGPU_EHIP);
     fill_devtask(h);
     if (!(h->d_grammar = shared_grammar_dev(h->device))) return fail(PCAGPU_ENOMEM);
     if ((rc = h->d_batch.ensure(h->batch_cap)) ||
@@ -568,6 +633,8 @@ int pcagpu_retask(pcagpu_t *h, const pca_task_t *task)
     for (int i = 0; i < task->num_inputs; i++)
         if (task->input_ports[i] > 7) return PCAGPU_EINVAL;
     (void)hipSetDevice(h->device);
+    int rc = settle_scan(h);
+    if (rc) return rc;
     h->task = *task;
     h->T = task->num_tests;
     fill_devtask(h);
@@ -615,6 +682,7 @@ void pcagpu_close(pcagpu_t *h)
     if (h->h_goff) (void)hipHostFree(h->h_goff);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->ev_up) (void)hipEventDestroy(h->ev_up);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -624,6 +692,8 @@ const char *pcagpu_device_name(pcagpu_t *h) { return h ? h->devname.c_str() : ""
 int pcagpu_begin_depth(pcagpu_t *h, int target_depth, int split_depth)
 {
     if (!h) return PCAGPU_EINVAL;
+    int rc = settle_scan(h);
+    if (rc) return rc;
     if (target_depth < 0 || target_depth > PCAGPU_MAX_KDEPTH || split_depth < 0 || split_depth > target_depth)
         return PCAGPU_EINVAL;
     if (h->task.num_inputs + target_depth + 2 > PCAGPU_MAX_PROG) return PCAGPU_EINVAL;
@@ -642,13 +712,15 @@ int pcagpu_begin_depth(pcagpu_t *h, int target_depth, int split_depth)
 int pcagpu_flush(pcagpu_t *h)
 {
     if (!h || !h->in_depth) return PCAGPU_ESTATE;
-    if (h->batch_n == 0) return (int)std::min<size_t>(h->raw.size(), 0x7FFFFFFF);
     (void)hipSetDevice(h->device);
-    const uint32_t n = h->batch_n;
     int rc;
-    if ((rc = scan_batch(h, n))) return rc;
-    h->batch_first += n;
-    h->batch_n = 0;
+    if (h->batch_n) {
+        const uint32_t n = h->batch_n;
+        if ((rc = scan_batch(h, n))) return rc;
+        h->batch_first += n;
+        h->batch_n = 0;
+    }
+    if ((rc = settle_scan(h))) return rc;
     return (int)std::min<size_t>(h->raw.size(), 0x7FFFFFFF);
 }
 
@@ -659,13 +731,21 @@ int pcagpu_submit(pcagpu_t *h, const pca_prefix_t *p, uint32_t n, uint64_t first
     for (uint32_t i = 0; i < n; i++) {
         if (p[i].num_insns != h->s) return PCAGPU_EINVAL;
         if (!h->ranks.empty() && first_rank + i <= h->ranks.back()) return PCAGPU_EINVAL;
+        int rc;
+        if (!h->up_waited && (rc = wait_uploads(h))) return rc;
         h->h_batch[h->batch_n++] = p[i];
         h->prefixes.push_back(p[i]);
         h->ranks.push_back(first_rank + i);
         h->stats.prefixes++;
         if (h->batch_n == h->batch_cap) {
-            int rc = pcagpu_flush(h);
-            if (rc < 0) return rc;
+            // a full batch goes to the device and runs while the caller
+            // produces the next one; the next full batch (or a flush)
+            // collects it
+            (void)hipSetDevice(h->device);
+            const uint32_t nb = h->batch_n;
+            if ((rc = scan_batch(h, nb))) return rc;
+            h->batch_first += nb;
+            h->batch_n = 0;
         }
     }
     return 0;
@@ -1069,6 +1149,8 @@ extern "C" long pcagpu_host_frontier(const pca_task_t *task, int D, int s, pca_p
 extern "C" long pcagpu_raw_hits(pcagpu_t *h, uint16_t *rows, long cap)
 {
     if (!h || !h->in_depth) return PCAGPU_ESTATE;
+    const int rc = settle_scan(h);
+    if (rc) return rc;
     std::vector<HostHit> v = h->raw;
     const int k = h->k;
     std::sort(v.begin(), v.end(), [k](const HostHit &a, const HostHit &b) {
@@ -1101,6 +1183,10 @@ uint64_t host_dg_mix(uint64_t x)
 extern "C" int pcagpu_alu_digest(pcagpu_t *h, int op, uint32_t b_lo, uint32_t b_hi, uint64_t *out)
 {
     if (!out || b_lo > b_hi || b_hi > 65536 || op < (int)OP_ADD || op > (int)OP_CMP) return PCAGPU_EINVAL;
+    if (h) {
+        const int rc = settle_scan(h);
+        if (rc) return rc;
+    }
     if (!h) {
         uint64_t acc = 0;
         for (uint32_t b = b_lo; b < b_hi; b++)
@@ -1130,13 +1216,17 @@ extern "C" int pcagpu_alu_digest(pcagpu_t *h, int op, uint32_t b_lo, uint32_t b_
 extern "C" int pcagpu_get_stats(pcagpu_t *h, pcagpu_stats_t *st)
 {
     if (!h || !st) return PCAGPU_EINVAL;
+    const int rc = settle_scan(h);   // a launch in flight counts once collected
+    if (rc) return rc;
     *st = h->stats;
     return 0;
 }
 
 extern "C" void pcagpu_reset_stats(pcagpu_t *h)
 {
-    if (h) std::memset(&h->stats, 0, sizeof(h->stats));
+    if (!h) return;
+    (void)settle_scan(h);
+    std::memset(&h->stats, 0, sizeof(h->stats));
 }
 
 // ------------------------------------------------------------ wavefront
@@ -1262,6 +1352,8 @@ extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, lo
     if (!h) return PCAGPU_EINVAL;
     if (D < 0 || D > PCAGPU_MAX_KDEPTH || s < 0 || s > D) return PCAGPU_EINVAL;
     (void)hipSetDevice(h->device);
+    const int rc0 = settle_scan(h);
+    if (rc0) return rc0;
     int s_done = 0, cur = 0;
     const long nl = gpu_frontier(h, D, s, 0, &s_done, &cur);
     if (nl < 0) return nl;
@@ -1279,6 +1371,9 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
 {
     if (!h || !o || !res) return PCAGPU_EINVAL;
     std::memset(res, 0, sizeof(*res));
+    (void)hipSetDevice(h->device);
+    const int rc0 = settle_scan(h);
+    if (rc0) return rc0;
     res->optimal_depth = -1;
     res->depths_done = -1;
     const double t0 = now_s();
@@ -1295,7 +1390,10 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
             // the solutions are the same)
             (void)hipSetDevice(h->device);
             int cur = 0, s_done = s;
+            const uint64_t fn0 = h->stats.frontier_nodes, fk0 = h->stats.frontier_kept;
             const long n = gpu_frontier(h, D, s, o->split < 0 ? kAutoChildren : 0, &s_done, &cur);
+            res->prefix_nodes[D] = h->stats.frontier_nodes - fn0;
+            res->prefix_pruned[D] = res->prefix_nodes[D] - (h->stats.frontier_kept - fk0);
             if (n < 0) return (int)n;
             s = s_done;
             std::vector<pca_prefix_t> f((size_t)n);
@@ -1324,6 +1422,8 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
                          },
                          &ds);
             if (err) return err;
+            res->prefix_nodes[D] = ds.nodes;
+            res->prefix_pruned[D] = ds.pruned;
         }
         long nsol = pcagpu_end_depth(h, out, cap);
         if (nsol < 0) return (int)nsol;

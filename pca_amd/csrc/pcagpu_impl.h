@@ -79,7 +79,22 @@ struct pcagpu {
     std::string devname;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev_up = nullptr;   // the pending scan launch's uploads are done
     pcagpu_opts_t opts;
+    // The scan launch in flight (queued by a flush, not yet collected): its
+    // batch stays in d_batch / d_blocks until it is collected, so an
+    // overflowing launch can be redone; up_waited: h_batch and h_blocks may
+    // be rewritten (the launch's uploads have finished).
+    struct {
+        bool on = false;
+        uint32_t n = 0, n_blocks = 0;
+        uint64_t units = 0;
+        ScanTile tile{};
+        int cand = -1;
+        int k = 0;
+        uint64_t batch_first = 0;
+    } pend;
+    bool up_waited = true;
     uint32_t waves_per_cu = 0;           // opts.waves_per_cu, or 0: from the kernel's occupancy
     // resident scan waves per CU by [multi-item][state levels], 0 = not yet asked
     uint32_t scan_wpc[2][pca::MAXLEV + 1] = {};
@@ -144,6 +159,11 @@ struct pcagpu {
 
 
 namespace pca {
+// Collects the scan launch in flight, if any: waits for it, accounts its
+// counters and hits (redoing it with a larger hit buffer on overflow), and
+// feeds the tile tuner and the test order.  Every entry point that reads
+// results or uses the handle's stream for other work calls it first.
+int settle_scan(pcagpu *h);
 int fill_devtask(pcagpu *h);
 // device test order: h->dt's per-test fields and h->order from `order`
 void set_test_order(pcagpu *h, const std::vector<int> &order);

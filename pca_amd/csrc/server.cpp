@@ -659,7 +659,9 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             pcagpu *h = in_launch[i]->h;
             const uint32_t hits = s->hit_count_host(i);
             if (hits > h->hit_cap) {
-                // the rescan with a larger buffer failed: this request fails
+                // overflow: this batch is rescanned alone (the flush grows the
+                // hit buffer until it fits); if that fails, the request fails
+                s->st.rescans++;
                 if (pcagpu_flush(h) < 0) fail_request(s, *in_launch[i]);
                 continue;
             }
