@@ -192,8 +192,15 @@ def test_leaf_scan_raw_hits_match_oracle(pca, oracle, name, D):
     t = pca.Task.load(golden_task_path(name))
     ot = oracle.task_from_bytes(bytes(t))
     front = oracle.frontier(ot, D, D)
+    # programs with hits (the first visitors among the oracle's last-level
+    # rows) and random others
+    par = oracle.frontier(ot, D, D - 1)
+    lrows, _ = oracle.last_level(ot, D, prefixes=par)
+    owners = {tuple(par[p][0]) + (oracle.gen_candidates(par[p][1], D - 1, D)[j],)
+              for p, j, _, _, first in lrows if first}
+    own = [i for i, (ins, _) in enumerate(front) if tuple(ins) in owners]
     rng = random.Random(9)
-    idxs = sorted(rng.sample(range(len(front)), min(300, len(front))))
+    idxs = sorted(set(own[:150]) | set(rng.sample(range(len(front)), min(150, len(front)))))
     sub = [front[i] for i in idxs]
     T = ot.num_tests
     tgt = [ot.tests[k].out[0] for k in range(T)]
