@@ -188,6 +188,30 @@ long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap);
  * pcagpu_end_depth.  Returns the total row count. */
 long pcagpu_raw_hits(pcagpu_t *h, uint16_t *rows, long cap);
 
+/* ---- allocation-free raw-hit scan ----
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the rows pcagpu_raw_hits returns after
+ * pcagpu_begin_depth(D, s) + pcagpu_submit(prefixes, n) + pcagpu_flush,
+ * computed without a handle and without allocating.  Every device buffer
+ * (grammar tables, task record, prefixes, scan blocks, hit records, launch
+ * counters) is one fixed array of the library, about 11 MiB per GPU, held
+ * until the process ends; the first call on a device (the current one)
+ * sets it up.  Runs on the current device's null stream and blocks until
+ * done.  At most PCAGPU_NOALLOC_MAX_PREFIXES prefixes, their scan blocks
+ * (k = D - s >= 2: one per 64 children of a node one level above the last;
+ * a 3-instruction suffix has up to about 2.3e5 per prefix) at most
+ * PCAGPU_NOALLOC_MAX_BLOCKS, and at most PCAGPU_NOALLOC_MAX_HITS raw hits:
+ * beyond them it returns PCAGPU_EOVERFLOW and writes nothing.  Writes up to
+ * cap rows (14 uint16 each, the pcagpu_raw_hits layout, prefix index into
+ * `prefixes`) and returns the row count.  Calls that overlap on one GPU
+ * race: they share its array. */
+#define PCAGPU_NOALLOC_MAX_PREFIXES 16384
+#define PCAGPU_NOALLOC_MAX_BLOCKS   (1 << 18)
+#define PCAGPU_NOALLOC_MAX_HITS     (1 << 18)
+long pcagpu_raw_hits_noalloc(const pca_task_t *task, int target_depth, int split_depth,
+                             const pca_prefix_t *prefixes, uint32_t n, uint16_t *rows, long cap);
+
 /* Exhaustive check of the 16-bit ALU (src/vm.c:79-136 semantics): the sum
  * over a in [0, 2^16), b in [b_lo, b_hi) of a 64-bit mix of (a, b, value,
  * flags, fault) for opcode op (ADD..ASR, CMP).  h == NULL computes it on

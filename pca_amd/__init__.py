@@ -15,8 +15,9 @@ library or a GPU is missing.
 """
 from ._lib import (PcaGpuError, Task, Prefix, Solution, Stats, lib, load_library,
                    disasm, gen_candidates, host_frontier)
-from .solver import Solver, SolveResult
+from .solver import Solver, SolveResult, raw_hits_noalloc
 from . import search_state
 
 __all__ = ["PcaGpuError", "Task", "Prefix", "Solution", "Stats", "lib", "load_library",
-           "disasm", "gen_candidates", "host_frontier", "Solver", "SolveResult", "search_state"]
+           "disasm", "gen_candidates", "host_frontier", "Solver", "SolveResult", "raw_hits_noalloc",
+           "search_state"]

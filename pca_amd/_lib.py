@@ -197,6 +197,8 @@ SIGNATURES = [
     ("pcagpu_open", C.c_int, [C.POINTER(Task), C.POINTER(Opts), C.POINTER(C.c_void_p)]),
     ("pcagpu_close", None, [C.c_void_p]),
     ("pcagpu_retask", C.c_int, [C.c_void_p, C.POINTER(Task)]),
+    ("pcagpu_raw_hits_noalloc", C.c_long, [C.POINTER(Task), C.c_int, C.c_int, C.POINTER(Prefix), C.c_uint32,
+                                           C.POINTER(C.c_uint16), C.c_long]),
     ("pcagpu_device_name", C.c_char_p, [C.c_void_p]),
     ("pcagpu_begin_depth", C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     ("pcagpu_submit", C.c_int, [C.c_void_p, C.POINTER(Prefix), C.c_uint32, C.c_uint64]),

@@ -16,7 +16,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpcagpu.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-SOURCES = ["kernels.hip", "value.hip", "pcagpu.cpp", "server.cpp", "complete.cpp", "host_core.cpp", "host_dfs.cpp"]
+SOURCES = ["kernels.hip", "value.hip", "noalloc.hip", "pcagpu.cpp", "server.cpp", "complete.cpp", "host_core.cpp", "host_dfs.cpp"]
 # the three drop-in command lines, one source built in three modes
 CLIS = {"gpu_hybrid": 0, "gpu_enumerate_oep": 1, "gpu_wavefront": 2}
 

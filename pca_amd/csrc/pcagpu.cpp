@@ -41,6 +41,24 @@ void set_test_order(pcagpu *h, const std::vector<int> &order)
     h->order = order;
 }
 
+void devtask_for(const pca_task_t *task, DevTask *dt)
+{
+    std::memset(dt, 0, sizeof(*dt));
+    dt->T = task->num_tests;
+    dt->n_in = task->num_inputs;
+    dt->max_cycles = (int)std::min<uint64_t>(task->max_cycles, 255);
+    TestState roots[PCAGPU_MAX_TESTS];
+    root_states(task, roots);
+    const std::vector<int> order = device_test_order(task);
+    for (int t = 0; t < task->num_tests; t++) {
+        dt->tgt[t] = task->tests[order[t]].out[0];
+        dt->tgtw[t] = dt->tgt[t];
+        for (int i = 0; i < 12; i++)
+            if (kLdiImms[i] == dt->tgt[t]) dt->tgtw[t] |= 1u << (16 + i);
+        dt->root[t] = roots[order[t]];
+    }
+}
+
 int fill_devtask(pcagpu *h)
 {
     std::memset(&h->dt, 0, sizeof(h->dt));
@@ -78,18 +96,10 @@ int adapt_test_order(pcagpu *h, const unsigned long long *ts)
     return 0;
 }
 
-const GrammarDev *shared_grammar_dev(int device, bool branchless)
+void grammar_image(bool branchless, GrammarDev *img)
 {
-    static std::mutex mu;
-    static std::map<int, const GrammarDev *> tables;
-    std::lock_guard<std::mutex> lock(mu);
-    const int key = 2 * device + (branchless ? 1 : 0);
-    auto it = tables.find(key);
-    if (it != tables.end()) return it->second;
     const CandTable &ct = branchless ? shared_grammar_branchless() : shared_grammar();
-    std::unique_ptr<GrammarDev> img(new (std::nothrow) GrammarDev);
-    if (!img) return nullptr;
-    std::memset(img.get(), 0, sizeof(GrammarDev));
+    std::memset(img, 0, sizeof(GrammarDev));
     for (int m = 0; m < 128; m++) {
         std::memcpy(img->lists[m], &ct.list[(size_t)m * CAND_LMAX], sizeof(img->lists[m]));
         img->count[m] = ct.n[m];
@@ -100,6 +110,19 @@ const GrammarDev *shared_grammar_dev(int device, bool branchless)
         std::memcpy(img->bits_init[m], &ct.bits_init[(size_t)m * NBK], sizeof(img->bits_init[m]));
         std::memcpy(img->bits_gidx[m], &ct.bits_gidx[(size_t)m * NBK * 64], sizeof(img->bits_gidx[m]));
     }
+}
+
+const GrammarDev *shared_grammar_dev(int device, bool branchless)
+{
+    static std::mutex mu;
+    static std::map<int, const GrammarDev *> tables;
+    std::lock_guard<std::mutex> lock(mu);
+    const int key = 2 * device + (branchless ? 1 : 0);
+    auto it = tables.find(key);
+    if (it != tables.end()) return it->second;
+    std::unique_ptr<GrammarDev> img(new (std::nothrow) GrammarDev);
+    if (!img) return nullptr;
+    grammar_image(branchless, img.get());
     GrammarDev *d = nullptr;
     if (hipMalloc((void **)&d, sizeof(GrammarDev)) != hipSuccess) return nullptr;
     if (hipMemcpy(d, img.get(), sizeof(GrammarDev), hipMemcpyHostToDevice) != hipSuccess) {
@@ -378,31 +401,36 @@ int settle_scan(pcagpu *h)
     return adapt_test_order(h, ts);
 }
 
-// Appends the scan blocks of staged prefix i (k >= 2) to h_blocks at *nb;
-// false, with *nb unchanged, if they do not fit.
-bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb, int bk)
+bool prefix_blocks(const CandTable &ct, int D, int s, uint32_t live, uint32_t i, ScanBlock *out, uint32_t cap,
+                   uint32_t *nb, int bk)
 {
-    const int mf_s = h->D - h->s - 1, mf_s1 = h->D - h->s - 2;
-    const uint32_t live = h->h_batch[i].live;
-    const int ns = h->ct.count(live, mf_s);
+    const int mf_s = D - s - 1, mf_s1 = D - s - 2;
+    const int ns = ct.count(live, mf_s);
     uint32_t b = *nb;
     auto push = [&](uint32_t js, int nc) {
         for (int j0 = 0; j0 < nc; j0 += bk) {
-            if (b >= h->block_cap) return false;
-            h->h_blocks[b++] = ScanBlock{i, (uint16_t)js, (uint16_t)j0};
+            if (b >= cap) return false;
+            out[b++] = ScanBlock{i, (uint16_t)js, (uint16_t)j0};
         }
         return true;
     };
-    if (h->k == 2) {
+    if (D - s == 2) {
         if (!push(0, ns)) return false;
     } else {
         for (int js = 0; js < ns; js++) {
-            const uint32_t c = h->ct.at(live, mf_s, js);
-            if (!push((uint32_t)js, h->ct.count(live_after(live, c), mf_s1))) return false;
+            const uint32_t c = ct.at(live, mf_s, js);
+            if (!push((uint32_t)js, ct.count(live_after(live, c), mf_s1))) return false;
         }
     }
     *nb = b;
     return true;
+}
+
+// Appends the scan blocks of staged prefix i (k >= 2) to h_blocks at *nb;
+// false, with *nb unchanged, if they do not fit.
+bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb, int bk)
+{
+    return prefix_blocks(h->ct, h->D, h->s, h->h_batch[i].live, i, h->h_blocks, h->block_cap, nb, bk);
 }
 
 // Block buffers (host and device) at full size on the first scan with

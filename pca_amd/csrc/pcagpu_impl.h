@@ -165,6 +165,15 @@ namespace pca {
 // results or uses the handle's stream for other work calls it first.
 int settle_scan(pcagpu *h);
 int fill_devtask(pcagpu *h);
+// the device task record of a task in its initial device test order
+void devtask_for(const pca_task_t *task, DevTask *dt);
+// the grammar tables as uploaded to the device (host image)
+void grammar_image(bool branchless, GrammarDev *img);
+// Appends the scan blocks (k = D - s >= 2; bk children each) of prefix i,
+// whose live mask is `live`, to out[*nb, cap); false, with *nb unchanged,
+// if they do not fit.
+bool prefix_blocks(const CandTable &ct, int D, int s, uint32_t live, uint32_t i, ScanBlock *out, uint32_t cap,
+                   uint32_t *nb, int bk);
 // device test order: h->dt's per-test fields and h->order from `order`
 void set_test_order(pcagpu *h, const std::vector<int> &order);
 // Folds one launch's per-position test counts (2 per position) into the

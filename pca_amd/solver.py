@@ -192,3 +192,30 @@ class Solver:
         if n < 0:
             raise PcaGpuError(n, "end_depth")
         return [_sol_tuple(out[i]) for i in range(min(n, cap))], int(n)
+
+
+def raw_hits_noalloc(task, target_depth, split, prefixes, cap=None):
+    """pcagpu_raw_hits_noalloc: the raw device hits below `prefixes`
+    [(insns, live)] at (target_depth, split), computed without a handle and
+    without allocating device memory, as (prefix index, flags, mask, j tuple)
+    rows in canonical order -- the rows Solver.raw_depth returns."""
+    n = len(prefixes)
+    arr = (Prefix * max(n, 1))()
+    for i, (ins, live) in enumerate(prefixes):
+        for q, v in enumerate(ins):
+            arr[i].insns[q] = v
+        arr[i].num_insns = split
+        arr[i].live = live
+    if cap is None:
+        cap = 1 << 18
+    rows = (C.c_uint16 * (14 * max(cap, 1)))()
+    m = lib().pcagpu_raw_hits_noalloc(C.byref(task), target_depth, split, arr, n, rows, cap)
+    if m < 0:
+        raise PcaGpuError(m, "pcagpu_raw_hits_noalloc")
+    k = target_depth - split
+    out = []
+    for i in range(min(m, cap)):
+        r = rows[i * 14:(i + 1) * 14]
+        out.append((r[0] | (r[1] << 16), r[4], r[5], tuple(r[6:6 + k])))
+    return out
+
