@@ -52,6 +52,141 @@ def cpu_baseline(task_path, seconds):
             "tried_per_s": tried / secs}
 
 
+# tasks with a published optimal program length (README.md:41-51,
+# PRESENTATION.md:75-80), in words incl. IN, OUT and HLT, and config 2's task
+OPTIMUM_TASKS = [("max", 3, 7), ("is_power2", 4, 7), ("saturating_add", 4, 8), ("is_power2_32", 4, None)]
+
+
+def optimum_bench(cpu_budget_s=20.0):
+    """Time to proven optimum: IDDFS on the GPU (wavefront producer,
+    automatic split) up to the published optimum, every depth below it proven
+    empty, the optimum's whole canonical set kept and re-verified on the CPU
+    VM, compared with the oracle's fixture (tests/golden/exact/).  Beside it,
+    the oracle's restatement of the reference CPU solver (32-bit table,
+    branch mode, one thread) run depth by depth on the same task within
+    cpu_budget_s."""
+    from pca_amd import Task, Solver
+    from oracle import oracle as O
+    out = {}
+    for name, maxd, published in OPTIMUM_TASKS:
+        path = os.path.join(ROOT, "tests", "golden", "tasks", name + ".json")
+        task = Task.load(path)
+        t0 = time.perf_counter()
+        with Solver(task) as sv:
+            t1 = time.perf_counter()
+            r = sv.solve(maxd, cap=1 << 14)
+            t2 = time.perf_counter()
+            r2 = sv.solve(maxd, cap=1 << 14)
+            t3 = time.perf_counter()
+        rec = {"tests": task.num_tests, "inputs": task.num_inputs, "optimal_depth": r.optimal_depth,
+               "program_words": task.num_inputs + r.optimal_depth + 2 if r.optimal_depth >= 0 else None,
+               "published_words": published, "solutions": r.num_solutions,
+               "cpu_verified": all(x[4] for x in r.solutions), "gpu_s": t2 - t1, "gpu_s_with_open": t2 - t0,
+               "gpu_s_second_solve": t3 - t2,
+               "per_depth_s": [round(x, 6) for x in r.depth_seconds[:r.optimal_depth + 1]],
+               "candidates": sum(r.candidates), "same_second_solve": r2.solutions == r.solutions}
+        fx = os.path.join(ROOT, "tests", "golden", "exact", f"{name}_d{maxd}.json")
+        if os.path.exists(fx):
+            with open(fx) as f:
+                want = json.load(f)
+            rec["matches_oracle_fixture"] = (want["optimal_depth"] == r.optimal_depth and
+                                             want["solutions"] == [x[0] for x in r.solutions])
+        # the CPU solver depth by depth within the budget
+        ot = O.load_task(path)
+        cpu = {"kind": "port", "cores": 1, "budget_s": cpu_budget_s, "per_depth_s": []}
+        left = cpu_budget_s
+        for d in range(maxd + 1):
+            c = O.solve(ot, d, O.OEP_REF32, max_solutions=100, start_depth=d, time_limit=left)
+            if c["timed_out"]:
+                cpu["unfinished_depth"] = d
+                break
+            cpu["per_depth_s"].append(round(c["seconds"], 4))
+            left -= c["seconds"]
+            if c["optimal_depth"] >= 0:
+                cpu["optimal_depth"] = c["optimal_depth"]
+                break
+        cpu["depths_done"] = len(cpu["per_depth_s"]) - 1
+        rec["cpu_oracle"] = cpu
+        out[name] = rec
+        print(f"optimum {name}: D*={r.optimal_depth} {r.num_solutions} programs in {t2 - t1:.3f} s", file=sys.stderr,
+              flush=True)
+    out["reference_published"] = ("hybrid.cu on an RTX 5070 Ti: max 7 insns 0.06 s, saturating_add 8 insns 1.2 s, "
+                                  "is_power2 7 insns 3.3 s end to end (PRESENTATION.md:75-80, README.md:59)")
+    return out
+
+
+def same_definition_bench(task, n_prefixes=200000, cpu_seconds=10.0):
+    """One candidate definition on both sides: the last position (k = 1) below
+    the first n depth-7 prefixes of the depth-8 tree (host DFS order), every
+    (prefix, instruction, live out-register) that passes the grammar filters.
+    GPU: the scan kernel's candidates over its device time and over the
+    flush's wall time.  CPU: the oracle's brute force of the same suffixes
+    (each candidate program run on the VM, stopping at its first failing
+    test, one thread), over a bounded prefix of the same list; the GPU count
+    of that prefix is checked equal."""
+    import pca_amd
+    from oracle import oracle as O
+    prefixes, _, _ = pca_amd.host_frontier(task, 8, 7, cap=n_prefixes)
+    ot = O.task_from_bytes(bytes(task))
+    t0 = time.perf_counter()
+    cpu_cand, n_cpu = 0, 0
+    while n_cpu < len(prefixes) and time.perf_counter() - t0 < cpu_seconds:
+        ins, live = prefixes[n_cpu]
+        _, _, c = O.suffix_hits(ot, 8, list(ins) + [0], 7, live, cap=1 << 16)
+        cpu_cand += c
+        n_cpu += 1
+    cpu_s = time.perf_counter() - t0
+    import ctypes as C
+    lib = pca_amd.lib()
+    arr = (pca_amd.Prefix * len(prefixes))()
+    for i, (ins, live) in enumerate(prefixes):
+        for q, v in enumerate(ins):
+            arr[i].insns[q] = v
+        arr[i].num_insns = 7
+        arr[i].live = live
+
+    def depth(sv, n):
+        # begin / submit / flush / end through the C ABI, the prefixes resident in host memory
+        assert lib.pcagpu_begin_depth(sv._h, 8, 7) == 0
+        assert lib.pcagpu_submit(sv._h, arr, n, 0) == 0
+        assert lib.pcagpu_flush(sv._h) >= 0
+        assert lib.pcagpu_end_depth(sv._h, None, 0) >= 0
+
+    res = {}
+    with pca_amd.Solver(task) as sv:
+        depth(sv, n_cpu)
+        res["gpu_candidates_of_cpu_sample"] = sv.stats()["candidates"]
+        depth(sv, len(prefixes))                       # warm
+        sv.reset_stats()
+        t1 = time.perf_counter()
+        depth(sv, len(prefixes))
+        wall = time.perf_counter() - t1
+        st = sv.stats()
+    res.update({"workload": f"is_power2_32, depth 8, last position below the first {len(prefixes)} depth-7 "
+                            "prefixes in DFS order",
+                "gpu_candidates": st["candidates"], "gpu_kernel_s": st["kernel_ns"] * 1e-9, "gpu_wall_s": wall,
+                "gpu_candidates_per_s_kernel": st["candidates"] / max(st["kernel_ns"] * 1e-9, 1e-12),
+                "gpu_candidates_per_s_wall": st["candidates"] / wall,
+                "cpu_prefixes": n_cpu, "cpu_candidates": cpu_cand, "cpu_s": cpu_s,
+                "cpu_candidates_per_s": cpu_cand / cpu_s, "cpu_cores": 1, "cpu_kind": "port (VM brute force)",
+                "counts_equal": res["gpu_candidates_of_cpu_sample"] == cpu_cand})
+    res["gpu_over_cpu_wall"] = res["gpu_candidates_per_s_wall"] / max(res["cpu_candidates_per_s"], 1e-12)
+    return res
+
+
+def host_producer_bench(task, cap=200000):
+    """The host prefix producer (branch-mode DFS with exact OEP keys,
+    incremental per-test state, one thread) at 32 tests: the first cap
+    split-5 prefixes of the depth-8 tree, nodes tried per second."""
+    import pca_amd
+    t0 = time.perf_counter()
+    pre, nodes, pruned = pca_amd.host_frontier(task, 8, 5, cap=cap)
+    secs = time.perf_counter() - t0
+    return {"workload": f"is_power2_32, depth 8, first {len(pre)} split-5 prefixes", "nodes": nodes,
+            "oep_pruned": pruned, "seconds": secs, "nodes_per_s": nodes / secs, "prefixes_per_s": len(pre) / secs,
+            "threads": 1}
+
+
 def serving_bench(rates=(10, 50, 200), n_req=200, seed=7):
     """Poisson arrivals of generated requests (depth budgets 4-8) at each
     rate, through the continuous-batching server and through the plain
@@ -74,6 +209,39 @@ def serving_bench(rates=(10, 50, 200), n_req=200, seed=7):
             "launches": st["launches"], "items_per_launch": st["items"] / max(st["launches"], 1),
             "same_results": same}
         print(f"serving rate {rate}: one-at-a-time done", file=sys.stderr, flush=True)
+    return out
+
+
+def serving_config4_bench(rates=(10, 50, 200), n_req=40, seed=1000):
+    """The serving stream with the config-2 task generator: each request is
+    the 32-test is_power2 task with its own seeded non-powers
+    (taskgen.is_power2_32(seed)), at a depth budget of 4 -- about half of
+    these prove an optimum of 4 instructions (72 programs), the others prove
+    that none of at most 4 exists.  Depth 5 of one request is 10^14 to
+    10^15 candidates (minutes of device time), so budgets above 4 cannot be
+    answered at these rates.  Poisson arrivals; the server against the plain
+    one-task-at-a-time path with a warm re-tasked handle, same schedule;
+    both serve one request first, off the clock."""
+    from pca_amd import taskgen, serving
+    out = {"requests_per_rate": n_req, "task": "taskgen.is_power2_32(seed), 32 tests", "depth_budget": 4}
+    for rate in rates:
+        tasks = [taskgen.is_power2_32(seed + rate * 1000 + i) for i in range(n_req)]
+        budgets = [4] * n_req
+        arr = serving.poisson_schedule(n_req, rate, seed=rate + 1)
+        # both paths first serve one request of the stream's kind off the
+        # clock (buffers grown, as in a long-running server)
+        warm = [(taskgen.is_power2_32(seed - 1), 4)]
+        recs, st = serving.poisson_run(tasks, budgets, arr, warm=warm, max_slots=16)
+        print(f"serving config 4, rate {rate}: server done", file=sys.stderr, flush=True)
+        seq = serving.sequential_run(tasks, budgets, arr, warm_tasks=warm)
+        span = max(r[1] for r in recs) - min(r[0] for r in recs)
+        out[f"{rate}_req_s"] = {
+            "server": serving.latency_stats(recs), "one_at_a_time": serving.latency_stats(seq),
+            "proven_optimum_4": sum(1 for r in recs if r[2] == 4), "proven_none_le_4": sum(1 for r in recs if r[2] < 0),
+            "server_candidates_per_s": st["candidates"] / max(span, 1e-9),
+            "launches": st["launches"], "items_per_launch": st["items"] / max(st["launches"], 1),
+            "same_results": all(a[2:] == b[2:] for a, b in zip(recs, seq))}
+        print(f"serving config 4, rate {rate}: one-at-a-time done", file=sys.stderr, flush=True)
     return out
 
 
@@ -187,8 +355,10 @@ def wavefront_bench(task, D=8, s=3):
 def completion_bench(reps=3):
     """Leaf completion (search_exhaustive_complete) on the GPU: the
     two-input fixture states with a budget of 3 instructions (the reference
-    loop's completion radius), per-call time against the reference
-    library's one-core time recorded with the fixtures."""
+    loop's completion radius), per-call time.  The reference library's
+    one-core times recorded with the fixtures (on the build host) are listed
+    for scale only; the ratio reported is against the oracle's port timed on
+    this host (cpu_baseline)."""
     import statistics
     from pca_amd import Task, Solver, search_state
     cdir = os.path.join(ROOT, "tests", "golden", "complete")
@@ -209,14 +379,11 @@ def completion_bench(reps=3):
         assert (found, ins) == (c["found"], c["insns"]), c["task"]
         ms = statistics.median(ts) * 1e3
         out.append({"task": c["task"], "path": c["path"], "max_depth": c["max_depth"], "found": found,
-                    "gpu_ms": ms, "gpu_children": explored, "reference_s": c["seconds"],
-                    "reference_explored": c["explored"], "speedup": c["seconds"] / (ms / 1e3)})
-    res = {"workload": "two-input tasks, start states 2 instructions deep, budget 3 (R = 3)",
-           "cases": out, "gpu_ms_total": sum(x["gpu_ms"] for x in out),
-           "reference_s_total": sum(x["reference_s"] for x in out),
-           "reference_timing": "reference search_exhaustive_complete, one core of the build host"}
-    res["speedup_total"] = res["reference_s_total"] / (res["gpu_ms_total"] / 1e3)
-    return res
+                    "gpu_ms": ms, "gpu_children": explored, "reference_s_build_host": c["seconds"],
+                    "reference_explored": c["explored"]})
+    return {"workload": "two-input tasks, start states 2 instructions deep, budget 3 (R = 3)",
+            "cases": out, "gpu_ms_total": sum(x["gpu_ms"] for x in out),
+            "reference_timing": "reference search_exhaustive_complete, one core of the build host (not this box)"}
 
 
 def completion_cpu_baseline():
@@ -300,6 +467,7 @@ def main():
     ap.add_argument("--suite-deep-budget", type=int, default=4,
                     help="a second suite field at this depth budget (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--no-optimum", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
@@ -355,12 +523,19 @@ def main():
     }
     # HBM bytes per launch from the committed rocprofv3 PMC passes of this
     # kernel and workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)
+    # (profiled on the same window: same depth, split, batch, warmup and steps)
     prof = os.path.join(ROOT, "profiles", "scan_kernel_is_power2_32_d8s5.json")
-    if os.path.exists(prof) and (D, s, B) == (8, 5, 16):
+    if os.path.exists(prof):
         with open(prof) as f:
-            line["roofline"]["traffic"] = json.load(f)["traffic_bytes_per_launch"]
+            pj = json.load(f)
+        w = pj.get("window", {})
+        if (w.get("depth"), w.get("split"), w.get("batch"), w.get("warmup"), w.get("steps")) == \
+                (D, s, B, args.warmup, args.steps):
+            line["roofline"]["traffic"] = pj["traffic_bytes_per_launch"]
+            line["roofline"]["traffic_source"] = "profiles/scan_kernel_is_power2_32_d8s5.json (same window)"
     if not args.no_serving:
         line["serving"] = serving_bench(n_req=args.serving_requests)
+        line["serving_config4"] = serving_config4_bench()
     if args.suite:
         line["suite"] = suite_bench(args.suite, args.suite_budget)
         if args.suite_deep_budget:
@@ -376,11 +551,25 @@ def main():
         print("leaf completion done", file=sys.stderr, flush=True)
     if not args.no_cpu:
         if not args.no_complete:
-            line["leaf_completion"]["cpu_baseline"] = completion_cpu_baseline()
+            lc = line["leaf_completion"]
+            cbc = completion_cpu_baseline()
+            lc["cpu_baseline"] = cbc
+            g = [x for x in lc["cases"] if x["task"] == cbc["task"] and x["path"] == cbc["path"]]
+            if g:
+                lc["same_box_speedup"] = cbc["seconds"] / (g[0]["gpu_ms"] * 1e-3)
+        if not args.no_optimum:
+            line["optimum"] = optimum_bench()
+        line["same_definition"] = same_definition_bench(task)
+        print("same-definition comparison done", file=sys.stderr, flush=True)
+        line["host_producer"] = host_producer_bench(task)
         cb = cpu_baseline(task_path, args.cpu_seconds)
         line["cpu_baseline"] = cb
         if cb:
+            # pre-OEP GPU candidates (every suffix below the prefixes) against
+            # the CPU solver's post-OEP leaf tries: not one definition; the
+            # same-definition figure is same_definition.gpu_over_cpu_wall
             line["speedup_vs_cpu"] = line["value"] / cb["value"]
+            line["speedup_vs_cpu_note"] = "pre-OEP GPU candidates vs post-OEP CPU leaf tries"
     print(json.dumps(line), flush=True)
     return 0
 

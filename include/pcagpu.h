@@ -262,6 +262,7 @@ typedef struct {
      * duplicates (GPU wavefront) */
     uint64_t prefix_nodes[PCAGPU_MAX_KDEPTH + 1];
     uint64_t prefix_pruned[PCAGPU_MAX_KDEPTH + 1];
+    double   depth_seconds[PCAGPU_MAX_KDEPTH + 1];  /* wall time of each depth */
 } pcagpu_result_t;
 
 /* Runs IDDFS on an open handle.  Writes up to cap solutions of the optimal

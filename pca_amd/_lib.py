@@ -158,7 +158,8 @@ class SolveRes(C.Structure):
                 ("candidates", C.c_uint64 * (MAX_KDEPTH + 1)),
                 ("prefixes", C.c_uint64 * (MAX_KDEPTH + 1)),
                 ("prefix_nodes", C.c_uint64 * (MAX_KDEPTH + 1)),
-                ("prefix_pruned", C.c_uint64 * (MAX_KDEPTH + 1))]
+                ("prefix_pruned", C.c_uint64 * (MAX_KDEPTH + 1)),
+                ("depth_seconds", C.c_double * (MAX_KDEPTH + 1))]
 
 
 class ServerOpts(C.Structure):

@@ -18,6 +18,7 @@
 #include <set>
 #include <new>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "host_dfs.h"
@@ -254,21 +255,21 @@ int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
     return 0;
 }
 
-// The same launch with plain stream calls.  upload = false relaunches the
-// kernel over the batch already on the device (the redo of a launch whose
-// hits overflowed).
+// The same launch with plain stream calls.  Uploads of the staged prefixes
+// and blocks are optional: prefixes already on the device (a device-resident
+// depth) are read in place, and the redo of a launch whose hits overflowed
+// reads what the first run uploaded.
 int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uint32_t n, uint32_t n_blocks,
-               bool upload)
+               bool up_prefixes, bool up_blocks)
 {
     LaunchOut *o = reinterpret_cast<LaunchOut *>(h->h_out);
-    if (upload) {
+    if (up_prefixes)
         HIP_TRY(hipMemcpyAsync(h->d_batch.p, h->h_batch, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
                                h->stream));
-        if (n_blocks)
-            HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
-                                   hipMemcpyHostToDevice, h->stream));
-        HIP_TRY(hipEventRecord(h->ev_up, h->stream));
-    }
+    if (up_blocks && n_blocks)
+        HIP_TRY(hipMemcpyAsync(h->d_blocks.p, h->h_blocks, (size_t)n_blocks * sizeof(ScanBlock),
+                               hipMemcpyHostToDevice, h->stream));
+    if (up_prefixes || up_blocks) HIP_TRY(hipEventRecord(h->ev_up, h->stream));
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_tstat.p, 0, 2 * MAXT * sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_counters.p, 0, 4 * sizeof(unsigned long long), h->stream));
@@ -284,11 +285,12 @@ int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
 }
 
 // The kernel arguments of a one-item scan launch over the device batch.
-int scan_args(pcagpu *h, uint32_t n, uint64_t units, ScanTile tile, int k, ScanItem *it, ScanLaunch *l)
+int scan_args(pcagpu *h, const pca_prefix_t *src, uint32_t n, uint64_t units, ScanTile tile, int k, ScanItem *it,
+              ScanLaunch *l)
 {
     std::memset(it, 0, sizeof(*it));
     it->task = h->d_task.p;
-    it->prefixes = h->d_batch.p;
+    it->prefixes = src;
     it->blocks = h->d_blocks.p;
     it->n_prefix = n;
     it->unit_begin = 0;
@@ -314,18 +316,24 @@ int scan_args(pcagpu *h, uint32_t n, uint64_t units, ScanTile tile, int k, ScanI
     return grid;
 }
 
-// Queues one scan launch over h_batch[0, n) and h_blocks[0, n_blocks)
-// (the previous one must have been collected).  Returns at once; the host
-// may refill the staging buffers after wait_uploads.
-int start_scan(pcagpu *h, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile, int cand)
+// Queues one scan launch over n prefixes and h_blocks[0, n_blocks) (the
+// previous one must have been collected): the staged h_batch (src ==
+// d_batch, uploaded by the launch) or prefixes already on the device at src.
+// Returns at once; the host may refill the staging buffers after
+// wait_uploads.
+int start_scan(pcagpu *h, const pca_prefix_t *src, uint32_t n, uint32_t n_blocks, uint64_t units, ScanTile tile,
+               int cand)
 {
     ScanItem it;
     ScanLaunch l;
-    const int grid = scan_args(h, n, units, tile, h->k, &it, &l);
-    int rc = h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks, true)
-                                 : graph_scan(h, l, it, grid, n, n_blocks);
+    const int grid = scan_args(h, src, n, units, tile, h->k, &it, &l);
+    const bool staged = src == h->d_batch.p;
+    int rc = !staged ? eager_scan(h, l, it, grid, n, n_blocks, false, true)
+             : h->opts.eager_flush ? eager_scan(h, l, it, grid, n, n_blocks, true, true)
+                                   : graph_scan(h, l, it, grid, n, n_blocks);
     if (rc) return rc;
     h->pend.on = true;
+    h->pend.src = src;
     h->pend.n = n;
     h->pend.n_blocks = n_blocks;
     h->pend.units = units;
@@ -371,8 +379,8 @@ int settle_scan(pcagpu *h)
         h->hit_cap = want;
         ScanItem it;
         ScanLaunch l;
-        const int grid = scan_args(h, p.n, p.units, p.tile, p.k, &it, &l);
-        if ((rc = eager_scan(h, l, it, grid, p.n, p.n_blocks, false))) return rc;
+        const int grid = scan_args(h, p.src, p.n, p.units, p.tile, p.k, &it, &l);
+        if ((rc = eager_scan(h, l, it, grid, p.n, p.n_blocks, false, false))) return rc;
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
     h->stats.scan_tile = (uint64_t)p.tile.block_kids | ((uint64_t)p.tile.mask_tests << 8);
@@ -426,12 +434,6 @@ bool prefix_blocks(const CandTable &ct, int D, int s, uint32_t live, uint32_t i,
     return true;
 }
 
-// Appends the scan blocks of staged prefix i (k >= 2) to h_blocks at *nb;
-// false, with *nb unchanged, if they do not fit.
-bool add_prefix_blocks(pcagpu *h, uint32_t i, uint32_t *nb, int bk)
-{
-    return prefix_blocks(h->ct, h->D, h->s, h->h_batch[i].live, i, h->h_blocks, h->block_cap, nb, bk);
-}
 
 // Block buffers (host and device) at full size on the first scan with
 // k >= 2; handles that only scan single suffix positions never need them.
@@ -454,30 +456,38 @@ int ensure_block_buffers(pcagpu *h)
 // first collects the one before it (they share the hit buffer and the
 // counters); the last one is left in flight.  For k >= 2 the blocks are
 // built while the previous launch still runs.
-int scan_batch(pcagpu *h, uint32_t n)
+// hostp: the batch's prefixes on the host (their live masks make the
+// blocks); src: the same on the device (d_batch: uploaded from h_batch by
+// the launch).  Hits are numbered from h->batch_first.
+int scan_batch(pcagpu *h, const pca_prefix_t *hostp, const pca_prefix_t *src, uint32_t n)
 {
     int cand, rc;
     ScanTile tile;
     if (h->k <= 1) {
         if ((rc = settle_scan(h))) return rc;
         tile = current_tile(h, &cand);
-        return start_scan(h, n, 0, n, tile, cand);
+        return start_scan(h, src, n, 0, n, tile, cand);
     }
     if ((rc = wait_uploads(h)) || (rc = ensure_block_buffers(h))) return rc;
     tile = current_tile(h, &cand);
     uint32_t nb = 0;
     for (uint32_t i = 0; i < n; i++) {
-        if (add_prefix_blocks(h, i, &nb, tile.block_kids)) continue;
+        if (prefix_blocks(h->ct, h->D, h->s, hostp[i].live, i, h->h_blocks, h->block_cap, &nb, tile.block_kids))
+            continue;
         if (nb == 0) return PCAGPU_EOVERFLOW;
-        if ((rc = settle_scan(h)) || (rc = start_scan(h, n, nb, nb, tile, cand)) || (rc = wait_uploads(h)))
+        if ((rc = settle_scan(h)) || (rc = start_scan(h, src, n, nb, nb, tile, cand)) || (rc = wait_uploads(h)))
             return rc;
+        if (h->deadline > 0 && now_s() > h->deadline) {
+            h->stopped = true;   // the time limit passed: the rest of the batch is not scanned
+            return 0;
+        }
         tile = current_tile(h, &cand);
         nb = 0;
         i--;
     }
     if (!nb) return 0;
     if ((rc = settle_scan(h))) return rc;
-    return start_scan(h, n, nb, nb, tile, cand);
+    return start_scan(h, src, n, nb, nb, tile, cand);
 }
 
 size_t handle_device_bytes(const pcagpu *h)
@@ -531,16 +541,23 @@ uint32_t prefixes_fitting(const pcagpu *h, const pca_prefix_t *p, uint32_t n, Sc
 
 int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units)
 {
+    return build_blocks_of(h, h->h_batch, h->batch_n, tile, n_blocks, n_units);
+}
+
+int build_blocks_of(pcagpu *h, const pca_prefix_t *hostp, uint32_t n, ScanTile tile, uint32_t *n_blocks,
+                    uint64_t *n_units)
+{
     *n_blocks = 0;
     if (h->k <= 1) {
-        *n_units = h->batch_n;
+        *n_units = n;
         return 0;
     }
     int rc;
     if ((rc = wait_uploads(h)) || (rc = ensure_block_buffers(h))) return rc;
     uint32_t nb = 0;
-    for (uint32_t i = 0; i < h->batch_n; i++)
-        if (!add_prefix_blocks(h, i, &nb, tile.block_kids)) return PCAGPU_EOVERFLOW;
+    for (uint32_t i = 0; i < n; i++)
+        if (!prefix_blocks(h->ct, h->D, h->s, hostp[i].live, i, h->h_blocks, h->block_cap, &nb, tile.block_kids))
+            return PCAGPU_EOVERFLOW;
     *n_blocks = nb;
     *n_units = nb;
     return 0;
@@ -702,6 +719,7 @@ void pcagpu_close(pcagpu_t *h)
     h->d_match.release();
     h->d_match_count.release();
     if (h->h_batch) (void)hipHostFree(h->h_batch);
+    if (h->h_front) (void)hipHostFree(h->h_front);
     if (h->g_exec) (void)hipGraphExecDestroy(h->g_exec);
     if (h->g_graph) (void)hipGraphDestroy(h->g_graph);
     if (h->h_out) (void)hipHostFree(h->h_out);
@@ -733,6 +751,9 @@ int pcagpu_begin_depth(pcagpu_t *h, int target_depth, int split_depth)
     h->prefixes.clear();
     h->ranks.clear();
     h->raw.clear();
+    h->dev_front = false;
+    h->d_src = nullptr;
+    h->n_front = 0;
     h->in_depth = true;
     return 0;
 }
@@ -743,8 +764,11 @@ int pcagpu_flush(pcagpu_t *h)
     (void)hipSetDevice(h->device);
     int rc;
     if (h->batch_n) {
+        // the staged batch, or a range of a device-resident depth
         const uint32_t n = h->batch_n;
-        if ((rc = scan_batch(h, n))) return rc;
+        const pca_prefix_t *hp = h->dev_front ? h->h_front + h->batch_first : h->h_batch;
+        const pca_prefix_t *src = h->dev_front ? h->d_src + h->batch_first : h->d_batch.p;
+        if ((rc = scan_batch(h, hp, src, n))) return rc;
         h->batch_first += n;
         h->batch_n = 0;
     }
@@ -754,7 +778,7 @@ int pcagpu_flush(pcagpu_t *h)
 
 int pcagpu_submit(pcagpu_t *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank)
 {
-    if (!h || !h->in_depth) return PCAGPU_ESTATE;
+    if (!h || !h->in_depth || h->dev_front) return PCAGPU_ESTATE;
     if (n && !p) return PCAGPU_EINVAL;
     for (uint32_t i = 0; i < n; i++) {
         if (p[i].num_insns != h->s) return PCAGPU_EINVAL;
@@ -771,7 +795,7 @@ int pcagpu_submit(pcagpu_t *h, const pca_prefix_t *p, uint32_t n, uint64_t first
             // collects it
             (void)hipSetDevice(h->device);
             const uint32_t nb = h->batch_n;
-            if ((rc = scan_batch(h, nb))) return rc;
+            if ((rc = scan_batch(h, h->h_batch, h->d_batch.p, nb))) return rc;
             h->batch_first += nb;
             h->batch_n = 0;
         }
@@ -817,7 +841,7 @@ Path parent_of(const Path &x)
 // kernel instructions of a node (positions 0 .. s+len-1) and its live mask
 int node_insns(const pcagpu *h, const Path &x, uint16_t *insns, uint32_t *live_out)
 {
-    const pca_prefix_t &p = h->prefixes[x.prefix];
+    const pca_prefix_t &p = h->pfx()[x.prefix];
     for (int q = 0; q < h->s; q++) insns[q] = p.insns[q];
     uint32_t live = p.live;
     for (int q = 0; q < x.len; q++) {
@@ -871,7 +895,7 @@ int for_group_chunks(pcagpu *h, uint32_t first, uint32_t n, int kk, F &&launch)
     uint32_t g = 0;
     uint64_t acc = 0;
     for (uint32_t i = 0; i < n; i++) {
-        const uint32_t live = h->prefixes[first + i].live;
+        const uint32_t live = h->pfx()[first + i].live;
         const int ns = h->ct.count(live, mf_s);
         if (kk == 2) {
             h->h_groups[g] = ScanGroup{i, 0, 0};
@@ -932,11 +956,14 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
     if (h->d_match.n == 0 && (rc = h->d_match.ensure(1 << 16))) return rc;
     HIP_TRY(hipMemcpy(h->d_q0.p, t0.data(), cap0 * 8, hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(h->d_qf.p, tf.data(), 2 * capf * 8, hipMemcpyHostToDevice));
-    const uint32_t total = (uint32_t)h->prefixes.size();
+    const uint32_t total = (uint32_t)h->n_pfx();
     for (uint32_t first = 0; first < total; first += h->batch_cap) {
         const uint32_t n = std::min<uint32_t>(h->batch_cap, total - first);
-        HIP_TRY(hipMemcpy(h->d_batch.p, h->prefixes.data() + first, (size_t)n * sizeof(pca_prefix_t),
-                          hipMemcpyHostToDevice));
+        // a device-resident depth is read in place
+        const pca_prefix_t *dp = h->dev_front ? h->d_src + first : h->d_batch.p;
+        if (!h->dev_front)
+            HIP_TRY(hipMemcpy(h->d_batch.p, h->prefixes.data() + first, (size_t)n * sizeof(pca_prefix_t),
+                              hipMemcpyHostToDevice));
         rc = for_group_chunks(h, first, n, kk, [&](uint32_t n_groups, uint64_t units) -> int {
             if (n_groups) {
                 HIP_TRY(hipMemcpy(h->d_groups.p, h->h_groups, (size_t)n_groups * sizeof(ScanGroup),
@@ -945,7 +972,7 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
             }
             for (;;) {
                 MatchArgs a;
-                a.prefixes = h->d_batch.p;
+                a.prefixes = dp;
                 a.n_prefix = n;
                 a.groups = h->d_groups.p;
                 a.group_off = h->d_goff.p;
@@ -996,9 +1023,54 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
 
 struct Node {
     Path path;
+    uint64_t skey;                   // (prefix, j0, j1) packed, the order of paths of length <= 2
     uint32_t live;                   // live mask after the full kernel
     uint8_t mask, dead;
 };
+
+// OEP keys (KL bytes each) of nodes[idx[a..b)] into keys; consecutive nodes
+// below one prefix reuse that prefix's replayed per-test states.
+void node_keys(const pcagpu *h, const std::vector<Node> &nodes, const std::vector<size_t> &idx, size_t a, size_t b,
+               uint8_t *keys, int KL)
+{
+    TestState base[MAXT], st[MAXT];
+    uint32_t cached = 0xFFFFFFFFu;
+    for (size_t q = a; q < b; q++) {
+        const Path &x = nodes[idx[q]].path;
+        uint16_t insns[PCAGPU_MAX_KDEPTH];
+        const int n = node_insns(h, x, insns, nullptr);
+        if (x.prefix != cached) {
+            for (int t = 0; t < h->T; t++) {
+                base[t] = h->dt.root[t];
+                for (int i = 0; i < h->s; i++) step_test(base[t], insns[i], (uint32_t)i);
+            }
+            cached = x.prefix;
+        }
+        for (int t = 0; t < h->T; t++) {
+            st[t] = base[t];
+            for (int i = h->s; i < n; i++) step_test(st[t], insns[i], (uint32_t)i);
+        }
+        oep_key_from_states(st, h->T, keys + (q - a) * (size_t)KL);
+    }
+}
+
+// f(lo, hi) over [0, n) in contiguous runs on up to 8 host threads (one
+// when n is small)
+template <typename F>
+void parallel_runs(size_t n, F &&f)
+{
+    const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nt = n < 8192 ? 1 : std::min<size_t>(std::min(hw, 8u), (n + 4095) / 4096);
+    if (nt <= 1) { f((size_t)0, n); return; }
+    std::vector<std::thread> th;
+    const size_t per = (n + nt - 1) / nt;
+    for (size_t t = 1; t < nt; t++) {
+        const size_t lo = t * per, hi = std::min(n, lo + per);
+        if (lo < hi) th.emplace_back([&f, lo, hi] { f(lo, hi); });
+    }
+    f((size_t)0, std::min(n, per));
+    for (auto &x : th) x.join();
+}
 
 }  // namespace
 
@@ -1010,20 +1082,42 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
     h->in_depth = false;
     const int D = h->D, k = h->k, T = h->T;
     const int KL = T * 17 > 0 ? T * 17 : 1;
+    // Only a hit that can be output (no dead test, a live matching register)
+    // needs a verdict, and only a node of the same state can prune it --
+    // which has the same register-match mask.  Hits of other masks take no
+    // further part (a task can have hundreds of thousands of them).
+    bool mask_used[256] = {};
+    std::vector<uint32_t> live_of(h->raw.size());
+    for (size_t i = 0; i < h->raw.size(); i++) {
+        const HostHit &x = h->raw[i];
+        Path p;
+        p.prefix = (uint32_t)x.prefix;
+        p.len = (uint8_t)k;
+        for (int q = 0; q < k; q++) p.j[q] = x.j[q];
+        uint16_t insns[PCAGPU_MAX_KDEPTH];
+        node_insns(h, p, insns, &live_of[i]);
+        if (!(x.flags & 1) && (x.mask & live_of[i] & 0xFEu)) mask_used[x.mask] = true;
+    }
     std::vector<Node> nodes;
-    nodes.reserve(h->raw.size());
-    for (const HostHit &x : h->raw) {
+    for (size_t i = 0; i < h->raw.size(); i++) {
+        const HostHit &x = h->raw[i];
+        if (!mask_used[x.mask]) continue;
         Node nd;
         nd.path.prefix = (uint32_t)x.prefix;
         nd.path.len = (uint8_t)k;
         for (int q = 0; q < k; q++) nd.path.j[q] = x.j[q];
-        uint16_t insns[PCAGPU_MAX_KDEPTH];
-        node_insns(h, nd.path, insns, &nd.live);
+        nd.live = live_of[i];
         nd.mask = x.mask;
         nd.dead = x.flags & 1;
+        nd.skey = ((uint64_t)nd.path.prefix << 24) | ((uint64_t)(k >= 1 ? nd.path.j[0] : 0) << 12) |
+                  (uint64_t)(k >= 2 ? nd.path.j[1] : 0);
         nodes.push_back(nd);
     }
-    std::sort(nodes.begin(), nodes.end(), [](const Node &a, const Node &b) { return a.path < b.path; });
+    // canonical (DFS rank) order; candidate indices are below 4096
+    if (k <= 2)
+        std::sort(nodes.begin(), nodes.end(), [](const Node &a, const Node &b) { return a.skey < b.skey; });
+    else
+        std::sort(nodes.begin(), nodes.end(), [](const Node &a, const Node &b) { return a.path < b.path; });
     std::vector<TestState> st((size_t)(T > 0 ? T : 1));
     std::vector<uint8_t> key((size_t)KL);
 
@@ -1106,12 +1200,24 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
     // each state (every node of a target-matching state is a hit)
     std::vector<char> keep(nodes.size(), 0);
     {
-        ExactKeySet set(KL);
+        // the first-visitor test over the remaining nodes: keys computed on
+        // host threads in runs, inserted in rank order
+        std::vector<size_t> idx;
+        idx.reserve(nodes.size());
         for (size_t i = 0; i < nodes.size(); i++) {
             if (k >= 2 && !survive[parent_of(nodes[i].path)]) continue;
             if (k == 0) { keep[i] = 1; continue; }
-            node_key(h, nodes[i].path, key.data(), st.data());
-            keep[i] = (char)set.insert(key.data());
+            idx.push_back(i);
+        }
+        ExactKeySet set(KL);
+        const size_t chunk = (size_t)1 << 16;
+        std::vector<uint8_t> keys(std::min(idx.size(), chunk) * (size_t)KL + 1);
+        for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
+            const size_t c1 = std::min(idx.size(), c0 + chunk);
+            parallel_runs(c1 - c0, [&](size_t lo, size_t hi) {
+                node_keys(h, nodes, idx, c0 + lo, c0 + hi, keys.data() + lo * (size_t)KL, KL);
+            });
+            for (size_t q = c0; q < c1; q++) keep[idx[q]] = (char)set.insert(keys.data() + (q - c0) * (size_t)KL);
         }
     }
     long total = 0;
@@ -1125,7 +1231,7 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
             if (out && total < cap) {
                 pca_solution_t &so = out[total];
                 std::memset(&so, 0, sizeof(so));
-                so.prefix_rank = h->ranks[nodes[i].path.prefix];
+                so.prefix_rank = h->rank_of(nodes[i].path.prefix);
                 int n = 0;
                 for (int q = 0; q < h->task.num_inputs; q++)
                     so.prog[n++] = enc_i(OP_IN, (uint32_t)q + 1, h->task.input_ports[q]);
@@ -1393,6 +1499,32 @@ extern "C" long pcagpu_frontier(pcagpu_t *h, int D, int s, pca_prefix_t *out, lo
     return (long)n;
 }
 
+int pca::hold_device_frontier(pcagpu *h, int cur, size_t n)
+{
+    if (!h->in_depth) return PCAGPU_ESTATE;
+    if (n > h->front_cap) {
+        if (h->h_front) (void)hipHostFree(h->h_front);
+        h->h_front = nullptr;
+        h->front_cap = 0;
+        if (hipHostMalloc((void **)&h->h_front, n * sizeof(pca_prefix_t)) != hipSuccess) return PCAGPU_ENOMEM;
+        h->front_cap = n;
+    }
+    if (n) HIP_TRY(hipMemcpy(h->h_front, h->d_front[cur].p, n * sizeof(pca_prefix_t), hipMemcpyDeviceToHost));
+    h->dev_front = true;
+    h->d_src = h->d_front[cur].p;
+    h->n_front = n;
+    return 0;
+}
+
+void pca::release_wavefront_scratch(pcagpu *h, int cur)
+{
+    h->d_table.release();
+    h->d_child_hash.release();
+    h->d_parent_rows.release();
+    h->d_keep_bits.release();
+    h->d_front[cur ^ 1].release();
+}
+
 // ------------------------------------------------------------ IDDFS
 extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_result_t *res,
                             pca_solution_t *out, long cap)
@@ -1405,12 +1537,20 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
     res->optimal_depth = -1;
     res->depths_done = -1;
     const double t0 = now_s();
+    h->deadline = o->time_limit > 0 ? t0 + o->time_limit : 0;
+    h->stopped = false;
+    struct ClearDeadline {
+        pcagpu *h;
+        ~ClearDeadline() { h->deadline = 0; }
+    } clear_deadline{h};
     const int maxd = std::min(o->max_depth, PCAGPU_MAX_PROG - 2 - h->task.num_inputs);
     for (int D = std::max(o->start_depth, 0); D <= maxd && D <= PCAGPU_MAX_KDEPTH; D++) {
         int s = o->split < 0 ? (D > 0 ? D - 1 : 0) : std::min(o->split, D);
         int rc;
         const uint64_t cand0 = h->stats.candidates, pre0 = h->stats.prefixes;
         bool stopped = false;
+        const double td = now_s();
+        double t_front = td, t_copy = td;   // phase ends (verbose breakdown)
         if (o->producer == 0) {
             // GPU wavefront to the split; with the automatic split it stops at
             // the deepest level whose child lists stay within kAutoChildren
@@ -1424,14 +1564,16 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
             res->prefix_pruned[D] = res->prefix_nodes[D] - (h->stats.frontier_kept - fk0);
             if (n < 0) return (int)n;
             s = s_done;
-            std::vector<pca_prefix_t> f((size_t)n);
-            if (n) HIP_TRY(hipMemcpy(f.data(), h->d_front[cur].p, (size_t)n * sizeof(pca_prefix_t),
-                                     hipMemcpyDeviceToHost));
+            t_front = now_s();
             if ((rc = pcagpu_begin_depth(h, D, s))) return rc;
+            if ((rc = hold_device_frontier(h, cur, (size_t)n))) return rc;
+            t_copy = now_s();
             for (long i = 0; i < n; i += h->batch_cap) {
-                uint32_t c = (uint32_t)std::min<long>(h->batch_cap, n - i);
-                if ((rc = pcagpu_submit(h, f.data() + i, c, (uint64_t)i))) return rc;
-                if (o->time_limit > 0 && now_s() - t0 > o->time_limit) { stopped = true; break; }
+                const uint32_t c = (uint32_t)std::min<long>(h->batch_cap, n - i);
+                h->batch_first = (uint64_t)i;
+                h->stats.prefixes += c;
+                if ((rc = scan_batch(h, h->h_front + i, h->d_src + i, c))) return rc;
+                if (h->stopped || (o->time_limit > 0 && now_s() - t0 > o->time_limit)) { stopped = true; break; }
             }
         } else {
             if ((rc = pcagpu_begin_depth(h, D, s))) return rc;
@@ -1442,6 +1584,7 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
                          [&](const pca_prefix_t &p) {
                              int r = pcagpu_submit(h, &p, 1, rank++);
                              if (r) { err = r; return false; }
+                             if (h->stopped) { stopped = true; return false; }
                              if (o->time_limit > 0 && (rank & 4095) == 0 && now_s() - t0 > o->time_limit) {
                                  stopped = true;
                                  return false;
@@ -1453,14 +1596,20 @@ extern "C" int pcagpu_solve(pcagpu_t *h, const pcagpu_solve_opts_t *o, pcagpu_re
             res->prefix_nodes[D] = ds.nodes;
             res->prefix_pruned[D] = ds.pruned;
         }
+        const double t_sub = now_s();
         long nsol = pcagpu_end_depth(h, out, cap);
         if (nsol < 0) return (int)nsol;
         res->candidates[D] = h->stats.candidates - cand0;
         res->prefixes[D] = h->stats.prefixes - pre0;
-        if (o->verbose)
+        const double t_end = now_s();
+        res->depth_seconds[D] = t_end - td;
+        if (o->verbose) {
             std::printf("  depth %d (split %d): %llu prefixes, %llu candidates, %ld solutions, %.3f s\n", D, s,
                         (unsigned long long)res->prefixes[D], (unsigned long long)res->candidates[D], nsol,
-                        now_s() - t0);
+                        t_end - t0);
+            std::printf("    phases: frontier %.4f s, copy %.4f s, submit %.4f s, end of depth %.4f s\n",
+                        t_front - td, t_copy - t_front, t_sub - t_copy, t_end - t_sub);
+        }
         if (stopped) { res->timed_out = 1; break; }
         res->depths_done = D;
         if (nsol > 0) {

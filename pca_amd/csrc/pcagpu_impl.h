@@ -87,6 +87,7 @@ struct pcagpu {
     // be rewritten (the launch's uploads have finished).
     struct {
         bool on = false;
+        const pca_prefix_t *src = nullptr;   // the launch's prefixes (device)
         uint32_t n = 0, n_blocks = 0;
         uint64_t units = 0;
         ScanTile tile{};
@@ -95,6 +96,10 @@ struct pcagpu {
         uint64_t batch_first = 0;
     } pend;
     bool up_waited = true;
+    // pcagpu_solve's time limit (absolute, now_s(); 0: none): a flush stops
+    // queueing launches once it has passed and sets `stopped`
+    double deadline = 0;
+    bool stopped = false;
     uint32_t waves_per_cu = 0;           // opts.waves_per_cu, or 0: from the kernel's occupancy
     // resident scan waves per CU by [multi-item][state levels], 0 = not yet asked
     uint32_t scan_wpc[2][pca::MAXLEV + 1] = {};
@@ -147,6 +152,17 @@ struct pcagpu {
     uint64_t batch_first = 0;   // depth-prefix index of h_batch[0]
     std::vector<pca_prefix_t> prefixes;
     std::vector<uint64_t> ranks;
+    // A device-resident depth (the GPU producer): the scan reads the
+    // prefixes in place at d_src, h_front holds their host copy (pinned),
+    // and prefix i has rank i.  Otherwise the prefixes are the submitted
+    // ones in `prefixes` / `ranks`.
+    bool dev_front = false;
+    const pca_prefix_t *d_src = nullptr;
+    pca_prefix_t *h_front = nullptr;
+    size_t front_cap = 0, n_front = 0;
+    const pca_prefix_t *pfx() const { return dev_front ? h_front : prefixes.data(); }
+    size_t n_pfx() const { return dev_front ? n_front : prefixes.size(); }
+    uint64_t rank_of(size_t i) const { return dev_front ? (uint64_t)i : ranks[i]; }
     std::vector<HostHit> raw;
     pcagpu_stats_t stats;
     std::vector<int> order;          // device test order (index into task tests)
@@ -207,6 +223,16 @@ int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_
 // scan blocks of the staged batch into h_blocks (k >= 2); returns the block
 // count and the unit count, or PCAGPU_EOVERFLOW if they do not fit one launch
 int build_blocks(pcagpu *h, ScanTile tile, uint32_t *n_blocks, uint64_t *n_units);
+// the same for n prefixes whose host view is hostp (a range of a
+// device-resident depth)
+int build_blocks_of(pcagpu *h, const pca_prefix_t *hostp, uint32_t n, ScanTile tile, uint32_t *n_blocks,
+                    uint64_t *n_units);
+// Makes the open depth device-resident: its n prefixes are d_front[cur]
+// (read in place by the scans), copied once to the pinned host view h_front.
+int hold_device_frontier(pcagpu *h, int cur, size_t n);
+// Frees the wavefront's large buffers except the frontier d_front[cur]
+// (table, keys, parent rows, survivor bits, the other level).
+void release_wavefront_scratch(pcagpu *h, int cur);
 // how many of the n prefixes p (not yet staged) have scan blocks that fit one
 // launch's block buffer together (n when k <= 1)
 uint32_t prefixes_fitting(const pcagpu *h, const pca_prefix_t *p, uint32_t n, ScanTile tile);

@@ -77,9 +77,15 @@ struct Slot {
     uint64_t req = 0;
     int max_depth = 0;
     int D = 0;
-    std::vector<pca_prefix_t> frontier;
+    std::vector<pca_prefix_t> frontier;   // a host-produced depth's prefixes
     size_t next = 0;
     uint32_t staged = 0;     // prefixes of this step's launch
+    size_t byte_cap = 0;     // device bytes this slot may keep between depths
+    // the depth's prefixes: the host-produced list, or the handle's
+    // device-resident frontier (its pinned host view)
+    bool dev() const { return h->dev_front; }
+    size_t fsize() const { return h->dev_front ? h->n_front : frontier.size(); }
+    const pca_prefix_t *fdata() const { return h->dev_front ? h->h_front : frontier.data(); }
 };
 
 struct Request {
@@ -165,14 +171,17 @@ int start_depth(Slot &sl)
         }
         sl.frontier.resize((size_t)n);
     } else {
+        // the wavefront's last level stays on the device and is scanned in
+        // place; the slot then gives back the expansion's large buffers if
+        // it holds more than its share of the HBM budget
         int cur = 0, s_done = s;
         const long n = gpu_frontier(h, D, s, kAutoChildren, &s_done, &cur);
         if (n < 0) return (int)n;
-        s = s_done;
-        sl.frontier.resize((size_t)n);
-        if (n > 0)
-            HIP_TRY(hipMemcpy(sl.frontier.data(), h->d_front[cur].p, (size_t)n * sizeof(pca_prefix_t),
-                              hipMemcpyDeviceToHost));
+        int rc = pcagpu_begin_depth(h, D, s_done);
+        if (rc) return rc;
+        if ((rc = hold_device_frontier(h, cur, (size_t)n))) return rc;
+        if (sl.byte_cap && handle_device_bytes(h) > sl.byte_cap) release_wavefront_scratch(h, cur);
+        return 0;
     }
     return pcagpu_begin_depth(h, D, s);
 }
@@ -242,7 +251,7 @@ Outcome resolve_depths(Slot &sl)
             o.rc = rc;
             return o;
         }
-        if (!sl.frontier.empty()) return o;
+        if (sl.fsize()) return o;
         // an empty frontier closes the depth at once
     }
 }
@@ -279,6 +288,7 @@ int open_slot(pcagpu_server *s, const pca_task_t *task, Slot **out)
     if (rc < 0) return rc;
     s->slots.push_back(Slot());
     s->slots.back().h = h;
+    s->slots.back().byte_cap = (size_t)(s->opts.hbm_budget / std::max<uint32_t>(s->opts.max_slots, 1u));
     s->st.slots_opened++;
     *out = &s->slots.back();
     return 0;
@@ -335,7 +345,7 @@ int admit(pcagpu_server *s)
         s->st.admitted++;
         int rc = start_depth(sl);
         if (rc < 0) { fail_request(s, sl); continue; }
-        if (sl.frontier.empty() && (rc = advance(s, sl)) < 0) fail_request(s, sl);
+        if (!sl.fsize() && (rc = advance(s, sl)) < 0) fail_request(s, sl);
     }
     return 0;
 }
@@ -456,7 +466,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     uint32_t levels = 1;
     std::vector<Slot *> in_launch;
     uint32_t active = 0;
-    for (const Slot &sl : s->slots) active += sl.busy && sl.next < sl.frontier.size();
+    for (const Slot &sl : s->slots) active += sl.busy && sl.next < sl.fsize();
     const uint32_t share = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes / std::max(active, 1u));
     auto align16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
     // staging is host work per slot (tile, prefix fitting, the batch and its
@@ -466,13 +476,14 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         uint32_t n = 0, nb = 0;
         uint64_t nu = 0;
         ScanTile tile{};
+        const pca_prefix_t *dsrc = nullptr;   // prefixes already on the device
     };
     std::vector<Slot *> act;
     size_t left = 0;
     for (Slot &sl : s->slots)
-        if (sl.busy && sl.next < sl.frontier.size()) {
+        if (sl.busy && sl.next < sl.fsize()) {
             act.push_back(&sl);
-            left += sl.frontier.size() - sl.next;
+            left += sl.fsize() - sl.next;
         }
     std::vector<Staged> stg(act.size());
     // (threads only pay off with work to share: many slots or big batches)
@@ -482,13 +493,29 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         Staged &g = stg[i];
         int cand;
         g.tile = current_tile(h, &cand);   // no tuning inside shared launches
-        uint32_t n = (uint32_t)std::min<size_t>({sl.frontier.size() - sl.next, (size_t)h->batch_cap, (size_t)share});
-        n = std::max<uint32_t>(prefixes_fitting(h, sl.frontier.data() + sl.next, n, g.tile), 1);
-        if ((g.rc = stage_prefixes(h, sl.frontier.data() + sl.next, n, (uint64_t)sl.next)) < 0) return;
-        if ((g.rc = build_blocks(h, g.tile, &g.nb, &g.nu)) < 0) return;
+        const pca_prefix_t *hp = sl.fdata() + sl.next;
+        // a device-resident depth is read in place (nothing is staged but
+        // its blocks): with single-prefix units it goes in one launch, with
+        // blocks as many prefixes as one launch's blocks hold
+        const size_t want = !sl.dev() ? std::min<size_t>(share, h->batch_cap)
+                            : h->k <= 1 ? (size_t)0x7FFFFFFF
+                                        : std::max<size_t>(share, h->batch_cap);
+        uint32_t n = (uint32_t)std::min<size_t>(sl.fsize() - sl.next, want);
+        n = std::max<uint32_t>(prefixes_fitting(h, hp, n, g.tile), 1);
+        if (sl.dev()) {
+            h->batch_first = sl.next;
+            h->batch_n = n;
+            h->stats.prefixes += n;
+            g.dsrc = h->d_src + sl.next;
+            if ((g.rc = build_blocks_of(h, hp, n, g.tile, &g.nb, &g.nu)) < 0) return;
+        } else {
+            if ((g.rc = stage_prefixes(h, hp, n, (uint64_t)sl.next)) < 0) return;
+            if ((g.rc = build_blocks(h, g.tile, &g.nb, &g.nu)) < 0) return;
+        }
         g.n = n;
     }, act.size() >= 8 || left >= (1u << 16));
     std::vector<uint32_t> staged_blocks;
+    std::vector<const pca_prefix_t *> staged_src;
     size_t stage_bytes = 0;
     for (size_t i = 0; i < act.size(); i++) {
         Slot &sl = *act[i];
@@ -502,7 +529,9 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             continue;
         }
         staged_blocks.push_back(g.nb);
-        stage_bytes += align16((size_t)g.n * sizeof(pca_prefix_t)) + align16((size_t)g.nb * sizeof(ScanBlock));
+        staged_src.push_back(g.dsrc);
+        stage_bytes += (g.dsrc ? 0 : align16((size_t)g.n * sizeof(pca_prefix_t))) +
+                       align16((size_t)g.nb * sizeof(ScanBlock));
         levels = std::max(levels, scan_levels(h->k));
         ScanItem it;
         std::memset(&it, 0, sizeof(it));
@@ -536,16 +565,20 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         size_t off = 0;
         for (size_t i = 0; i < s->items.size(); i++) {
             const pcagpu *h = in_launch[i]->h;
-            const size_t pb = (size_t)s->items[i].n_prefix * sizeof(pca_prefix_t);
             const size_t bb = (size_t)staged_blocks[i] * sizeof(ScanBlock);
-            std::memcpy(s->h_stage.p + off, h->h_batch, pb);
-            s->items[i].prefixes = reinterpret_cast<const pca_prefix_t *>(s->d_stage.p + off);
-            off += align16(pb);
+            if (staged_src[i]) {
+                s->items[i].prefixes = staged_src[i];
+            } else {
+                const size_t pb = (size_t)s->items[i].n_prefix * sizeof(pca_prefix_t);
+                std::memcpy(s->h_stage.p + off, h->h_batch, pb);
+                s->items[i].prefixes = reinterpret_cast<const pca_prefix_t *>(s->d_stage.p + off);
+                off += align16(pb);
+            }
             if (bb) std::memcpy(s->h_stage.p + off, h->h_blocks, bb);
             s->items[i].blocks = reinterpret_cast<const ScanBlock *>(s->d_stage.p + off);
             off += align16(bb);
         }
-        HIP_TRY(hipMemcpyAsync(s->d_stage.p, s->h_stage.p, off, hipMemcpyHostToDevice, s->stream));
+        if (off) HIP_TRY(hipMemcpyAsync(s->d_stage.p, s->h_stage.p, off, hipMemcpyHostToDevice, s->stream));
     }
     // ---- the launches: an item whose units alone fill the chip several
     // times over runs in a launch of its own (the one-item kernel, its task
@@ -698,7 +731,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     // slot order
     std::vector<Slot *> fin;
     for (Slot &sl : s->slots)
-        if (sl.busy && sl.next >= sl.frontier.size()) fin.push_back(&sl);
+        if (sl.busy && sl.next >= sl.fsize()) fin.push_back(&sl);
     std::vector<Outcome> res(fin.size());
     run_parallel(s, fin.size(), [&](size_t i) { res[i] = resolve_depths(*fin[i]); });
     for (size_t i = 0; i < fin.size(); i++)

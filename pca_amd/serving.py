@@ -118,12 +118,17 @@ def _programs(sols):
     return [x[0] for x in sols]
 
 
-def poisson_run(tasks, budgets, arrivals, **server_kw):
+def poisson_run(tasks, budgets, arrivals, warm=(), **server_kw):
     """Replays arrivals (seconds) against a Server; returns per-request
     (arrival, done, optimal_depth, num_solutions, programs) -- programs: the
     canonical program list, the first SERVER_KEEP of them -- and the server
-    stats.  Programs are read back after the run, off the clock."""
+    stats.  Programs are read back after the run, off the clock.  `warm`:
+    (task, budget) requests served before the clock starts (a long-running
+    server's buffers already grown)."""
     with Server(**server_kw) as sv:
+        for t, b in warm:
+            sv.submit(t, b)
+        sv.drain()
         ids = {}
         recs = {}
         i = 0
@@ -148,14 +153,19 @@ def poisson_run(tasks, budgets, arrivals, **server_kw):
         return out, sv.stats()
 
 
-def sequential_run(tasks, budgets, arrivals, warm=True):
+def sequential_run(tasks, budgets, arrivals, warm=True, warm_tasks=()):
     """The same arrivals through the plain one-task-at-a-time GPU path
     (pcagpu_solve).  warm: one resident handle re-tasked per request, opened
     before the first arrival (allocation off the clock, as the server's
-    warm slots); otherwise a handle opened per request."""
+    warm slots), after solving warm_tasks [(task, budget)]; otherwise a
+    handle opened per request."""
     recs = []
     sv = Solver(tasks[0]) if warm and tasks else None
     try:
+        if sv is not None:
+            for t, b in warm_tasks:
+                sv.retask(t)
+                sv.solve(b, cap=1)
         t0 = time.perf_counter()
         for task, budget, arr in zip(tasks, budgets, arrivals):
             now = time.perf_counter() - t0

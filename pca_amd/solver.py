@@ -23,6 +23,7 @@ class SolveResult:
     seconds: float
     candidates: list = field(default_factory=list)
     prefixes: list = field(default_factory=list)
+    depth_seconds: list = field(default_factory=list)
 
 
 def _sol_tuple(s):
@@ -105,7 +106,8 @@ class Solver:
                            solutions=[_sol_tuple(out[i]) for i in range(n)],
                            num_solutions=int(res.num_solutions), depths_done=res.depths_done,
                            timed_out=bool(res.timed_out), seconds=res.seconds,
-                           candidates=list(res.candidates), prefixes=list(res.prefixes))
+                           candidates=list(res.candidates), prefixes=list(res.prefixes),
+                           depth_seconds=list(res.depth_seconds))
 
     # exhaustive leaf completion (src/search_state.c:418-450) -------------
     def exhaustive_complete(self, state, max_depth):

@@ -232,3 +232,24 @@ def _halts(oracle, ot, prog, k):
     st = oracle.replay_state(ot, prog[ot.num_inputs:-2])
     one.tests[0].out[0] = st[k][1]
     return oracle.evaluate(one, prog) == 1
+
+
+def test_server_deep_requests_match_oracle(pca):
+    """Deep requests through the continuous-batching server (device-resident
+    frontiers of millions of prefixes, scanned in place, next to small
+    requests): the same sets as the oracle's fixtures."""
+    from pca_amd.serving import Server, DONE
+    reqs = [("is_power2_32", 4), ("max", 3), ("is_power2", 4), ("gen_chain_016", 3), ("negate", 3)]
+    print("[server-deep]", reqs, flush=True)
+    with Server(max_slots=4) as sv:
+        ids = [sv.submit(pca.Task.load(golden_task_path(n)), d) for n, d in reqs]
+        recs = {r["id"]: r for r in sv.drain()}
+        for rid, (name, d) in zip(ids, reqs):
+            assert recs[rid]["status"] == DONE
+            sols, n = sv.solutions(rid)
+            if name == "negate":
+                assert recs[rid]["optimal_depth"] == 1 and n == 7
+                continue
+            want = _fixture(name, d)
+            assert (recs[rid]["optimal_depth"], n) == (want["optimal_depth"], want["num_solutions"])
+            assert [x[0] for x in sols] == want["solutions"][:4096]
