@@ -62,18 +62,39 @@ PCA_HD uint32_t flags_sub(uint32_t a, uint32_t b)
     return flags_zn(r) | (a >= b ? FLAG_C : 0u) | (((a ^ b) & (a ^ r) & 0x8000u) ? FLAG_V : 0u);
 }
 
+// The quotient x / y of C's int division (truncated toward zero) for x, y
+// in [-32768, 32767], y != 0.  On the device it comes from a float
+// reciprocal instead of the integer-division sequence: x and y are exact in
+// float, and the product with v_rcp_f32's estimate is within 2^-22 of x / y
+// relative, i.e. within 2^-7 / |y| absolute -- less than the distance from
+// any non-integer quotient (a multiple of 1 / |y|) to an integer, so
+// truncation is exact there; an integer quotient can come out one short in
+// magnitude, which the remainder (then of size |y|) shows and one step
+// fixes.  Checked over all 2^32 operand pairs against the CPU VM.
+PCA_HD int32_t sdiv16_q(int32_t x, int32_t y)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    int32_t q = (int32_t)((float)x * __builtin_amdgcn_rcpf((float)y));
+    const int32_t r = x - q * y;
+    if ((r < 0 ? -r : r) >= (y < 0 ? -y : y)) q += (x ^ y) < 0 ? -1 : 1;
+    return q;
+#else
+    return x / y;
+#endif
+}
+
 // C semantics of (int16_t)a / (int16_t)b and % after int promotion
 // (src/vm.c:94-103): truncation toward zero, -32768 / -1 = 32768 -> 0x8000,
 // -32768 % -1 = 0.  b != 0 is the caller's job.
 PCA_HD uint32_t sdiv16(uint32_t a, uint32_t b)
 {
-    int32_t x = (int16_t)(uint16_t)a, y = (int16_t)(uint16_t)b;
-    return (uint32_t)(x / y) & 0xFFFFu;
+    const int32_t x = (int16_t)(uint16_t)a, y = (int16_t)(uint16_t)b;
+    return (uint32_t)sdiv16_q(x, y) & 0xFFFFu;
 }
 PCA_HD uint32_t smod16(uint32_t a, uint32_t b)
 {
-    int32_t x = (int16_t)(uint16_t)a, y = (int16_t)(uint16_t)b;
-    return (uint32_t)(x % y) & 0xFFFFu;
+    const int32_t x = (int16_t)(uint16_t)a, y = (int16_t)(uint16_t)b;
+    return (uint32_t)(x - sdiv16_q(x, y) * y) & 0xFFFFu;
 }
 
 // BR condition codes (src/pca.h:62-70, src/vm.c:159-172).  Code 7 is never
@@ -186,11 +207,20 @@ PCA_HD uint32_t ps_reg(const PackedState &s, uint32_t k)
     return (k & 1u) ? v >> 16 : v & 0xFFFFu;
 }
 
+// (written as a chain on the word index, not a loop over the words: a loop
+// makes the compiler keep the state in a memory array to index it)
 PCA_HD void ps_set_reg(PackedState &s, uint32_t k, uint32_t v)
 {
     const uint32_t sh = (k & 1u) * 16u, keep = ~(0xFFFFu << sh), d = k >> 1;
-    for (uint32_t i = 0; i < 4; i++)
-        if (d == i) s.w[i] = (s.w[i] & keep) | ((v & 0xFFFFu) << sh);
+    const uint32_t nv = (v & 0xFFFFu) << sh;
+    if (d == 0)
+        s.w[0] = (s.w[0] & keep) | nv;
+    else if (d == 1)
+        s.w[1] = (s.w[1] & keep) | nv;
+    else if (d == 2)
+        s.w[2] = (s.w[2] & keep) | nv;
+    else
+        s.w[3] = (s.w[3] & keep) | nv;
 }
 
 PCA_HD void step_packed(PackedState &s, uint32_t insn, uint32_t pos)

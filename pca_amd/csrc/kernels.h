@@ -212,7 +212,7 @@ struct ScanLaunch {
 // copy it), and the node parameters.
 struct ScanNodeArgs {
     ScanLaunch l;
-    ScanItem it;
+    ScanArgs a1;
     DevTask task;
     void *ptrs[3];
 };

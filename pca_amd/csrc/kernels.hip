@@ -10,11 +10,17 @@
 // per-test registers live in LDS, one row per test), lanes = candidates of the
 // last kernel position, and each lane exits the test loop on its first
 // mismatch; the wave leaves the loop when a ballot finds no lane still alive.
+#include <cstring>
+
 #include "kernels.h"
 
 namespace pca {
 
 namespace {
+
+// two 16-bit lanes of one 32-bit register (packed 16-bit integer ops)
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+typedef short i16x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ void wave_sync()
 {
@@ -232,9 +238,17 @@ __device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, c
     return summarize_state(task, st, act, pos, lane, my_tgt, regmatch);
 }
 
-// jpath: the wave's LDS row of candidate indices at positions s..D-2.
+// The candidate indices of a scanned parent at positions s..D-2: a parent at
+// most two positions below the split keeps them in registers (j0, j1),
+// deeper ones are read from the wave's LDS row.
+struct JPath {
+    uint32_t j0, j1;
+    const uint32_t *lds;   // null: j0, j1
+};
+__device__ __forceinline__ uint32_t jp_at(const JPath &p, int q) { return p.lds ? p.lds[q] : (q == 0 ? p.j0 : p.j1); }
+
 // The record is written field by field (no private copy).
-__device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const uint32_t *jpath,
+__device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const JPath &jpath,
                                          uint32_t jlast, uint32_t mask, uint32_t flags)
 {
     const uint32_t slot = atomicAdd(a.hit_count, 1u);
@@ -245,7 +259,7 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
         h->flags = (uint8_t)flags;
         h->pad = 0;
         for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++)
-            h->j[q] = (uint16_t)(q + 1 < a.k ? jpath[q] : (q + 1 == a.k ? jlast : 0u));
+            h->j[q] = (uint16_t)(q + 1 < a.k ? jp_at(jpath, q) : (q + 1 == a.k ? jlast : 0u));
     }
 }
 
@@ -263,20 +277,21 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 // register already equals the target: that regmatch mask is ANDed into the
 // rd masks of all groups (rm_and) and prunes the groups left without an rd.
 // DIV and MOD keep a per-lane rd mask, since a zero divisor (a fault) makes
-// a test behave like that for that lane only.  `a.cand.bits_init` points at
-// the kernel's LDS copy of the per-mask group bits.
-__device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const TestState *P,
-                                                 uint32_t live, uint32_t prefix_idx, const uint32_t *jpath,
+// a test behave like that for that lane only.  The per-mask group bits are
+// read from the grammar table with scalar loads (a wave-uniform address).
+__device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const CandDev &cd,
+                                                 const TestState *P,
+                                                 uint32_t live, uint32_t prefix_idx, const JPath &jpath,
                                                  int lane, WaveCounters &wc, const uint8_t *regmatch,
                                                  uint32_t *queue, uint32_t (*tst)[2], const ParentSummary &ps)
 {
     const int T = task.T;
     const int m = (live >> 1) & 127;
     if (lane == 0) {
-        wc.cand += a.cand.cand_total[m];
-        wc.suf += (unsigned)a.cand.count[m];
+        wc.cand += cd.cand_total[m];
+        wc.suf += (unsigned)cd.count[m];
     }
-    const uint64_t *B = a.cand.bits_init + m * NBKINDS;
+    const uint64_t *__restrict__ B = &cd.g->bits_init[m][0];
     uint64_t mk[NBKINDS];
 #pragma unroll
     for (int q = 0; q < NBKINDS; q++) mk[q] = B[q];
@@ -337,7 +352,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
                 fault = fault || okd != 0u || okm != 0u;
             } else {
                 const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
-                const int32_t qt = sx / sy;
+                const int32_t qt = sdiv16_q(sx, sy);
                 if (((uint32_t)qt & 0xFFFFu) != tg) okd = 0;
                 if (((uint32_t)(sx - qt * sy) & 0xFFFFu) != tg) okm = 0;
             }
@@ -359,13 +374,13 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     // phase B: the survivors, one group per lane (queue entry: kind | bit << 4
     // | rd mask << 10 | fault << 18), tested to the end with an early exit
     const uint32_t fl_base = ps.dead ? 1u : 0u;
-    const uint16_t *GX = &a.cand.g->bits_gidx[m][0][0];
-    const uint8_t *GM = a.cand.g->vg_rdmask[m];
-    const uint16_t *GJ = &a.cand.g->vg_j[m][0][0];
+    const uint16_t *GX = &cd.g->bits_gidx[m][0][0];
+    const uint8_t *GM = cd.g->vg_rdmask[m];
+    const uint16_t *GJ = &cd.g->vg_j[m][0][0];
     const int t0 = t;
     // survivor-phase reach / end counts on one parent in eight are enough to
     // order the tests, at an eighth of their cost
-    const bool tst8 = tst && ((prefix_idx + (a.k >= 2 ? jpath[0] : 0u)) & 7u) == 0u;
+    const bool tst8 = tst && ((prefix_idx + (a.k >= 2 ? jp_at(jpath, 0) : 0u)) & 7u) == 0u;
     auto drain = [&](uint32_t cnt) {
         const bool valid = (uint32_t)lane < cnt;
         const uint32_t e = valid ? queue[lane] : 0u;
@@ -375,13 +390,58 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         const uint32_t xa = bit >> 3, yb = bit & 7u, i = bit & 7u;
         const uint32_t aimm = (i & 1u) ? (i >> 1) + 1u : 0x10000u - ((i >> 1) + 1u);
         const bool divmod = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
-        const bool any_div = __any(divmod && ok != 0u);
+        bool any_div = __any(divmod && ok != 0u);
         uint32_t n_ev = 0;
+        const uint32_t aimm2 = (aimm & 0xFFFFu) * 0x10001u;
         for (int tt = t0; tt < T; tt++) {
             const uint64_t al = __ballot(ok != 0u);
             if (al == 0ull) break;
             const uint32_t before = (uint32_t)__popcll(al);
             n_ev += before;
+            if (any_div) any_div = __any(divmod && ok != 0u);
+            if (!any_div && tt + 1 < T && ((ps.skipm >> tt) & 3ull) == 0ull) {
+                // tests tt and tt+1 at once: each 16-bit half of a register
+                // holds one test's operand, and the packed 16-bit ops compute
+                // both values (mod 2^16, as the VM does)
+                const uint32_t tw0 = task.tgtw[tt], tw1 = task.tgtw[tt + 1];
+                const uint32_t tg2 = (tw0 & 0xFFFFu) | (tw1 << 16);
+                const uint32_t x2 = (uint32_t)P[tt].r[xa] | ((uint32_t)P[tt + 1].r[xa] << 16);
+                const uint32_t y2 = (uint32_t)P[tt].r[yb] | ((uint32_t)P[tt + 1].r[yb] << 16);
+                const u16x2 X = __builtin_bit_cast(u16x2, x2), Y = __builtin_bit_cast(u16x2, y2);
+                const u16x2 S = Y & (unsigned short)15;
+                uint32_t v = __builtin_bit_cast(uint32_t, X + Y);
+                v = q == 1u ? __builtin_bit_cast(uint32_t, X - Y) : v;
+                v = q == 2u ? __builtin_bit_cast(uint32_t, X * Y) : v;
+                v = q == 5u ? (x2 & y2) : v;
+                v = q == 6u ? (x2 | y2) : v;
+                v = q == 7u ? (x2 ^ y2) : v;
+                v = q == 8u ? __builtin_bit_cast(uint32_t, X << S) : v;
+                v = q == 9u ? __builtin_bit_cast(uint32_t, X >> S) : v;
+                v = q == 10u ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(i16x2, x2) >> __builtin_bit_cast(i16x2, S))
+                             : v;
+                v = q == 11u ? __builtin_bit_cast(uint32_t, X + __builtin_bit_cast(u16x2, aimm2)) : v;
+                const uint32_t d = v ^ tg2;
+                bool m0 = (d & 0xFFFFu) == 0u, m1 = (d >> 16) == 0u;
+                if (q == 12u) {
+                    m0 = (tw0 >> (16 + bit)) & 1u;
+                    m1 = (tw1 >> (16 + bit)) & 1u;
+                }
+                // the second test is evaluated for the lanes the first kept
+                const uint32_t mid = (uint32_t)__popcll(__ballot(ok != 0u && m0));
+                n_ev += mid;
+                if (!(m0 && m1)) ok = 0;
+                if (tst8) {
+                    const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
+                    if (lane == 0) {
+                        tst[tt][0] += before;
+                        tst[tt][1] += before - mid;
+                        tst[tt + 1][0] += mid;
+                        tst[tt + 1][1] += mid - after;
+                    }
+                }
+                tt++;
+                continue;
+            }
             if ((ps.skipm >> tt) & 1ull) {
                 ok &= uni(regmatch[tt]);
                 continue;
@@ -409,7 +469,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
                     match = true;
                 } else {
                     const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
-                    const int32_t qt = sx / sy;
+                    const int32_t qt = sdiv16_q(sx, sy);
                     match = ((uint32_t)(q == OP_DIV - OP_ADD ? qt : sx - qt * sy) & 0xFFFFu) == tg;
                 }
             }
@@ -461,8 +521,9 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
 
 // Parent where some register already equals the target: every candidate
 // that leaves it alone is a hit, so go candidate by candidate.
-__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const TestState *P,
-                                            uint32_t live, uint32_t prefix_idx, const uint32_t *jpath, int lane,
+__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const CandDev &cd,
+                                            const TestState *P,
+                                            uint32_t live, uint32_t prefix_idx, const JPath &jpath, int lane,
                                             WaveCounters &wc, uint32_t *queue, const ParentSummary &ps)
 {
     const int T = task.T;
@@ -471,8 +532,8 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
     const uint32_t dead = ps.dead;
 
     const int m = (live >> 1) & 127;
-    const int n = a.cand.count[m];
-    const uint16_t *L = a.cand.g->lists[m];
+    const int n = cd.count[m];
+    const uint16_t *L = cd.g->lists[m];
     uint32_t n_cand = 0, n_suf = 0, n_tests = 0;
     uint32_t qn = 0;   // queued survivors (wave-uniform)
     const bool two = T >= 2;
@@ -635,10 +696,9 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
 // unit_begin.  The per-wave state levels live in dynamic LDS, lch.levels of
 // them per wave.
 template <bool MULTI>
-__device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem &one, const DevTask &task1)
+__device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs &a1, const DevTask &task1)
 {
     extern __shared__ TestState s_dyn[];
-    __shared__ uint64_t s_bits[128][NBKINDS];
     __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
     __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
     __shared__ uint32_t s_queue[SCAN_WAVES][128];
@@ -655,42 +715,32 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
         s_split[i] = lch.cand.split[i];
         s_ctot[i] = lch.cand.cand_total[i];
     }
-    for (int i = threadIdx.x; i < 128 * NBKINDS; i += SCAN_WG) (&s_bits[0][0])[i] = (&lch.cand.g->bits_init[0][0])[i];
     __syncthreads();
     CandDev cand = lch.cand;
     cand.count = s_cnt;
     cand.split = s_split;
     cand.cand_total = s_ctot;
-    cand.bits_init = &s_bits[0][0];
     const uint32_t nlev = lch.levels;
     auto lev = [&](int L) -> TestState * { return s_dyn + ((uint32_t)w * nlev + (uint32_t)L) * MAXT; };
 
     WaveCounters wc;
-    // current item (wave-uniform) and its context
+    // current item (wave-uniform) and its context: the one-item kernel reads
+    // its work item from the kernel arguments (a1), where the fields stay
+    // (reloaded through the scalar cache, not held in registers); the
+    // multi-item kernel copies the current item's fields into am
     uint64_t it_begin = 0, it_end = 0;
     const DevTask *taskp = nullptr;
-    ScanArgs a;
-    a.cand = cand;
-    a.counters = lch.counters;
-    a.work = lch.work;
+    ScanArgs am;
+    am.counters = lch.counters;
+    am.work = lch.work;
+    const ScanArgs &a = MULTI ? am : a1;
     uint32_t cur_i = 0xFFFFFFFFu, cur_js = 0xFFFFFFFFu;   // cached levels 0 / 1
     uint32_t my_tgt = 0;                                   // lane = test: its target
     if constexpr (!MULTI) {
         taskp = &task1;
         my_tgt = lane < task1.T ? task1.tgt[lane] : 0u;
-        a.prefixes = one.prefixes;
-        a.n_prefix = one.n_prefix;
-        a.blocks = one.blocks;
-        a.n_units = one.n_units;
-        a.D = one.D;
-        a.tile = one.tile;
-        a.s = one.s;
-        a.k = one.k;
-        a.hits = one.hits;
-        a.hit_cap = one.hit_cap;
-        a.hit_count = one.hit_count;
         it_begin = 0;
-        it_end = one.n_units;
+        it_end = a1.n_units;
     }
     const uint64_t grab = lch.grab > 1u ? lch.grab : 1u;
     uint64_t ug_next = 0, ug_end = 0;   // the wave's current run of units
@@ -713,17 +763,17 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
             }
             const ScanItem &it = lch.items[uni(lo)];
             taskp = it.task;
-            a.prefixes = it.prefixes;
-            a.n_prefix = it.n_prefix;
-            a.blocks = it.blocks;
-            a.n_units = it.n_units;
-            a.D = it.D;
-            a.tile = it.tile;
-            a.s = it.s;
-            a.k = it.k;
-            a.hits = it.hits;
-            a.hit_cap = it.hit_cap;
-            a.hit_count = it.hit_count;
+            am.prefixes = it.prefixes;
+            am.n_prefix = it.n_prefix;
+            am.blocks = it.blocks;
+            am.n_units = it.n_units;
+            am.D = it.D;
+            am.tile = it.tile;
+            am.s = it.s;
+            am.k = it.k;
+            am.hits = it.hits;
+            am.hit_cap = it.hit_cap;
+            am.hit_count = it.hit_count;
             it_begin = it.unit_begin;
             it_end = it.unit_begin + it.n_units;
             cur_i = 0xFFFFFFFFu;
@@ -760,7 +810,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
         const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
         uint32_t nb = 1, lvp = live0, cw = 0;
         if (k >= 3 && js != cur_js) {
-            const uint32_t c1 = uni(cand_at(a.cand, live0, a.D - a.s - 1, (int)js));
+            const uint32_t c1 = uni(cand_at(cand, live0, a.D - a.s - 1, (int)js));
             step_level(task, lev(0), lev(1), c1, a.s, lane);
             if (lane == 0) s_live[w][1] = live_after(live0, c1);
             cur_js = js;
@@ -772,10 +822,10 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                 if (lane == 0) s_idx[w][1] = js;
             }
             const int mfp = a.D - a.s - Lu;   // children at position s + Lu - 1
-            const int nc = cand_count(a.cand, lvp, mfp);
+            const int nc = cand_count(cand, lvp, mfp);
             nb = (int)j0 < nc ? (uint32_t)min(nc - (int)j0, (int)a.tile.block_kids) : 0u;
             // the block's candidate words, one per lane, in one coalesced read
-            cw = (uint32_t)lane < nb ? cand_at(a.cand, lvp, mfp, (int)j0 + lane) : 0u;
+            cw = (uint32_t)lane < nb ? cand_at(cand, lvp, mfp, (int)j0 + lane) : 0u;
         }
         for (uint32_t ci = 0; ci < nb; ci++) {
             // when the child is itself the scanned parent it is stepped and
@@ -803,7 +853,9 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
             int L = Lu;
             for (;;) {
                 if (L == k - 1) {
-                    const uint32_t *jpath = &s_idx[w][1];
+                    JPath jpath;
+                    jpath.lds = &s_idx[w][1];
+                    jpath.j0 = jpath.j1 = 0;
                     {
                         const TestState *PP = lev(L);
                         const uint32_t lvL = uni(s_live[w][L]);
@@ -812,9 +864,9 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                         wc.parents += lane == 0;
                         wave_sync();
                         if (ps.pm == 0u)
-                            scan_parent_bits(task, a, PP, lvL, pi, jpath, lane, wc, s_rm[w], s_queue[w], tst, ps);
+                            scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, s_rm[w], s_queue[w], tst, ps);
                         else
-                            scan_parent(task, a, PP, lvL, pi, jpath, lane, wc, s_queue[w], ps);
+                            scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, s_queue[w], ps);
                     }
                     wave_sync();
                     if (L == Lu) break;
@@ -826,7 +878,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                 const int pos = a.s + L;
                 const int mf = a.D - pos - 1;
                 const uint32_t lvL = uni(s_live[w][L]);
-                const int n = cand_count(a.cand, lvL, mf);
+                const int n = cand_count(cand, lvL, mf);
                 const int ix = (int)uni(s_idx[w][L + 1]);
                 if (ix >= n) {
                     if (L == Lu) break;
@@ -835,7 +887,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
                     wave_sync();
                     continue;
                 }
-                const uint32_t c = uni(cand_at(a.cand, lvL, mf, ix));
+                const uint32_t c = uni(cand_at(cand, lvL, mf, ix));
                 step_level(task, lev(L), lev(L + 1), c, pos, lane);
                 if (lane == 0) {
                     s_live[w][L + 1] = live_after(lvL, c);
@@ -871,14 +923,37 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanItem 
 // waves per SIMD (168 VGPRs, 21 spilled), measured on the server's 1,000-task
 // suite at depth budget 4 against 4 waves (128 VGPRs, 95 spilled) and 2
 // (188, none): 4.70 s of kernel time against 4.97 and 4.80.
-__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanItem one, DevTask task1)
+__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false>(lch, one, task1);
+    scan_body<false>(lch, a1, task1);
 }
 
-__global__ __launch_bounds__(SCAN_WG, 3) void scan_kernel_multi(ScanLaunch lch, ScanItem one, DevTask task1)
+__global__ __launch_bounds__(SCAN_WG, 3) void scan_kernel_multi(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<true>(lch, one, task1);
+    scan_body<true>(lch, a1, task1);
+}
+
+// The kernel argument of a one-item launch: its work item with the launch's
+// counters and grammar tables.
+ScanArgs one_item_args(const ScanLaunch &l, const ScanItem &it)
+{
+    ScanArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.prefixes = it.prefixes;
+    a.n_prefix = it.n_prefix;
+    a.blocks = it.blocks;
+    a.n_units = it.n_units;
+    a.D = it.D;
+    a.s = it.s;
+    a.k = it.k;
+    a.tile = it.tile;
+    a.cand = l.cand;
+    a.hits = it.hits;
+    a.hit_cap = it.hit_cap;
+    a.hit_count = it.hit_count;
+    a.counters = l.counters;
+    a.work = l.work;
+    return a;
 }
 
 // ------------------------------------------------------------ wavefront
@@ -1435,10 +1510,10 @@ void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &ta
                       hipKernelNodeParams *p)
 {
     args->l = l;
-    args->it = it;
+    args->a1 = one_item_args(l, it);
     args->task = task;
     args->ptrs[0] = &args->l;
-    args->ptrs[1] = &args->it;
+    args->ptrs[1] = &args->a1;
     args->ptrs[2] = &args->task;
     p->func = reinterpret_cast<void *>(scan_kernel_one);
     p->gridDim = dim3((unsigned)grid);
@@ -1478,10 +1553,11 @@ void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask 
                  hipStream_t st)
 {
     const size_t lds = scan_dyn_lds(l.levels);
+    const ScanArgs a1 = one_item_args(l, host_items[0]);
     if (l.n_items == 1)
-        hipLaunchKernelGGL(scan_kernel_one, dim3(grid), dim3(SCAN_WG), lds, st, l, host_items[0], task0);
+        hipLaunchKernelGGL(scan_kernel_one, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
     else
-        hipLaunchKernelGGL(scan_kernel_multi, dim3(grid), dim3(SCAN_WG), lds, st, l, host_items[0], task0);
+        hipLaunchKernelGGL(scan_kernel_multi, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
 }
 
 void launch_expand_count(const ExpandArgs &a, uint64_t *counts, hipStream_t st)
