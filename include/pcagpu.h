@@ -114,7 +114,11 @@ typedef struct {
     uint32_t eager_flush;     /* 1: plain stream calls per scan launch; 0: one
                                  replayed HIP graph (copies, clears, kernel,
                                  counter read-back)                          */
-    uint32_t reserved[1];
+    uint32_t test_flags;      /* testing only (0 in use): bit 0 gives the
+                                 wavefront's OEP table 64 slots, so it
+                                 overflows (PCAGPU_EOVERFLOW); bit 1 keeps 3
+                                 bits of its 64-bit keys, so distinct states
+                                 share keys (PCAGPU_ECOLLISION)              */
 } pcagpu_opts_t;
 
 typedef struct pcagpu pcagpu_t;

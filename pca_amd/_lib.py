@@ -144,7 +144,7 @@ class Stats(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("device", C.c_int), ("batch_capacity", C.c_uint32), ("hit_capacity", C.c_uint32),
                 ("waves_per_cu", C.c_uint32), ("fixed_test_order", C.c_uint32), ("scan_tile", C.c_uint32),
-                ("eager_flush", C.c_uint32), ("reserved", C.c_uint32 * 1)]
+                ("eager_flush", C.c_uint32), ("test_flags", C.c_uint32)]
 
 
 class SolveOpts(C.Structure):

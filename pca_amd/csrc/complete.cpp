@@ -117,9 +117,9 @@ int scan_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_
         launch_scan(l, &it, dt, grid, h->stream);
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventRecord(h->ev1, h->stream));
-        HIP_TRY(hipMemcpyAsync(cnt, h->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipMemcpyAsync(&hits, h->d_hit_count.p, sizeof(hits), hipMemcpyDeviceToHost, h->stream));
-        HIP_TRY(hipStreamSynchronize(h->stream));
+        HIP_TRY_S(h->stream, hipMemcpyAsync(cnt, h->d_counters.p, sizeof(cnt), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY_S(h->stream, hipMemcpyAsync(&hits, h->d_hit_count.p, sizeof(hits), hipMemcpyDeviceToHost, h->stream));
+        HIP_TRY_S(h->stream, hipStreamSynchronize(h->stream));
         float ms = 0.f;
         HIP_TRY(hipEventElapsedTime(&ms, h->ev0, h->ev1));
         h->stats.kernel_ns += (uint64_t)((double)ms * 1e6);

@@ -167,6 +167,9 @@ struct ExpandArgs {
     // per atomic on `work` (0 or 1: one)
     uint32_t slices;
     uint32_t grab;
+    // the children's keys are ANDed with this (all ones; fewer bits only to
+    // test that colliding keys are caught)
+    uint64_t key_mask;
 };
 
 void launch_alu_digest(uint32_t op, uint32_t b_lo, uint32_t b_hi, unsigned long long *out, hipStream_t st);

@@ -1068,7 +1068,7 @@ __global__ __launch_bounds__(SCAN_WG) void expand_insert_kernel(DevTask task, Ex
                 child_row(s_st[w][t], c, (uint32_t)a.pos, cs);
                 h = row_words_hash(h, cs);
             }
-            h |= 1ull;
+            h = (h & a.key_mask) | 1ull;
             const unsigned long long idx = base + (unsigned long long)j;
             a.child_hash[idx] = h;   // survive reads it back instead of recomputing
             uint64_t slot = (h >> 1) & a.table_mask;

@@ -152,14 +152,14 @@ extern "C" long pcagpu_raw_hits_noalloc(const pca_task_t *task, int target_depth
     unsigned char *ptr = d->ptr;
     unsigned long long *ctl = reinterpret_cast<unsigned long long *>(ptr + kOffCtl);
     const hipStream_t st = nullptr;
-    HIP_TRY(hipMemcpyAsync(ptr + kOffTask, &dt, sizeof(DevTask), hipMemcpyHostToDevice, st));
+    HIP_TRY_S(st, hipMemcpyAsync(ptr + kOffTask, &dt, sizeof(DevTask), hipMemcpyHostToDevice, st));
     if (n)
-        HIP_TRY(hipMemcpyAsync(ptr + kOffPrefixes, prefixes, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
+        HIP_TRY_S(st, hipMemcpyAsync(ptr + kOffPrefixes, prefixes, (size_t)n * sizeof(pca_prefix_t), hipMemcpyHostToDevice,
                                st));
     if (nb)
-        HIP_TRY(hipMemcpyAsync(ptr + kOffBlocks, g_blocks, (size_t)nb * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
+        HIP_TRY_S(st, hipMemcpyAsync(ptr + kOffBlocks, g_blocks, (size_t)nb * sizeof(ScanBlock), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(na_clear_kernel, dim3(1), dim3(64), 0, st, ctl, kCtlWords);
-    HIP_TRY(hipGetLastError());
+    HIP_TRY_S(st, hipGetLastError());
     uint32_t hits = 0;
     if (units) {
         ScanItem it;
@@ -189,13 +189,13 @@ extern "C" long pcagpu_raw_hits_noalloc(const pca_task_t *task, int target_depth
         const int grid = na_grid(d, units, l.levels);
         l.grab = scan_grab(k, units, grid);
         launch_scan(l, &it, dt, grid, st);
-        HIP_TRY(hipGetLastError());
-        HIP_TRY(hipMemcpyAsync(g_ctl, ctl, sizeof(g_ctl), hipMemcpyDeviceToHost, st));
+        HIP_TRY_S(st, hipGetLastError());
+        HIP_TRY_S(st, hipMemcpyAsync(g_ctl, ctl, sizeof(g_ctl), hipMemcpyDeviceToHost, st));
         HIP_TRY(hipStreamSynchronize(st));
         hits = (uint32_t)(g_ctl[5] & 0xFFFFFFFFull);
         if (hits > kNaHits) return PCAGPU_EOVERFLOW;
         if (hits) {
-            HIP_TRY(hipMemcpyAsync(g_hits, ptr + kOffHits, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost, st));
+            HIP_TRY_S(st, hipMemcpyAsync(g_hits, ptr + kOffHits, (size_t)hits * sizeof(HitRec), hipMemcpyDeviceToHost, st));
             HIP_TRY(hipStreamSynchronize(st));
         }
     } else {

@@ -995,8 +995,8 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
                 launch_match(h->dt, a, grid_for(h, units), h->stream);
                 HIP_TRY(hipGetLastError());
                 uint32_t cnt = 0;
-                HIP_TRY(hipMemcpyAsync(&cnt, h->d_match_count.p, 4, hipMemcpyDeviceToHost, h->stream));
-                HIP_TRY(hipStreamSynchronize(h->stream));
+                HIP_TRY_S(h->stream, hipMemcpyAsync(&cnt, h->d_match_count.p, 4, hipMemcpyDeviceToHost, h->stream));
+                HIP_TRY_S(h->stream, hipStreamSynchronize(h->stream));
                 if (cnt > a.cap) {
                     h->d_match.release();
                     int r2 = h->d_match.ensure((size_t)cnt + cnt / 2 + 1024);
@@ -1341,8 +1341,8 @@ extern "C" int pcagpu_alu_digest(pcagpu_t *h, int op, uint32_t b_lo, uint32_t b_
     launch_alu_digest((uint32_t)op, b_lo, b_hi, h->d_counters.p + 6, h->stream);
     HIP_TRY(hipGetLastError());
     unsigned long long v = 0;
-    HIP_TRY(hipMemcpyAsync(&v, h->d_counters.p + 6, sizeof(v), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY_S(h->stream, hipMemcpyAsync(&v, h->d_counters.p + 6, sizeof(v), hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY_S(h->stream, hipStreamSynchronize(h->stream));
     *out = v;
     return 0;
 }
@@ -1377,6 +1377,7 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     a.D = D;
     a.pos = pos;
     a.cand = cd;
+    a.key_mask = (h->opts.test_flags & 2u) ? 7ull : ~0ull;
     if ((rc = h->d_child_off.ensure(n + 1)) || (rc = h->d_keep_cnt.ensure(n + 1)) ||
         (rc = h->d_scan_tmp.ensure(exclusive_scan_tmp_words(n) + 16)))
         return rc;
@@ -1385,14 +1386,16 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     launch_expand_count(a, h->d_child_off.p, h->stream);
     launch_exclusive_scan(h->d_child_off.p, n, h->d_scan_tmp.p, h->stream);
     uint64_t children = 0;
-    HIP_TRY(hipMemcpyAsync(&children, h->d_child_off.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY_S(h->stream, hipMemcpyAsync(&children, h->d_child_off.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                        h->stream));
+    HIP_TRY_S(h->stream, hipStreamSynchronize(h->stream));
     if (max_children && children > max_children) {
         *kept_out = 0;
         return 1;   // not expanded: d_front[cur] is untouched
     }
     uint64_t tcap = 1024;
     while (tcap < 2 * children) tcap <<= 1;
+    if (h->opts.test_flags & 1u) tcap = 64;
     if ((rc = h->d_table.ensure(2 * tcap)) || (rc = h->d_keep_bits.ensure(children / 32 + 8)) ||
         (rc = h->d_child_hash.ensure(children ? children : 1)) || (rc = h->d_parent_rows.ensure(n * MAXT)))
         return rc;
@@ -1425,10 +1428,11 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     launch_exclusive_scan(h->d_keep_cnt.p, n, h->d_scan_tmp.p, h->stream);
     uint64_t kept = 0;
     unsigned long long ctr[2] = {0, 0};
-    HIP_TRY(hipMemcpyAsync(&kept, h->d_keep_cnt.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
-    HIP_TRY(hipMemcpyAsync(ctr, h->d_counters.p + 4, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           h->stream));
-    HIP_TRY(hipStreamSynchronize(h->stream));
+    HIP_TRY_S(h->stream, hipMemcpyAsync(&kept, h->d_keep_cnt.p + n, sizeof(uint64_t), hipMemcpyDeviceToHost,
+                                        h->stream));
+    HIP_TRY_S(h->stream, hipMemcpyAsync(ctr, h->d_counters.p + 4, 2 * sizeof(unsigned long long),
+                                        hipMemcpyDeviceToHost, h->stream));
+    HIP_TRY_S(h->stream, hipStreamSynchronize(h->stream));
     if (ctr[0] || ctr[1]) {
         h->stats.oep_collisions += ctr[0];
         return ctr[0] ? PCAGPU_ECOLLISION : PCAGPU_EOVERFLOW;

@@ -15,6 +15,18 @@
         if ((expr) != hipSuccess) return PCAGPU_EHIP;  \
     } while (0)
 
+// For the calls of a sequence that reads back into host memory of the
+// caller's frame (or a pageable buffer): on failure the stream is drained
+// before returning, so that no copy already queued can still write into
+// (or read from) memory that the return frees.
+#define HIP_TRY_S(st, expr)                            \
+    do {                                               \
+        if ((expr) != hipSuccess) {                    \
+            (void)hipStreamSynchronize(st);            \
+            return PCAGPU_EHIP;                        \
+        }                                              \
+    } while (0)
+
 namespace pca {
 
 struct HostHit {

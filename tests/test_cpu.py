@@ -93,6 +93,19 @@ def test_last_level_equals_iddfs(oracle, name, d):
     assert len(sols) == want["num_solutions"]
 
 
+@pytest.mark.parametrize("name,d,cycles", [("add", 2, 4), ("add", 1, 5), ("abs", 2, 4), ("sign", 2, 4),
+                                           ("max", 2, 5)])
+def test_last_level_with_binding_cycle_limit_equals_iddfs(oracle, name, d, cycles):
+    """The same with max_cycles at or below the program length, where the
+    evaluator's cycle limit (tasks/spec.c:188-215) decides: the VM paths of
+    both agree."""
+    t = oracle.load_task(golden_task_path(name))
+    t.max_cycles = cycles
+    sols, _, _ = oracle.solve_at_depth(t, d, threads=2)
+    want = oracle.solve(t, d, oracle.OEP_EXACT, max_solutions=0, start_depth=d)
+    assert sols == want["solutions"]
+
+
 @pytest.mark.parametrize("name,d", [("ident0", 2), ("is_power2", 2), ("gen_chain_040", 2)])
 def test_last_level_rows_match_vm_bruteforce(oracle, name, d):
     """The rows of or_last_level against a plain loop over the same children:
