@@ -284,6 +284,7 @@ def value_bench(batch=512, steps=30, seed=5):
     nvalid = np.full(batch, task.num_tests, dtype=np.uint8)
     depth = rng.integers(0, 8, size=batch).astype(np.int32)
     label = (rng.random(batch) < 0.3).astype(np.float32)
+    remaining = np.where(label > 0, rng.integers(1, 9, size=batch), -1).astype(np.int32)
     net = ValueNet(seed=seed)
     X, nv, dp = net.upload(rec, nvalid, depth)
 
@@ -298,7 +299,7 @@ def value_bench(batch=512, steps=30, seed=5):
 
     def train_step():
         net.forward(X, nv, dp)
-        net.backward(label)
+        net.backward(label, remaining)
         net.adam_step()
 
     t_train = timed(train_step, steps)

@@ -74,6 +74,13 @@ int pcav_colsum(int M, int N, const float *G, int ldg, float *out, void *stream)
 /* Binary cross-entropy with logits over B states: loss[0] = mean of
  * softplus(y) - label*y; dlogit[b] = (sigmoid(y_b) - label_b) / B. */
 int pcav_bce(int B, const float *logit, const float *label, float *dlogit, float *loss, void *stream);
+/* The reach + remaining-steps loss of python/train.py:44-60 on head outputs
+ * y [B x nh]: column 0 the reach logit (BCE against label, batch mean),
+ * columns 1..nh-1 logits over 0..nh-2 remaining steps (cross-entropy against
+ * remaining, over the n_pos rows with label > 0.5, their mean, weight w);
+ * dy [B x nh] gets the gradient, loss[0] the total. */
+int pcav_reach_dist_loss(int B, int nh, const float *y, const float *label, const int32_t *remaining, int n_pos,
+                         float w, float *dy, float *loss, void *stream);
 
 /* One Adam step on n FP32 master weights p (moments m, v; step >= 1, bias
  * corrected), then p16 = bf16(p) when p16 is not NULL. */

@@ -240,6 +240,8 @@ SIGNATURES = [
                                      C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pcav_colsum", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]),
     ("pcav_bce", C.c_int, [C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    ("pcav_reach_dist_loss", C.c_int, [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_float,
+                                        C.c_void_p, C.c_void_p, C.c_void_p]),
     ("pcav_adam", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_float,
                             C.c_float, C.c_float, C.c_float, C.c_int, C.c_void_p]),
     ("pcav_to_bf16", C.c_int, [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p]),

@@ -575,6 +575,48 @@ __global__ void bce_kernel(int B, const float *y, const float *label, float *dy,
     if (threadIdx.x == 0) loss[0] = part[0] / (float)B;
 }
 
+// The two-head loss of python/train.py:44-60: column 0 of each row of y
+// (B x nh) is the reach logit -- binary cross-entropy against label, mean
+// over the batch; columns 1..nh-1 are the remaining-steps logits --
+// cross-entropy against remaining (clamped to 0..nh-2) over the rows whose
+// label is positive, mean over those n_pos rows, times w.  dy gets the
+// gradient of the total, loss[0] the total.  One block, fixed reduction
+// order (deterministic).
+__global__ void reach_dist_loss_kernel(int B, int nh, const float *y, const float *label, const int32_t *remaining,
+                                       int n_pos, float w, float *dy, float *loss)
+{
+    __shared__ float part[256];
+    float s = 0.f;
+    for (int b = threadIdx.x; b < B; b += blockDim.x) {
+        const float *yb = y + (int64_t)b * nh;
+        float *db = dy + (int64_t)b * nh;
+        const float x = yb[0], t = label[b];
+        s += (fmaxf(x, 0.f) - t * x + log1pf(expf(-fabsf(x)))) / (float)B;
+        db[0] = (1.f / (1.f + expf(-x)) - t) / (float)B;
+        const bool pos = t > 0.5f && n_pos > 0 && w != 0.f;
+        if (pos) {
+            const int r = min(max(remaining[b], 0), nh - 2);
+            float m = yb[1];
+            for (int j = 2; j < nh; j++) m = fmaxf(m, yb[j]);
+            float se = 0.f;
+            for (int j = 1; j < nh; j++) se += expf(yb[j] - m);
+            const float lse = m + logf(se);
+            for (int j = 1; j < nh; j++)
+                db[j] = w * (expf(yb[j] - lse) - (j - 1 == r ? 1.f : 0.f)) / (float)n_pos;
+            s += w * (lse - yb[1 + r]) / (float)n_pos;
+        } else {
+            for (int j = 1; j < nh; j++) db[j] = 0.f;
+        }
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) loss[0] = part[0];
+}
+
 __global__ void adam_kernel(int64_t n, float *p, const float *g, float *m, float *v, uint16_t *p16, float lr,
                             float b1, float b2, float eps, float c1, float c2)
 {
@@ -744,6 +786,17 @@ int pcav_bce(int B, const float *logit, const float *label, float *dlogit, float
     if (B <= 0 || !logit || !label || !dlogit || !loss) return PCAGPU_EINVAL;
     clear_error();
     hipLaunchKernelGGL(bce_kernel, dim3(1), dim3(256), 0, st(stream), B, logit, label, dlogit, loss);
+    return status();
+}
+
+int pcav_reach_dist_loss(int B, int nh, const float *y, const float *label, const int32_t *remaining, int n_pos,
+                         float w, float *dy, float *loss, void *stream)
+{
+    if (B <= 0 || nh < 2 || !y || !label || !dy || !loss || (n_pos > 0 && !remaining) || n_pos < 0 || n_pos > B)
+        return PCAGPU_EINVAL;
+    clear_error();
+    hipLaunchKernelGGL(reach_dist_loss_kernel, dim3(1), dim3(256), 0, st(stream), B, nh, y, label, remaining, n_pos,
+                       w, dy, loss);
     return status();
 }
 

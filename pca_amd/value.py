@@ -6,8 +6,12 @@ Network (per search state = a kernel prefix of a task):
   8 x 16 register bits, 4 flag bits: the reference's per-test layout,
   python/search_eval.py:206-251) -> 512 -> 256 (ReLU, shared weights);
   mean over the task's tests; concat a 16-wide embedding of the depth;
-  -> 256 (ReLU) -> 1 logit: "a solution is reachable from here within the
-  depth budget".  294,289 parameters.  Weights are FP32 masters with BF16
+  -> 256 (ReLU) -> 10 outputs: a reach logit ("a solution is reachable from
+  here within the depth budget") and logits over the remaining steps 0..8
+  to the nearest solution (distance-to-solution), the two heads of the
+  reference's model and loss (python/model.py:41-44, python/train.py:44-60:
+  BCE on reach + 0.5 x cross-entropy on the distance of the reachable
+  states).  296,594 parameters.  Weights are FP32 masters with BF16
   copies for the MFMA GEMMs, FP32 accumulation; activations are stored as
   BF16.  Device memory comes from the library's own allocator (one HIP
   runtime in the process) and every arithmetic step is a pcav_* kernel.
@@ -29,6 +33,9 @@ H1, H2, EMB, H3 = 512, 256, 16, 256
 MAX_T = 32
 MAX_DEPTH = 8
 REC_WORDS = 16
+NDIST = MAX_DEPTH + 1      # remaining steps 0..8
+NH = 1 + NDIST             # head outputs: reach logit, remaining-steps logits
+DIST_WEIGHT = 0.5          # python/train.py: loss_reach + 0.5 * loss_dist
 
 
 def _check(rc, what):
@@ -106,7 +113,7 @@ class ValueNet:
     # parameter blocks in one flat FP32 buffer: name, shape (GEMM weights are
     # stored K x N, row-major)
     LAYOUT = [("W1", (PER_TEST, H1)), ("b1", (H1,)), ("W2", (H1, H2)), ("b2", (H2,)),
-              ("E", (MAX_DEPTH + 1, EMB)), ("W3", (H2 + EMB, H3)), ("b3", (H3,)), ("W4", (H3, 1)), ("b4", (1,))]
+              ("E", (MAX_DEPTH + 1, EMB)), ("W3", (H2 + EMB, H3)), ("b3", (H3,)), ("W4", (H3, NH)), ("b4", (NH,))]
 
     def __init__(self, seed=0, device=0):
         self.device = device
@@ -183,7 +190,8 @@ class ValueNet:
         return X, nv, dp
 
     def forward(self, X, nv, dp):
-        """Logits [B] (FP32, device) of a batch; keeps the activations for
+        """Head outputs [B, NH] (FP32, device) of a batch: column 0 the reach
+        logit, 1.. the remaining-steps logits; keeps the activations for
         backward()."""
         B = nv.shape[0]
         R = B * MAX_T
@@ -192,7 +200,7 @@ class ValueNet:
         h2 = self._buf("h2", (R, H2), np.uint16)
         z = self._buf("z", (B, H2 + EMB), np.uint16)
         h3 = self._buf("h3", (B, H3), np.uint16)
-        y = self._buf("y", (B, 1), np.float32)
+        y = self._buf("y", (B, NH), np.float32)
         self._gemm(R, H1, PER_TEST, X, PER_TEST, 0, w["W1"], H1, 0, bias=self.views["b1"], relu=True, C16=h1,
                    ldc16=H1)
         self._gemm(R, H2, H1, h1, H1, 0, w["W2"], H2, 0, bias=self.views["b2"], relu=True, C16=h2, ldc16=H2)
@@ -200,30 +208,40 @@ class ValueNet:
                                        None), "pool_forward")
         self._gemm(B, H3, H2 + EMB, z, H2 + EMB, 0, w["W3"], H3, 0, bias=self.views["b3"], relu=True, C16=h3,
                    ldc16=H3)
-        self._gemm(B, 1, H3, h3, H3, 0, w["W4"], 1, 0, bias=self.views["b4"], C32=y, ldc=1)
+        self._gemm(B, NH, H3, h3, H3, 0, w["W4"], NH, 0, bias=self.views["b4"], C32=y, ldc=NH)
         self._act = (X, nv, dp, h1, h2, z, h3, y, B)
         return y
 
     # ------------------------------------------------------------ backward
-    def backward(self, label):
-        """BCE loss of the last forward() against labels (host array [B]);
-        fills the gradient buffer and returns the loss (host float)."""
+    def backward(self, label, remaining=None, dist_weight=DIST_WEIGHT):
+        """Loss of the last forward(): BCE of the reach logit against labels
+        (host array [B]) plus dist_weight x cross-entropy of the remaining
+        steps (host int array [B], read where the label is positive; None:
+        reach only); fills the gradient buffer and returns the loss (host
+        float)."""
         X, nv, dp, h1, h2, z, h3, y, B = self._act
         R = B * MAX_T
         w, gv = self.v16, self.gviews
+        label = np.asarray(label, dtype=np.float32)
         lab = self._buf("label", (B,), np.float32)
-        lab.upload(np.asarray(label, dtype=np.float32))
-        dy = self._buf("dy", (B, 1), np.float32)
+        lab.upload(label)
+        n_pos = 0
+        rem = self._buf("remaining", (B,), np.int32)
+        if remaining is not None and dist_weight:
+            rem.upload(np.asarray(remaining, dtype=np.int32))
+            n_pos = int((label > 0.5).sum())
+        dy = self._buf("dy", (B, NH), np.float32)
         loss = self._buf("loss", (1,), np.float32)
-        _check(lib().pcav_bce(B, y.ptr, lab.ptr, dy.ptr, loss.ptr, None), "bce")
-        dy16 = self._buf("dy16", (B, 1), np.uint16)
-        _check(lib().pcav_to_bf16(B, dy.ptr, dy16.ptr, None), "to_bf16")
+        _check(lib().pcav_reach_dist_loss(B, NH, y.ptr, lab.ptr, rem.ptr, n_pos, float(dist_weight), dy.ptr,
+                                          loss.ptr, None), "reach_dist_loss")
+        dy16 = self._buf("dy16", (B, NH), np.uint16)
+        _check(lib().pcav_to_bf16(B * NH, dy.ptr, dy16.ptr, None), "to_bf16")
         self.g.zero()
         # head: y = h3 W4 + b4
-        self._gemm(H3, 1, B, h3, H3, 1, dy16, 1, 0, C32=gv["W4"], ldc=1)
-        _check(lib().pcav_colsum(B, 1, dy.ptr, 1, gv["b4"].ptr, None), "colsum")
+        self._gemm(H3, NH, B, h3, H3, 1, dy16, NH, 0, C32=gv["W4"], ldc=NH)
+        _check(lib().pcav_colsum(B, NH, dy.ptr, NH, gv["b4"].ptr, None), "colsum")
         dh3 = self._buf("dh3", (B, H3), np.float32)
-        self._gemm(B, H3, 1, dy16, 1, 0, w["W4"], 1, 1, C32=dh3, ldc=H3)
+        self._gemm(B, H3, NH, dy16, NH, 0, w["W4"], NH, 1, C32=dh3, ldc=H3)
         dh3_16 = self._buf("dh3_16", (B, H3), np.uint16)
         _check(lib().pcav_relu_backward(B * H3, h3.ptr, dh3.ptr, dh3_16.ptr, None), "relu_bwd")
         # h3 = relu(z W3 + b3)
@@ -261,11 +279,21 @@ class ValueNet:
             return np.zeros(0, dtype=np.float32)
         rec, _ = state_records(task, progs)
         X, nv, dp = self.upload(rec, [task.num_tests] * len(progs), [depth] * len(progs))
-        return self.forward(X, nv, dp).numpy().reshape(-1)
+        return self.forward(X, nv, dp).numpy()[:, 0].copy()
 
-    def train(self, rec, nvalid, depth, label, steps, batch=256, lr=1e-3, seed=0, log=None):
-        """Adam on minibatches drawn from host arrays; returns the loss curve
-        (every 50 steps and the last)."""
+    def predict_remaining(self, task, progs):
+        """The remaining-steps head's choice (0..8) for kernel prefixes (each
+        embedded at its own length, as in training)."""
+        if not progs:
+            return np.zeros(0, dtype=np.int32)
+        rec, _ = state_records(task, progs)
+        X, nv, dp = self.upload(rec, [task.num_tests] * len(progs), [len(p) for p in progs])
+        return self.forward(X, nv, dp).numpy()[:, 1:].argmax(1).astype(np.int32)
+
+    def train(self, rec, nvalid, depth, label, steps, batch=256, lr=1e-3, seed=0, log=None, remaining=None):
+        """Adam on minibatches drawn from host arrays (remaining: the
+        distance labels, or None for the reach head alone); returns the loss
+        curve (every 50 steps and the last)."""
         rng = np.random.default_rng(seed)
         n = rec.shape[0]
         losses = []
@@ -273,7 +301,7 @@ class ValueNet:
             idx = rng.integers(0, n, size=min(batch, n))
             X, nv, dp = self.upload(rec[idx], nvalid[idx], depth[idx])
             self.forward(X, nv, dp)
-            loss = self.backward(label[idx])
+            loss = self.backward(label[idx], None if remaining is None else remaining[idx])
             self.adam_step(lr=lr)
             if it % 50 == 0 or it == steps - 1:
                 losses.append(loss)
@@ -300,20 +328,23 @@ def _live_after(live, c):
 def labelled_states(task, depth_opt, solutions, negatives_per_node=4, seed=0):
     """Training states from one exhaustively solved task.  Positives: every
     proper prefix of an optimal program (a solution is reachable in the
-    remaining steps).  Negatives: children of those prefixes that are not a
-    prefix of any optimal program in the set.  Returns (progs, depths,
-    labels)."""
+    remaining steps), with its distance to a solution, depth_opt minus its
+    length (the exhaustive solve proves no shorter completion exists: it
+    would be a shorter program).  Negatives: children of those prefixes that
+    are not a prefix of any optimal program in the set (distance -1, not
+    used).  Returns (progs, depths, labels, remaining)."""
     rng = np.random.default_rng(seed)
     live0 = sum(1 << (i + 1) for i in range(task.num_inputs))
     pos_prefixes = set()
     for prog in solutions:
         for d in range(depth_opt):
             pos_prefixes.add(tuple(prog[:d]))
-    progs, depths, labels = [], [], []
+    progs, depths, labels, remaining = [], [], [], []
     for p in sorted(pos_prefixes):
         progs.append(list(p))
         depths.append(len(p))
         labels.append(1)
+        remaining.append(depth_opt - len(p))
         live = live0
         for c in p:
             live = _live_after(live, c)
@@ -323,7 +354,8 @@ def labelled_states(task, depth_opt, solutions, negatives_per_node=4, seed=0):
             progs.append(list(p) + [int(c)])
             depths.append(len(p) + 1)
             labels.append(0)
-    return progs, depths, labels
+            remaining.append(-1)
+    return progs, depths, labels, remaining
 
 
 # ---------------------------------------------------------------- search

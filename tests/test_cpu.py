@@ -382,7 +382,8 @@ def test_value_oracle_gradients():
     import numpy as np
     from oracle import value_oracle as vo
     rng = np.random.default_rng(3)
-    n = 180 * 512 + 512 + 512 * 256 + 256 + 9 * 16 + 272 * 256 + 256 + 256 + 1
+    n = 180 * 512 + 512 + 512 * 256 + 256 + 9 * 16 + 272 * 256 + 256 + 256 * 10 + 10
+    assert vo.n_params() == n
     p = rng.normal(0, 0.05, n)
     B = 3
     rec = rng.integers(0, 1 << 16, size=(B, 32, 16)).astype(np.uint16)
@@ -392,17 +393,36 @@ def test_value_oracle_gradients():
     nvalid = [32, 7, 19]
     depth = [0, 3, 8]
     label = [1.0, 0.0, 1.0]
-    _, _, g = vo.forward_backward(p, X, nvalid, depth, label, bf16_mode=False)
-    idx = np.concatenate([rng.integers(0, n, 150), [n - 1, n - 257, 180 * 512 + 3, n - 300]])
+    remaining = [4, -1, 0]
+    _, _, g = vo.forward_backward(p, X, nvalid, depth, label, remaining, bf16_mode=False)
+    # the head's weights and biases (reach and distance columns) and samples elsewhere
+    idx = np.concatenate([rng.integers(0, n, 150), n - 10 + np.arange(10), n - 10 - 256 * 10 + np.arange(0, 2560, 97),
+                          [180 * 512 + 3, n - 3000]])
     h = 1e-6
     for i in idx:
         pp, pm = p.copy(), p.copy()
         pp[i] += h
         pm[i] -= h
-        lp = vo.forward_backward(pp, X, nvalid, depth, label, bf16_mode=False)[1]
-        lm = vo.forward_backward(pm, X, nvalid, depth, label, bf16_mode=False)[1]
+        lp = vo.forward_backward(pp, X, nvalid, depth, label, remaining, bf16_mode=False)[1]
+        lm = vo.forward_backward(pm, X, nvalid, depth, label, remaining, bf16_mode=False)[1]
         fd = (lp - lm) / (2 * h)
         assert abs(fd - g[i]) <= 1e-6 + 1e-4 * abs(fd), (i, fd, g[i])
+    # closed forms with a zero head (every output 0): BCE of logit 0 is log 2
+    # whatever the label; the distance term is log 9 per positive state,
+    # weighted 0.5; the distance gradient of a positive state is
+    # 0.5 * (1/9 - [j == remaining]) / n_pos, and zero for a negative state
+    z = p.copy()
+    z[n - 10 - 2560:] = 0
+    y, loss, gz = vo.forward_backward(z, X, nvalid, depth, label, remaining, bf16_mode=False)
+    assert np.allclose(y, 0)
+    assert abs(loss - (np.log(2) + 0.5 * np.log(9))) < 1e-12
+    assert abs(vo.forward_backward(z, X, nvalid, depth, label, None, bf16_mode=False)[1] - np.log(2)) < 1e-12
+    gb = gz[n - 10:]
+    want_b = np.full(10, 2 * 0.5 / 9 / 2)
+    want_b[1 + 4] -= 0.5 / 2
+    want_b[1 + 0] -= 0.5 / 2
+    want_b[0] = ((0.5 - 1) + 0.5 + (0.5 - 1)) / 3
+    assert np.allclose(gb, want_b, atol=1e-12)
     # features: bit f of word f // 16, least significant first, flags last
     assert X[0, 3] == (rec[0, 0, 0] >> 3) & 1
     assert X[1, 16 + 5] == (rec[0, 1, 1] >> 5) & 1

@@ -124,19 +124,27 @@ def test_features_match_oracle(pca):
 
 
 def test_forward_backward_match_oracle(pca):
-    """Logits, loss and every parameter gradient of one batch against the
-    oracle rounding at the same BF16 points (FP32 accumulation order is the
-    only difference)."""
-    from pca_amd.value import ValueNet
+    """Head outputs (reach logit and remaining-steps logits), the two-head
+    loss and every parameter gradient of one batch against the oracle
+    rounding at the same BF16 points (FP32 accumulation order is the only
+    difference); also with the distance term off."""
+    from pca_amd.value import ValueNet, NH
     rec, nvalid, depth = _records(pca)
     B = len(nvalid)
     label = (np.arange(B) % 3 == 0).astype(np.float32)
+    remaining = np.where(label > 0, np.arange(B) % 9, -1).astype(np.int32)
     net = ValueNet(seed=2)
     X, nv, dp = net.upload(rec, nvalid, depth)
-    y = net.forward(X, nv, dp).numpy().reshape(-1)
-    loss = net.backward(label)
+    y = net.forward(X, nv, dp).numpy()
+    assert y.shape == (B, NH)
+    loss_reach = net.backward(label)
+    _, loss_reach_ref, _ = vo.forward_backward(net.parameters(), vo.features(rec), nvalid, depth, label, None)
+    assert abs(loss_reach - loss_reach_ref) < 1e-2
+    loss = net.backward(label, remaining)
     p = net.parameters()
-    y_ref, loss_ref, g_ref = vo.forward_backward(p, vo.features(rec), nvalid, depth, label, bf16_mode=True)
+    y_ref, loss_ref, g_ref = vo.forward_backward(p, vo.features(rec), nvalid, depth, label, remaining,
+                                                 bf16_mode=True)
+    assert loss_ref > loss_reach_ref + 0.5
     print(f"[value] B={B} loss gpu {loss:.6f} oracle {loss_ref:.6f}", flush=True)
     assert np.allclose(y, y_ref, rtol=2e-2, atol=2e-2)
     assert abs(loss - loss_ref) < 1e-2
@@ -171,25 +179,27 @@ def trained(pca):
     from pca_amd.value import ValueNet, labelled_states, state_records
     t0 = time.time()
     train = _solved(pca, 101, 1200, 3)
-    depths, labels, recs, nvalid = [], [], [], []
+    depths, labels, recs, nvalid, remaining = [], [], [], [], []
     for t, d, sols in train:
         # solutions come back as full programs; keep the kernel part
         kernels = [s[t.num_inputs:t.num_inputs + d] for s in sols]
-        p, dp, lb = labelled_states(t, d, kernels, negatives_per_node=24, seed=len(recs))
+        p, dp, lb, rm = labelled_states(t, d, kernels, negatives_per_node=24, seed=len(recs))
         r, _ = state_records(t, p)
         recs.append(r)
         nvalid += [t.num_tests] * len(p)
         depths += dp
         labels += lb
+        remaining += rm
     rec = np.concatenate(recs)
     nvalid = np.array(nvalid, dtype=np.uint8)
     depths = np.array(depths, dtype=np.int32)
     labels = np.array(labels, dtype=np.float32)
+    remaining = np.array(remaining, dtype=np.int32)
     print(f"[train] tasks {len(train)} states {len(labels)} positive {int(labels.sum())} "
           f"data {time.time() - t0:.1f}s", flush=True)
     net = ValueNet(seed=0)
     t0 = time.time()
-    losses = net.train(rec, nvalid, depths, labels, steps=1500, batch=512, lr=1e-3)
+    losses = net.train(rec, nvalid, depths, labels, steps=1500, batch=512, lr=1e-3, remaining=remaining)
     print(f"[train] loss {losses[0]:.3f} -> {losses[-1]:.3f} in {time.time() - t0:.1f}s", flush=True)
     return net, losses
 
@@ -263,3 +273,37 @@ def test_guided_search_with_gpu_leaf_completion(pca, trained):
     print(f"[completion] {time.time() - t0:.1f}s", flush=True)
     assert found["guided+R3"] > found["plain+R3"]
     assert found["guided+R3"] > found["guided"]
+
+
+def test_distance_head_predicts_remaining_steps(pca, trained):
+    """The remaining-steps head on held-out positives (proper prefixes of
+    optimal programs of held-out tasks like the training ones, with their
+    distance to a solution, optimum minus length, proven by the exhaustive
+    solver): its choice is never out of 0..8, and is right more often than
+    the most common distance and than the best guess from the prefix length
+    alone (the most common distance at each length, counted on the held-out
+    set itself), so the head reads the state, not only its length.  Most
+    held-out positives are one step from a solution, so both baselines are
+    high; measured: 0.905 against 0.870 and 0.889."""
+    from pca_amd.value import labelled_states
+    net, _ = trained
+    held = _solved(pca, 303, 120, 3)
+    right, total = 0, 0
+    by_len = np.zeros((9, 9), dtype=np.int64)       # [length, distance] counts
+    for t, d, sols in held:
+        kernels = [s[t.num_inputs:t.num_inputs + d] for s in sols]
+        p, dp, lb, rm = labelled_states(t, d, kernels, negatives_per_node=0)
+        pos = [(q, r) for q, l, r in zip(p, lb, rm) if l == 1]
+        pred = net.predict_remaining(t, [q for q, _ in pos])
+        assert pred.min() >= 0 and pred.max() <= 8
+        want = np.array([r for _, r in pos])
+        right += int((pred == want).sum())
+        total += len(pos)
+        for (q, r) in pos:
+            by_len[len(q), r] += 1
+    majority = by_len.sum(0).max() / total
+    length_only = by_len.max(1).sum() / total
+    print(f"[distance] held-out positives {total}: exact {right / total:.3f}, majority {majority:.3f}, "
+          f"best by length {length_only:.3f}", flush=True)
+    assert total >= 100
+    assert right / total > length_only >= majority
