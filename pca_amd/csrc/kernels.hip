@@ -106,8 +106,10 @@ __device__ __forceinline__ void step_level(const DevTask &task, const TestState 
     }
 }
 
+// The wave's work counters, a row in LDS that lane 0 adds to (no registers
+// held for them): [0] candidates [1] suffixes [2] tests [3] parents.
 struct WaveCounters {
-    unsigned long long cand = 0, suf = 0, tests = 0, parents = 0;
+    unsigned long long *c;
 };
 
 // Value a last-position candidate of opcode OP writes (flags are not needed
@@ -247,6 +249,31 @@ struct JPath {
 };
 __device__ __forceinline__ uint32_t jp_at(const JPath &p, int q) { return p.lds ? p.lds[q] : (q == 0 ? p.j0 : p.j1); }
 
+// Pooled survivor phase: parents whose surviving value groups share one
+// 64-lane drain.  Each wave keeps NSLOT parent states in LDS slots, with
+// what a drain needs of each parent.
+constexpr uint32_t NSLOT = 4;
+// Everything a wave keeps in LDS besides the state levels above the scanned
+// parents, in one block (one base address, fields at fixed offsets).
+struct WaveLds {
+    TestState slot[NSLOT][MAXT];          // the scanned parents' states
+    uint32_t queue[128];                  // pooled survivors
+    uint32_t cq[128];                     // scan_parent's candidates
+    uint32_t live[MAXLEV], idx[MAXLEV];   // the suffix walk: live masks, candidate indices
+    uint8_t srm[NSLOT][MAXT];             // per slot and test, registers equal to the target
+    unsigned long long skip[NSLOT];       // per slot, tests the last instruction does not run on
+    uint32_t meta[NSLOT][2];              // per slot, prefix index; live-mask index | dead << 8
+    uint32_t path[NSLOT][MAXLEV];         // per slot, candidate indices at positions s..D-2
+    unsigned long long wc[4];             // work counters (WaveCounters)
+    uint32_t tst[MAXT][2];                // survivor-phase test statistics
+};
+struct Pool {                      // wave-uniform bookkeeping
+    uint32_t qn;                   // entries queued
+    uint32_t pend;                 // entries queued per slot, 8 bits each
+    uint32_t cur;                  // slot of the parent being scanned
+    uint32_t nd;                   // drains so far
+};
+
 // The record is written field by field (no private copy).
 __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const JPath &jpath,
                                          uint32_t jlast, uint32_t mask, uint32_t flags)
@@ -263,8 +290,148 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
     }
 }
 
-// Value a value-group representative writes on one test; DIV/MOD only when
-// the wave holds any (the caller guards), everything else branch-free.
+// Survivor phase of the pooled value groups: queue entries [0, cnt), one
+// per lane, each tested from test t0 (where phase A stopped) to the last
+// with an early exit, against its own parent's state (its slot).  A group
+// still matching on every test gives one hit per allowed rd, with its
+// parent's prefix and suffix path.  A test that some entry's parent skips
+// (pending BR, VM stopped) keeps that entry's rd values whose register
+// already equals the target.
+__device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &a, const CandDev &cd,
+                                           WaveLds &sv, uint32_t cnt, int t0, int lane, WaveCounters &wc,
+                                           uint32_t (*tst)[2], Pool &pool)
+{
+    const int T = task.T;
+    const bool valid = (uint32_t)lane < cnt;
+    const uint32_t e = valid ? sv.queue[lane] : 0u;
+    const uint32_t q = e & 15u, bit = (e >> 4) & 63u, s = (e >> 19) & (NSLOT - 1u);
+    uint32_t ok = valid ? (e >> 10) & 0xFFu : 0u;
+    bool flt = (e >> 18) & 1u;
+    // tests skipped by any entry's parent (uniform), entries left per slot
+    uint64_t skor = 0;
+#pragma unroll
+    for (uint32_t z = 0; z < NSLOT; z++) {
+        const uint64_t b = __ballot(valid && s == z);
+        if (b) skor |= uni64(sv.skip[z]);
+        pool.pend -= (uint32_t)__popcll(b) << (8u * z);
+    }
+    const uint64_t skl = skor ? sv.skip[s] : 0ull;
+    const TestState *P = sv.slot[s];
+    const uint8_t *rmv = sv.srm[s];
+    // survivor-phase reach / end counts from one drain in eight are enough
+    // to order the tests, at an eighth of their cost
+    const bool tst8 = tst && (pool.nd & 7u) == 0u;
+    pool.nd++;
+    const uint32_t xa = bit >> 3, yb = bit & 7u, i = bit & 7u;
+    const uint32_t aimm = (i & 1u) ? (i >> 1) + 1u : 0x10000u - ((i >> 1) + 1u);
+    const bool divmod = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
+    bool any_div = __any(divmod && ok != 0u);
+    uint32_t n_ev = 0;
+    const uint32_t aimm2 = (aimm & 0xFFFFu) * 0x10001u;
+    for (int tt = t0; tt < T; tt++) {
+        const uint64_t al = __ballot(ok != 0u);
+        if (al == 0ull) break;
+        const uint32_t before = (uint32_t)__popcll(al);
+        n_ev += before;
+        if (any_div) any_div = __any(divmod && ok != 0u);
+        if (!any_div && tt + 1 < T && ((skor >> tt) & 3ull) == 0ull) {
+            // tests tt and tt+1 at once: each 16-bit half of a register
+            // holds one test's operand, and the packed 16-bit ops compute
+            // both values (mod 2^16, as the VM does)
+            const uint32_t tw0 = task.tgtw[tt], tw1 = task.tgtw[tt + 1];
+            const uint32_t tg2 = (tw0 & 0xFFFFu) | (tw1 << 16);
+            const uint32_t x2 = (uint32_t)P[tt].r[xa] | ((uint32_t)P[tt + 1].r[xa] << 16);
+            const uint32_t y2 = (uint32_t)P[tt].r[yb] | ((uint32_t)P[tt + 1].r[yb] << 16);
+            const u16x2 X = __builtin_bit_cast(u16x2, x2), Y = __builtin_bit_cast(u16x2, y2);
+            const u16x2 S = Y & (unsigned short)15;
+            uint32_t v = __builtin_bit_cast(uint32_t, X + Y);
+            v = q == 1u ? __builtin_bit_cast(uint32_t, X - Y) : v;
+            v = q == 2u ? __builtin_bit_cast(uint32_t, X * Y) : v;
+            v = q == 5u ? (x2 & y2) : v;
+            v = q == 6u ? (x2 | y2) : v;
+            v = q == 7u ? (x2 ^ y2) : v;
+            v = q == 8u ? __builtin_bit_cast(uint32_t, X << S) : v;
+            v = q == 9u ? __builtin_bit_cast(uint32_t, X >> S) : v;
+            v = q == 10u ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(i16x2, x2) >> __builtin_bit_cast(i16x2, S))
+                         : v;
+            v = q == 11u ? __builtin_bit_cast(uint32_t, X + __builtin_bit_cast(u16x2, aimm2)) : v;
+            const uint32_t d = v ^ tg2;
+            bool m0 = (d & 0xFFFFu) == 0u, m1 = (d >> 16) == 0u;
+            if (q == 12u) {
+                m0 = (tw0 >> (16 + bit)) & 1u;
+                m1 = (tw1 >> (16 + bit)) & 1u;
+            }
+            // the second test is evaluated for the lanes the first kept
+            const uint32_t mid = (uint32_t)__popcll(__ballot(ok != 0u && m0));
+            n_ev += mid;
+            if (!(m0 && m1)) ok = 0;
+            if (tst8) {
+                const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
+                if (lane == 0) {
+                    tst[tt][0] += before;
+                    tst[tt][1] += before - mid;
+                    tst[tt + 1][0] += mid;
+                    tst[tt + 1][1] += mid - after;
+                }
+            }
+            tt++;
+            continue;
+        }
+        const bool sk = (skl >> tt) & 1ull;
+        const uint32_t tw = task.tgtw[tt];
+        const uint32_t tg = tw & 0xFFFFu;
+        const uint32_t x = P[tt].r[xa], y = P[tt].r[yb];
+        const uint32_t sh = y & 15u;
+        uint32_t v = x + y;
+        v = q == 1u ? x - y : v;
+        v = q == 2u ? x * y : v;
+        v = q == 5u ? (x & y) : v;
+        v = q == 6u ? (x | y) : v;
+        v = q == 7u ? (x ^ y) : v;
+        v = q == 8u ? x << sh : v;
+        v = q == 9u ? x >> sh : v;
+        v = q == 10u ? (uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) : v;
+        v = q == 11u ? x + aimm : v;
+        bool match = (v & 0xFFFFu) == tg;
+        if (q == 12u) match = (tw >> (16 + bit)) & 1u;
+        if (any_div && divmod && !sk) {
+            if (y == 0u) {
+                ok &= rmv[tt];
+                flt = true;
+                match = true;
+            } else {
+                const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
+                const int32_t qt = sdiv16_q(sx, sy);
+                match = ((uint32_t)(q == OP_DIV - OP_ADD ? qt : sx - qt * sy) & 0xFFFFu) == tg;
+            }
+        }
+        if (sk)
+            ok &= rmv[tt];
+        else if (!match)
+            ok = 0;
+        if (tst8) {
+            const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
+            if (lane == 0) {
+                tst[tt][0] += before;
+                tst[tt][1] += before - after;
+            }
+        }
+    }
+    if (lane == 0) wc.c[2] += n_ev;
+    if (ok != 0u) {
+        const uint32_t mt = sv.meta[s][1], mm = mt & 127u;
+        const uint32_t g = cd.g->bits_gidx[mm][q][bit];
+        const uint32_t okf = cd.g->vg_rdmask[mm][g] & ok;
+        const uint32_t fl = ((mt >> 8) & 1u) | ((divmod && flt) ? 1u : 0u);
+        JPath jp;
+        jp.j0 = jp.j1 = 0;
+        jp.lds = sv.path[s];
+        const uint32_t pidx = sv.meta[s][0];
+        for (uint32_t r = 1; r < 8; r++)
+            if (okf >> r & 1u) emit_hit(a, pidx, jp, cd.g->vg_j[mm][g][r], 1u << r, fl);
+    }
+}
+
 // Parent with no target-matching register (the common case): scan the last
 // position by value groups held as wave bit masks.  Lane 8*a + b holds the
 // source pair (a, b); each ALU op keeps a 64-bit mask of its (op, a, b)
@@ -279,17 +446,20 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 // DIV and MOD keep a per-lane rd mask, since a zero divisor (a fault) makes
 // a test behave like that for that lane only.  The per-mask group bits are
 // read from the grammar table with scalar loads (a wave-uniform address).
+// The parent's state P sits in the wave's slot pool.cur; its surviving
+// groups are queued for the pooled survivor phase (pool_drain).
 __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const CandDev &cd,
                                                  const TestState *P,
                                                  uint32_t live, uint32_t prefix_idx, const JPath &jpath,
                                                  int lane, WaveCounters &wc, const uint8_t *regmatch,
-                                                 uint32_t *queue, uint32_t (*tst)[2], const ParentSummary &ps)
+                                                 WaveLds &sv, Pool &pool, uint32_t (*tst)[2],
+                                                 const ParentSummary &ps)
 {
     const int T = task.T;
     const int m = (live >> 1) & 127;
     if (lane == 0) {
-        wc.cand += cd.cand_total[m];
-        wc.suf += (unsigned)cd.count[m];
+        wc.c[0] += cd.cand_total[m];
+        wc.c[1] += (unsigned)cd.count[m];
     }
     const uint64_t *__restrict__ B = &cd.g->bits_init[m][0];
     uint64_t mk[NBKINDS];
@@ -367,156 +537,66 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
             }
         }
     }
-    if (lane == 0) wc.tests += evals;
+    if (lane == 0) wc.c[2] += evals;
     if (rm_and == 0u) return;   // no rd value is possible for any group
     mk[OP_DIV - OP_ADD] = __ballot(okd != 0u);
     mk[OP_MOD - OP_ADD] = __ballot(okm != 0u);
-    // phase B: the survivors, one group per lane (queue entry: kind | bit << 4
-    // | rd mask << 10 | fault << 18), tested to the end with an early exit
-    const uint32_t fl_base = ps.dead ? 1u : 0u;
-    const uint16_t *GX = &cd.g->bits_gidx[m][0][0];
-    const uint8_t *GM = cd.g->vg_rdmask[m];
-    const uint16_t *GJ = &cd.g->vg_j[m][0][0];
-    const int t0 = t;
-    // survivor-phase reach / end counts on one parent in eight are enough to
-    // order the tests, at an eighth of their cost
-    const bool tst8 = tst && ((prefix_idx + (a.k >= 2 ? jp_at(jpath, 0) : 0u)) & 7u) == 0u;
-    auto drain = [&](uint32_t cnt) {
-        const bool valid = (uint32_t)lane < cnt;
-        const uint32_t e = valid ? queue[lane] : 0u;
-        const uint32_t q = e & 15u, bit = (e >> 4) & 63u;
-        uint32_t ok = valid ? (e >> 10) & 0xFFu : 0u;
-        bool flt = (e >> 18) & 1u;
-        const uint32_t xa = bit >> 3, yb = bit & 7u, i = bit & 7u;
-        const uint32_t aimm = (i & 1u) ? (i >> 1) + 1u : 0x10000u - ((i >> 1) + 1u);
-        const bool divmod = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
-        bool any_div = __any(divmod && ok != 0u);
-        uint32_t n_ev = 0;
-        const uint32_t aimm2 = (aimm & 0xFFFFu) * 0x10001u;
-        for (int tt = t0; tt < T; tt++) {
-            const uint64_t al = __ballot(ok != 0u);
-            if (al == 0ull) break;
-            const uint32_t before = (uint32_t)__popcll(al);
-            n_ev += before;
-            if (any_div) any_div = __any(divmod && ok != 0u);
-            if (!any_div && tt + 1 < T && ((ps.skipm >> tt) & 3ull) == 0ull) {
-                // tests tt and tt+1 at once: each 16-bit half of a register
-                // holds one test's operand, and the packed 16-bit ops compute
-                // both values (mod 2^16, as the VM does)
-                const uint32_t tw0 = task.tgtw[tt], tw1 = task.tgtw[tt + 1];
-                const uint32_t tg2 = (tw0 & 0xFFFFu) | (tw1 << 16);
-                const uint32_t x2 = (uint32_t)P[tt].r[xa] | ((uint32_t)P[tt + 1].r[xa] << 16);
-                const uint32_t y2 = (uint32_t)P[tt].r[yb] | ((uint32_t)P[tt + 1].r[yb] << 16);
-                const u16x2 X = __builtin_bit_cast(u16x2, x2), Y = __builtin_bit_cast(u16x2, y2);
-                const u16x2 S = Y & (unsigned short)15;
-                uint32_t v = __builtin_bit_cast(uint32_t, X + Y);
-                v = q == 1u ? __builtin_bit_cast(uint32_t, X - Y) : v;
-                v = q == 2u ? __builtin_bit_cast(uint32_t, X * Y) : v;
-                v = q == 5u ? (x2 & y2) : v;
-                v = q == 6u ? (x2 | y2) : v;
-                v = q == 7u ? (x2 ^ y2) : v;
-                v = q == 8u ? __builtin_bit_cast(uint32_t, X << S) : v;
-                v = q == 9u ? __builtin_bit_cast(uint32_t, X >> S) : v;
-                v = q == 10u ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(i16x2, x2) >> __builtin_bit_cast(i16x2, S))
-                             : v;
-                v = q == 11u ? __builtin_bit_cast(uint32_t, X + __builtin_bit_cast(u16x2, aimm2)) : v;
-                const uint32_t d = v ^ tg2;
-                bool m0 = (d & 0xFFFFu) == 0u, m1 = (d >> 16) == 0u;
-                if (q == 12u) {
-                    m0 = (tw0 >> (16 + bit)) & 1u;
-                    m1 = (tw1 >> (16 + bit)) & 1u;
-                }
-                // the second test is evaluated for the lanes the first kept
-                const uint32_t mid = (uint32_t)__popcll(__ballot(ok != 0u && m0));
-                n_ev += mid;
-                if (!(m0 && m1)) ok = 0;
-                if (tst8) {
-                    const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
-                    if (lane == 0) {
-                        tst[tt][0] += before;
-                        tst[tt][1] += before - mid;
-                        tst[tt + 1][0] += mid;
-                        tst[tt + 1][1] += mid - after;
-                    }
-                }
-                tt++;
-                continue;
-            }
-            if ((ps.skipm >> tt) & 1ull) {
-                ok &= uni(regmatch[tt]);
-                continue;
-            }
-            const uint32_t tw = task.tgtw[tt];
-            const uint32_t tg = tw & 0xFFFFu;
-            const uint32_t x = P[tt].r[xa], y = P[tt].r[yb];
-            const uint32_t sh = y & 15u;
-            uint32_t v = x + y;
-            v = q == 1u ? x - y : v;
-            v = q == 2u ? x * y : v;
-            v = q == 5u ? (x & y) : v;
-            v = q == 6u ? (x | y) : v;
-            v = q == 7u ? (x ^ y) : v;
-            v = q == 8u ? x << sh : v;
-            v = q == 9u ? x >> sh : v;
-            v = q == 10u ? (uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) : v;
-            v = q == 11u ? x + aimm : v;
-            bool match = (v & 0xFFFFu) == tg;
-            if (q == 12u) match = (tw >> (16 + bit)) & 1u;
-            if (any_div && divmod) {
-                if (y == 0u) {
-                    ok &= regmatch[tt];
-                    flt = true;
-                    match = true;
-                } else {
-                    const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
-                    const int32_t qt = sdiv16_q(sx, sy);
-                    match = ((uint32_t)(q == OP_DIV - OP_ADD ? qt : sx - qt * sy) & 0xFFFFu) == tg;
-                }
-            }
-            if (!match) ok = 0;
-            if (tst8) {
-                const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
-                if (lane == 0) {
-                    tst[tt][0] += before;
-                    tst[tt][1] += before - after;
-                }
-            }
-        }
-        if (lane == 0) wc.tests += n_ev;
-        if (ok != 0u) {
-            const uint32_t g = GX[q * 64 + bit];
-            const uint32_t okf = GM[g] & ok;
-            const uint32_t fl = fl_base | ((divmod && flt) ? 1u : 0u);
-            for (uint32_t r = 1; r < 8; r++)
-                if (okf >> r & 1u) emit_hit(a, prefix_idx, jpath, GJ[g * 8 + r], 1u << r, fl);
-        }
-    };
-    uint32_t qn = 0;
-    for (int q = 0; q < NBKINDS; q++) {
-        const uint64_t bits = mk[q];
-        if (bits == 0ull) continue;
-        if (bits >> lane & 1ull) {
-            const uint32_t okq = q == OP_DIV - OP_ADD ? okd : (q == OP_MOD - OP_ADD ? okm : rm_and);
-            const bool fq = (q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD) && fault;
-            queue[qn + (uint32_t)__popcll(bits & ((1ull << lane) - 1ull))] =
-                (uint32_t)q | ((uint32_t)lane << 4) | (okq << 10) | (fq ? 1u << 18 : 0u);
-        }
-        qn += (uint32_t)__popcll(bits);
-        if (qn >= 64) {
-            wave_sync();
-            drain(64);
-            wave_sync();
-            const uint32_t rest = qn - 64;
-            if ((uint32_t)lane < rest) queue[lane] = queue[64 + lane];
-            qn = rest;
-            wave_sync();
-        }
+    uint64_t anyk = 0;
+#pragma unroll
+    for (int q = 0; q < NBKINDS; q++) anyk |= mk[q];
+    if (anyk == 0ull) return;
+    // phase B is pooled: the survivors wait in the wave's queue with those of
+    // earlier parents (whose states stay in their slots) and are tested 64
+    // at a time by pool_drain.  Queue entry: kind | bit << 4 | rd mask << 10
+    // | fault << 18 | slot << 19.
+    const uint32_t cur = pool.cur;
+    if (lane == 0) {
+        sv.meta[cur][0] = prefix_idx;
+        sv.meta[cur][1] = (uint32_t)m | (ps.dead ? 0x100u : 0u);
+        sv.skip[cur] = ps.skipm;
     }
-    if (qn) {
-        wave_sync();
-        drain(qn);
-        wave_sync();
+    if (lane + 1 < a.k) sv.path[cur][lane] = jp_at(jpath, lane);
+    const uint32_t next = (cur + 1u) & (NSLOT - 1u);
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint32_t done = 0;   // kinds pushed, in kind order
+    for (;;) {
+#pragma unroll
+        for (int q = 0; q < NBKINDS; q++) {
+            const uint64_t bits = mk[q];
+            const uint32_t c = (uint32_t)__popcll(bits);
+            if (done == (uint32_t)q && pool.qn + c <= 128u) {
+                if (bits >> lane & 1ull) {
+                    const uint32_t okq = q == OP_DIV - OP_ADD ? okd : (q == OP_MOD - OP_ADD ? okm : rm_and);
+                    const bool fq = (q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD) && fault;
+                    sv.queue[pool.qn + (uint32_t)__popcll(bits & lt)] =
+                        (uint32_t)q | ((uint32_t)lane << 4) | (okq << 10) | (fq ? 1u << 18 : 0u) | (cur << 19);
+                }
+                pool.qn += c;
+                pool.pend += c << (8u * cur);
+                done = (uint32_t)q + 1u;
+            }
+        }
+        const bool all = done == (uint32_t)NBKINDS;
+        // drain 64 when that many wait; once this parent is queued, drain
+        // the rest too if the next parent's slot still has entries
+        uint32_t cnt = 0;
+        if (pool.qn >= 64u)
+            cnt = 64u;
+        else if (all && ((pool.pend >> (8u * next)) & 0xFFu) != 0u)
+            cnt = pool.qn;
+        if (cnt) {
+            wave_sync();
+            pool_drain(task, a, cd, sv, cnt, TA, lane, wc, tst, pool);
+            wave_sync();
+            const uint32_t rest = pool.qn - cnt;
+            if (cnt == 64u && (uint32_t)lane < rest) sv.queue[lane] = sv.queue[64 + lane];
+            pool.qn = rest;
+            wave_sync();
+            continue;
+        }
+        if (all) break;
     }
+    pool.cur = next;
 }
 
 // Parent where some register already equals the target: every candidate
@@ -652,9 +732,12 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
         drain(qn);
         wave_sync();
     }
-    wc.cand += n_cand;
-    wc.suf += n_suf;
-    wc.tests += n_tests;
+    const unsigned long long c0 = wave_sum(n_cand), c1 = wave_sum(n_suf), c2 = wave_sum(n_tests);
+    if (lane == 0) {
+        wc.c[0] += c0;
+        wc.c[1] += c1;
+        wc.c[2] += c2;
+    }
 }
 
 // k == 0: the prefix is the whole kernel; check its registers directly.
@@ -671,9 +754,9 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
     const bool cyc = act && ((uint32_t)st.exec + 2u > (uint32_t)task.max_cycles);
     const uint32_t dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
     if (lane == 0) {
-        wc.parents++;
-        wc.cand += (unsigned)__popc(live & 0xFEu);
-        wc.suf++;
+        wc.c[3]++;
+        wc.c[0] += (unsigned)__popc(live & 0xFEu);
+        wc.c[1]++;
         if (pm & 0xFEu) {
             const uint32_t slot = atomicAdd(a.hit_count, 1u);
             if (slot < a.hit_cap) {
@@ -693,22 +776,22 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
 // arguments (the single-task path; no per-unit item lookup, the context is
 // read from the kernel-argument segment).  MULTI = true: lch.items[0..n_items)
 // in device memory, the item of each unit found by binary search over
-// unit_begin.  The per-wave state levels live in dynamic LDS, lch.levels of
-// them per wave.
+// unit_begin.  Per wave, dynamic LDS holds the state levels above the
+// scanned parents (max(lch.levels, 2) - 1 of them); the scanned parents
+// themselves (level k - 1) sit in the NSLOT slots of the wave's WaveLds,
+// where they stay while their survivors wait in the pooled queue.
 template <bool MULTI>
 __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs &a1, const DevTask &task1)
 {
     extern __shared__ TestState s_dyn[];
-    __shared__ uint32_t s_live[SCAN_WAVES][MAXLEV];
-    __shared__ uint32_t s_idx[SCAN_WAVES][MAXLEV];
-    __shared__ uint32_t s_queue[SCAN_WAVES][128];
-    __shared__ uint8_t s_rm[SCAN_WAVES][MAXT];
+    __shared__ WaveLds s_wave[SCAN_WAVES];
     __shared__ int s_cnt[128], s_split[128];
     __shared__ uint32_t s_ctot[128];
-    __shared__ uint32_t s_tst[SCAN_WAVES][MAXT][2];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    for (int i = threadIdx.x; i < SCAN_WAVES * MAXT * 2; i += SCAN_WG) (&s_tst[0][0][0])[i] = 0u;
-    uint32_t (*tst)[2] = (!MULTI && lch.tstat) ? s_tst[w] : nullptr;
+    WaveLds &sv = s_wave[w];
+    for (int i = lane; i < MAXT * 2; i += 64) (&sv.tst[0][0])[i] = 0u;
+    if (lane < 4) sv.wc[lane] = 0ull;
+    uint32_t (*tst)[2] = (!MULTI && lch.tstat) ? sv.tst : nullptr;
     // per-mask tables live in LDS: no global round trip per parent
     for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
         s_cnt[i] = lch.cand.count[i];
@@ -720,10 +803,13 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     cand.count = s_cnt;
     cand.split = s_split;
     cand.cand_total = s_ctot;
-    const uint32_t nlev = lch.levels;
-    auto lev = [&](int L) -> TestState * { return s_dyn + ((uint32_t)w * nlev + (uint32_t)L) * MAXT; };
+    const uint32_t nreg = (lch.levels > 2u ? lch.levels : 2u) - 1u;
+    auto lev = [&](int L) -> TestState * { return s_dyn + ((uint32_t)w * nreg + (uint32_t)L) * MAXT; };
+    Pool pool;
+    pool.qn = pool.pend = pool.cur = pool.nd = 0;
 
     WaveCounters wc;
+    wc.c = sv.wc;
     // current item (wave-uniform) and its context: the one-item kernel reads
     // its work item from the kernel arguments (a1), where the fields stay
     // (reloaded through the scalar cache, not held in registers); the
@@ -744,6 +830,15 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     }
     const uint64_t grab = lch.grab > 1u ? lch.grab : 1u;
     uint64_t ug_next = 0, ug_end = 0;   // the wave's current run of units
+    // the pooled survivors of the item the wave leaves (fewer than 64); the
+    // one-item kernel has a single such drain, after its unit loop
+    auto flush_pool = [&]() {
+        const int ta = taskp->T < (int)a.tile.mask_tests ? taskp->T : (int)a.tile.mask_tests;
+        wave_sync();
+        pool_drain(*taskp, a, cand, sv, pool.qn, ta, lane, wc, tst, pool);
+        wave_sync();
+        pool.qn = 0;
+    };
     for (;;) {
         if (ug_next >= ug_end) {
             unsigned long long g0 = 0;
@@ -755,6 +850,9 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
         }
         const uint64_t ug = ug_next++;
         if (MULTI && (ug < it_begin || ug >= it_end)) {
+            if constexpr (MULTI) {
+                if (pool.qn) flush_pool();
+            }
             // the item holding this unit: largest ii with unit_begin <= ug
             uint32_t lo = 0, hi = lch.n_items;
             while (hi - lo > 1) {
@@ -794,9 +892,12 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
         }
         const pca_prefix_t *P = &a.prefixes[pi];
         const uint32_t live0 = uni(P->live);
-        if (pi != cur_i) {
-            replay_prefix(task, P, a.s, lev(0), lane);
-            cur_i = pi;
+        // the scanned parents (level k - 1) sit in the slots: for k == 1 the
+        // prefix itself, replayed into the current slot every time
+        auto lvl = [&](int L) -> TestState * { return L == k - 1 ? sv.slot[pool.cur] : lev(L); };
+        if (k == 1 || pi != cur_i) {
+            replay_prefix(task, P, a.s, k == 1 ? lvl(0) : lev(0), lane);
+            cur_i = k == 1 ? 0xFFFFFFFFu : pi;
             cur_js = 0xFFFFFFFFu;
             wave_sync();
         }
@@ -804,7 +905,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
             leaf_parent(task, a, lev(0), live0, pi, lane, wc);
             continue;
         }
-        if (lane == 0) s_live[w][0] = live0;
+        if (lane == 0) sv.live[0] = live0;
         // level of the scanned parents' first level: 0 (the prefix itself,
         // k == 1), 1 (k == 2), 2 (k >= 3); the block's children sit there
         const int Lu = k <= 1 ? 0 : (k == 2 ? 1 : 2);
@@ -812,14 +913,14 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
         if (k >= 3 && js != cur_js) {
             const uint32_t c1 = uni(cand_at(cand, live0, a.D - a.s - 1, (int)js));
             step_level(task, lev(0), lev(1), c1, a.s, lane);
-            if (lane == 0) s_live[w][1] = live_after(live0, c1);
+            if (lane == 0) sv.live[1] = live_after(live0, c1);
             cur_js = js;
         }
         if (k >= 2) {
             if (k >= 3) {
                 wave_sync();
-                lvp = uni(s_live[w][1]);
-                if (lane == 0) s_idx[w][1] = js;
+                lvp = uni(sv.live[1]);
+                if (lane == 0) sv.idx[1] = js;
             }
             const int mfp = a.D - a.s - Lu;   // children at position s + Lu - 1
             const int nc = cand_count(cand, lvp, mfp);
@@ -839,78 +940,76 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 if (act) {
                     st = load_ps(&lev(Lu - 1)[lane]);
                     step_packed(st, c, (uint32_t)(a.s + Lu - 1));
-                    store_ps(&lev(Lu)[lane], st);
+                    store_ps(&lvl(Lu)[lane], st);
                 }
-                if (fused) ps_child = summarize_state(task, st, act, (uint32_t)(a.D - 1), lane, my_tgt, s_rm[w]);
+                if (fused)
+                    ps_child = summarize_state(task, st, act, (uint32_t)(a.D - 1), lane, my_tgt, sv.srm[pool.cur]);
                 if (lane == 0) {
-                    s_live[w][Lu] = live_after(lvp, c);
-                    s_idx[w][Lu] = j0 + ci;
+                    sv.live[Lu] = live_after(lvp, c);
+                    sv.idx[Lu] = j0 + ci;
                 }
             }
-            if (lane == 0 && Lu + 1 < MAXLEV) s_idx[w][Lu + 1] = 0;
+            if (lane == 0 && Lu + 1 < MAXLEV) sv.idx[Lu + 1] = 0;
             wave_sync();
             // depth-first over positions s+Lu .. D-2, scanning position D-1
             int L = Lu;
             for (;;) {
                 if (L == k - 1) {
                     JPath jpath;
-                    jpath.lds = &s_idx[w][1];
+                    jpath.lds = &sv.idx[1];
                     jpath.j0 = jpath.j1 = 0;
                     {
-                        const TestState *PP = lev(L);
-                        const uint32_t lvL = uni(s_live[w][L]);
+                        const TestState *PP = lvl(L);
+                        const uint32_t lvL = uni(sv.live[L]);
                         const ParentSummary ps =
-                            fused ? ps_child : summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt, s_rm[w]);
-                        wc.parents += lane == 0;
+                            fused ? ps_child
+                                  : summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt, sv.srm[pool.cur]);
+                        if (lane == 0) wc.c[3]++;
                         wave_sync();
                         if (ps.pm == 0u)
-                            scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, s_rm[w], s_queue[w], tst, ps);
+                            scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.srm[pool.cur], sv, pool,
+                                             tst, ps);
                         else
-                            scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, s_queue[w], ps);
+                            scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
                     }
                     wave_sync();
                     if (L == Lu) break;
                     L--;
-                    if (lane == 0) s_idx[w][L + 1]++;
+                    if (lane == 0) sv.idx[L + 1]++;
                     wave_sync();
                     continue;
                 }
                 const int pos = a.s + L;
                 const int mf = a.D - pos - 1;
-                const uint32_t lvL = uni(s_live[w][L]);
+                const uint32_t lvL = uni(sv.live[L]);
                 const int n = cand_count(cand, lvL, mf);
-                const int ix = (int)uni(s_idx[w][L + 1]);
+                const int ix = (int)uni(sv.idx[L + 1]);
                 if (ix >= n) {
                     if (L == Lu) break;
                     L--;
-                    if (lane == 0) s_idx[w][L + 1]++;
+                    if (lane == 0) sv.idx[L + 1]++;
                     wave_sync();
                     continue;
                 }
                 const uint32_t c = uni(cand_at(cand, lvL, mf, ix));
-                step_level(task, lev(L), lev(L + 1), c, pos, lane);
+                step_level(task, lev(L), lvl(L + 1), c, pos, lane);
                 if (lane == 0) {
-                    s_live[w][L + 1] = live_after(lvL, c);
-                    if (L + 2 < MAXLEV) s_idx[w][L + 2] = 0;
+                    sv.live[L + 1] = live_after(lvL, c);
+                    if (L + 2 < MAXLEV) sv.idx[L + 2] = 0;
                 }
                 wave_sync();
                 L++;
             }
         }
     }
-    const unsigned long long c0 = wave_sum(wc.cand), c1 = wave_sum(wc.suf), c2 = wave_sum(wc.tests),
-                             c3 = wave_sum(wc.parents);
-    if (lane == 0) {
-        atomicAdd(&lch.counters[0], c0);
-        atomicAdd(&lch.counters[1], c1);
-        atomicAdd(&lch.counters[2], c2);
-        atomicAdd(&lch.counters[3], c3);
-    }
+    if (pool.qn) flush_pool();
+    wave_sync();
+    if (lane < 4) atomicAdd(&lch.counters[lane], sv.wc[lane]);
     if (!MULTI && lch.tstat) {
         __syncthreads();
         for (int i = threadIdx.x; i < MAXT * 2; i += SCAN_WG) {
             uint32_t v = 0;
-            for (int ww = 0; ww < SCAN_WAVES; ww++) v += (&s_tst[ww][0][0])[i];
+            for (int ww = 0; ww < SCAN_WAVES; ww++) v += (&s_wave[ww].tst[0][0])[i];
             if (v) atomicAdd(&lch.tstat[i], (unsigned long long)v);
         }
     }
@@ -1497,7 +1596,11 @@ void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st)
     hipLaunchKernelGGL(table_init_kernel, dim3((unsigned)((cap + 255) / 256)), dim3(256), 0, st, table, cap);
 }
 
-size_t scan_dyn_lds(uint32_t levels) { return (size_t)SCAN_WAVES * levels * MAXT * sizeof(TestState); }
+size_t scan_dyn_lds(uint32_t levels)
+{
+    const size_t per_wave = (levels > 2u ? levels : 2u) - 1u;
+    return (size_t)SCAN_WAVES * per_wave * MAXT * sizeof(TestState);
+}
 
 hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
 {
