@@ -57,6 +57,7 @@ struct GrammarDev {
     uint16_t vg_j[128][VGMAX][8];             // list index of (group, rd)
     uint64_t bits_init[128][NBKINDS];         // groups present per kind (ADD..ASR, ADDI, LDI)
     uint16_t bits_gidx[128][NBKINDS][64];     // group of a bit (0xFFFF: none)
+    uint16_t kind_bits[128][64];              // per lane (bit position), the kinds with a group there
 };
 
 // What the kernels see: the shared tables, and the per-mask scalars, which a

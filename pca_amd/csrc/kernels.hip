@@ -265,6 +265,7 @@ struct WaveLds {
     uint32_t meta[NSLOT][2];              // per slot, prefix index; live-mask index | dead << 8
     uint32_t path[NSLOT][MAXLEV];         // per slot, candidate indices at positions s..D-2
     unsigned long long wc[4];             // work counters (WaveCounters)
+    uint32_t evl[64];                     // per lane, phase-A group evaluations
     uint32_t tst[MAXT][2];                // survivor-phase test statistics
 };
 struct Pool {                      // wave-uniform bookkeeping
@@ -433,19 +434,20 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
 }
 
 // Parent with no target-matching register (the common case): scan the last
-// position by value groups held as wave bit masks.  Lane 8*a + b holds the
-// source pair (a, b); each ALU op keeps a 64-bit mask of its (op, a, b)
-// groups still matching, updated by one compare ballot per test, so one
-// test of every group of the live mask costs one evaluation per op, and an
-// op with no group left is skipped.  ADDI groups use lane 8*rd + i (source
-// rd, i-th immediate of -1, 1, -2, 2, -3, 3); LDI groups are per-test
-// constants (DevTask::tgtw).  A test on which the last instruction does
-// not run (skipped by a pending BR, VM stopped) keeps the rd values whose old
-// register already equals the target: that regmatch mask is ANDed into the
-// rd masks of all groups (rm_and) and prunes the groups left without an rd.
-// DIV and MOD keep a per-lane rd mask, since a zero divisor (a fault) makes
-// a test behave like that for that lane only.  The per-mask group bits are
-// read from the grammar table with scalar loads (a wave-uniform address).
+// position by value groups.  Lane 8*a + b holds the source pair (a, b) and
+// a mask (kl) with one bit per ALU kind whose (op, a, b) group exists for
+// the live mask and still matches: one test of every group of the parent
+// costs one evaluation and one compare per op across the wave, with no
+// wave-uniform state per kind (no ballots until the survivors are queued).
+// ADDI groups use lane 8*rd + i (source rd, i-th immediate of -1, 1, -2, 2,
+// -3, 3); LDI groups are per-test constants (DevTask::tgtw, lane i the i-th
+// immediate).  A test on which the last instruction does not run (skipped
+// by a pending BR, VM stopped) keeps the rd values whose old register
+// already equals the target: that regmatch mask is ANDed into the rd masks
+// of all groups (rm_and) and prunes the groups left without an rd.  DIV and
+// MOD keep a per-lane rd mask, since a zero divisor (a fault) makes a test
+// behave like that for that lane only.  The per-lane kind masks of each
+// live mask come from the grammar table (GrammarDev::kind_bits).
 // The parent's state P sits in the wave's slot pool.cur; its surviving
 // groups are queued for the pooled survivor phase (pool_drain).
 __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const CandDev &cd,
@@ -461,36 +463,29 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         wc.c[0] += cd.cand_total[m];
         wc.c[1] += (unsigned)cd.count[m];
     }
-    const uint64_t *__restrict__ B = &cd.g->bits_init[m][0];
-    uint64_t mk[NBKINDS];
-#pragma unroll
-    for (int q = 0; q < NBKINDS; q++) mk[q] = B[q];
+    // per lane (source pair a, b; ADDI rd, i; LDI i), bit q: the group of
+    // kind q on this lane is still matching (DIV and MOD: rd masks okd, okm)
+    uint32_t kl = cd.g->kind_bits[m][lane];
     // every DIV / MOD group has all of r1..r7 as rd (the host checks this)
-    uint32_t okd = (mk[OP_DIV - OP_ADD] >> lane & 1ull) ? 0xFEu : 0u;
-    uint32_t okm = (mk[OP_MOD - OP_ADD] >> lane & 1ull) ? 0xFEu : 0u;
-    mk[OP_DIV - OP_ADD] = mk[OP_MOD - OP_ADD] = 0;   // tracked per lane in okd / okm
+    uint32_t okd = (kl >> (OP_DIV - OP_ADD) & 1u) ? 0xFEu : 0u;
+    uint32_t okm = (kl >> (OP_MOD - OP_ADD) & 1u) ? 0xFEu : 0u;
+    kl &= ~((1u << (OP_DIV - OP_ADD)) | (1u << (OP_MOD - OP_ADD)));
     bool fault = false;
     uint32_t rm_and = 0xFFu;
     const uint32_t ra = (uint32_t)lane >> 3, rb = (uint32_t)lane & 7u;
     const uint32_t addi_imm = (rb & 1u) ? (rb >> 1) + 1u : 0x10000u - ((rb >> 1) + 1u);
-    uint32_t evals = 0;
-    // phase A: the first tests by masks, one evaluation per op for all groups
+    uint32_t ev = 0;   // this lane's group evaluations
+    // phase A: the first tests, every group of the parent at once
     const int TA = T < (int)a.tile.mask_tests ? T : (int)a.tile.mask_tests;
     int t = 0;
     for (; t < TA; t++) {
-        const uint64_t dvm = __ballot(okd != 0u), mdm = __ballot(okm != 0u);
-        uint64_t any = dvm | mdm;
-        uint32_t alive = (uint32_t)__popcll(dvm) + (uint32_t)__popcll(mdm);
-#pragma unroll
-        for (int q = 0; q < NBKINDS; q++) {
-            any |= mk[q];
-            alive += (uint32_t)__popcll(mk[q]);
-        }
-        if (any == 0ull) break;
+        const uint32_t nal = (uint32_t)__popc(kl) + (okd != 0u ? 1u : 0u) + (okm != 0u ? 1u : 0u);
+        if (!__any(nal != 0u)) break;
         // group evaluations of this test; once a skipped test has narrowed
         // the rd values (rm_and), groups left without one are not told
         // apart here, so such parents are not counted further
-        if (rm_and == 0xFFu) evals += alive;
+        if (rm_and == 0xFFu) ev += nal;
+        const unsigned long long alive = (tst && t >= 2) ? wave_sum((unsigned long long)nal) : 0ull;
         if ((ps.skipm >> t) & 1ull) {
             const uint32_t rmt = uni(regmatch[t]);
             okd &= rmt;
@@ -502,18 +497,19 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         const uint32_t tw = task.tgtw[t];
         const uint32_t tg = tw & 0xFFFFu;
         const uint32_t sh = y & 15u;
-        if (mk[0]) mk[0] &= __ballot(((x + y) & 0xFFFFu) == tg);
-        if (mk[1]) mk[1] &= __ballot(((x - y) & 0xFFFFu) == tg);
-        if (mk[2]) mk[2] &= __ballot(((x * y) & 0xFFFFu) == tg);
-        if (mk[5]) mk[5] &= __ballot((x & y) == tg);
-        if (mk[6]) mk[6] &= __ballot((x | y) == tg);
-        if (mk[7]) mk[7] &= __ballot((x ^ y) == tg);
-        if (mk[8]) mk[8] &= __ballot(((x << sh) & 0xFFFFu) == tg);
-        if (mk[9]) mk[9] &= __ballot((x >> sh) == tg);
-        if (mk[10]) mk[10] &= __ballot(((uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) & 0xFFFFu) == tg);
-        if (mk[11]) mk[11] &= __ballot(((x + addi_imm) & 0xFFFFu) == tg);
-        mk[12] &= tw >> 16;
-        if (dvm | mdm) {
+        uint32_t keep = (lane < 16 && ((tw >> (16 + lane)) & 1u)) ? 1u << 12 : 0u;
+        keep |= (((x + y) & 0xFFFFu) == tg ? 1u : 0u) << 0;
+        keep |= (((x - y) & 0xFFFFu) == tg ? 1u : 0u) << 1;
+        keep |= (((x * y) & 0xFFFFu) == tg ? 1u : 0u) << 2;
+        keep |= ((x & y) == tg ? 1u : 0u) << 5;
+        keep |= ((x | y) == tg ? 1u : 0u) << 6;
+        keep |= ((x ^ y) == tg ? 1u : 0u) << 7;
+        keep |= (((x << sh) & 0xFFFFu) == tg ? 1u : 0u) << 8;
+        keep |= ((x >> sh) == tg ? 1u : 0u) << 9;
+        keep |= (((uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) & 0xFFFFu) == tg ? 1u : 0u) << 10;
+        keep |= (((x + addi_imm) & 0xFFFFu) == tg ? 1u : 0u) << 11;
+        kl &= keep;
+        if (__any(okd != 0u || okm != 0u)) {
             const bool z = y == 0u;
             if (z) {
                 const uint32_t rmt = regmatch[t];
@@ -528,23 +524,17 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
             }
         }
         if (tst && t >= 2) {   // positions 2.. are reordered by these counts
-            uint32_t after = (uint32_t)__popcll(__ballot(okd != 0u)) + (uint32_t)__popcll(__ballot(okm != 0u));
-#pragma unroll
-            for (int q = 0; q < NBKINDS; q++) after += (uint32_t)__popcll(mk[q]);
+            const unsigned long long after =
+                wave_sum((unsigned long long)((uint32_t)__popc(kl) + (okd != 0u ? 1u : 0u) + (okm != 0u ? 1u : 0u)));
             if (lane == 0) {
-                tst[t][0] += alive;
-                tst[t][1] += alive - after;
+                tst[t][0] += (uint32_t)alive;
+                tst[t][1] += (uint32_t)(alive - after);
             }
         }
     }
-    if (lane == 0) wc.c[2] += evals;
+    if (ev) __hip_atomic_fetch_add(&sv.evl[lane], ev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     if (rm_and == 0u) return;   // no rd value is possible for any group
-    mk[OP_DIV - OP_ADD] = __ballot(okd != 0u);
-    mk[OP_MOD - OP_ADD] = __ballot(okm != 0u);
-    uint64_t anyk = 0;
-#pragma unroll
-    for (int q = 0; q < NBKINDS; q++) anyk |= mk[q];
-    if (anyk == 0ull) return;
+    if (!__any(kl != 0u || okd != 0u || okm != 0u)) return;
     // phase B is pooled: the survivors wait in the wave's queue with those of
     // earlier parents (whose states stay in their slots) and are tested 64
     // at a time by pool_drain.  Queue entry: kind | bit << 4 | rd mask << 10
@@ -562,7 +552,9 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     for (;;) {
 #pragma unroll
         for (int q = 0; q < NBKINDS; q++) {
-            const uint64_t bits = mk[q];
+            const uint64_t bits = __ballot(q == OP_DIV - OP_ADD   ? okd != 0u
+                                           : q == OP_MOD - OP_ADD ? okm != 0u
+                                                                  : ((kl >> q) & 1u) != 0u);
             const uint32_t c = (uint32_t)__popcll(bits);
             if (done == (uint32_t)q && pool.qn + c <= 128u) {
                 if (bits >> lane & 1ull) {
@@ -791,6 +783,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     WaveLds &sv = s_wave[w];
     for (int i = lane; i < MAXT * 2; i += 64) (&sv.tst[0][0])[i] = 0u;
     if (lane < 4) sv.wc[lane] = 0ull;
+    sv.evl[lane] = 0u;
     uint32_t (*tst)[2] = (!MULTI && lch.tstat) ? sv.tst : nullptr;
     // per-mask tables live in LDS: no global round trip per parent
     for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
@@ -1003,6 +996,9 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
         }
     }
     if (pool.qn) flush_pool();
+    wave_sync();
+    const unsigned long long evs = wave_sum((unsigned long long)sv.evl[lane]);
+    if (lane == 0) sv.wc[2] += evs;
     wave_sync();
     if (lane < 4) atomicAdd(&lch.counters[lane], sv.wc[lane]);
     if (!MULTI && lch.tstat) {

@@ -110,6 +110,11 @@ void grammar_image(bool branchless, GrammarDev *img)
         std::memcpy(img->vg_j[m], &ct.vg_j[(size_t)m * VG_MAX * 8], sizeof(img->vg_j[m]));
         std::memcpy(img->bits_init[m], &ct.bits_init[(size_t)m * NBK], sizeof(img->bits_init[m]));
         std::memcpy(img->bits_gidx[m], &ct.bits_gidx[(size_t)m * NBK * 64], sizeof(img->bits_gidx[m]));
+        for (int l = 0; l < 64; l++) {
+            uint32_t kinds = 0;
+            for (int q = 0; q < NBKINDS; q++) kinds |= (uint32_t)((img->bits_init[m][q] >> l) & 1ull) << q;
+            img->kind_bits[m][l] = (uint16_t)kinds;
+        }
     }
 }
 
