@@ -452,9 +452,9 @@ def window(task, D, s, B, warmup, steps, progress=False):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--steps", type=int, default=20)
     # the first launches try the scan tiles (one each) before the winner is kept
-    ap.add_argument("--warmup", type=int, default=6)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--depth", type=int, default=8)
     ap.add_argument("--split", type=int, default=5)
