@@ -65,19 +65,18 @@ PCA_HD uint32_t flags_sub(uint32_t a, uint32_t b)
 // The quotient x / y of C's int division (truncated toward zero) for x, y
 // in [-32768, 32767], y != 0.  On the device it comes from a float
 // reciprocal instead of the integer-division sequence: x and y are exact in
-// float, and the product with v_rcp_f32's estimate is within 2^-22 of x / y
-// relative, i.e. within 2^-7 / |y| absolute -- less than the distance from
-// any non-integer quotient (a multiple of 1 / |y|) to an integer, so
-// truncation is exact there; an integer quotient can come out one short in
-// magnitude, which the remainder (then of size |y|) shows and one step
-// fixes.  Checked over all 2^32 operand pairs against the CPU VM.
+// float; v_rcp_f32's estimate of 1/y (within one ulp), raised by the factor
+// 1 + 2^-20, times x lies above |x / y| in magnitude (so an integer quotient
+// is not truncated to the integer below it) by less than 2^-4 / |y|, which
+// is less than the distance from any non-integer quotient (a multiple of
+// 1 / |y|) to the next integer, so truncation gives the exact quotient.
+// Checked over all 2^32 operand pairs against the CPU VM on the device, and
+// on the CPU with the reciprocal rounded to nearest and one ulp either side.
 PCA_HD int32_t sdiv16_q(int32_t x, int32_t y)
 {
 #if defined(__HIP_DEVICE_COMPILE__)
-    int32_t q = (int32_t)((float)x * __builtin_amdgcn_rcpf((float)y));
-    const int32_t r = x - q * y;
-    if ((r < 0 ? -r : r) >= (y < 0 ? -y : y)) q += (x ^ y) < 0 ? -1 : 1;
-    return q;
+    const float r = __builtin_amdgcn_rcpf((float)y) * (1.0f + 0x1p-20f);
+    return (int32_t)((float)x * r);
 #else
     return x / y;
 #endif
@@ -223,6 +222,11 @@ PCA_HD void ps_set_reg(PackedState &s, uint32_t k, uint32_t v)
         s.w[3] = (s.w[3] & keep) | nv;
 }
 
+// FLAGS = false: the flags byte is left as it was (for a state whose flags
+// nothing reads: the last kernel position holds no BR, and its value-writing
+// instructions do not read flags); registers, resume, frozen and exec are
+// stepped exactly.
+template <bool FLAGS = true>
 PCA_HD void step_packed(PackedState &s, uint32_t insn, uint32_t pos)
 {
     const uint32_t w4 = s.w[4];
@@ -237,7 +241,7 @@ PCA_HD void step_packed(PackedState &s, uint32_t insn, uint32_t pos)
         return;
     }
     if (op == OP_CMP) {
-        s.w[4] = (nw4 & ~0xFFu) | flags_sub(ps_reg(s, rd), ps_reg(s, rs));
+        s.w[4] = FLAGS ? (nw4 & ~0xFFu) | flags_sub(ps_reg(s, rd), ps_reg(s, rs)) : nw4;
         return;
     }
     const uint32_t a = ps_reg(s, op == OP_ADDI ? rd : rs), b = ps_reg(s, rt);
@@ -250,7 +254,7 @@ PCA_HD void step_packed(PackedState &s, uint32_t insn, uint32_t pos)
     }
     ps_set_reg(s, rd, v);
     s.w[0] &= 0xFFFF0000u;   // r0 stays 0
-    s.w[4] = (nw4 & ~0xFFu) | (f & 0xFFu);
+    s.w[4] = FLAGS ? (nw4 & ~0xFFu) | (f & 0xFFu) : nw4;
 }
 
 // Live mask after an instruction (tools/enumerate.c:427-433).

@@ -64,6 +64,21 @@ __device__ __forceinline__ unsigned long long wave_sum(unsigned long long v)
     return v;
 }
 
+// Inclusive prefix sum over the wave's 64 lanes in six DPP adds: shifts by
+// 1, 2, 4 and 8 lanes within each 16-lane row (lanes shifted in from outside
+// the row read 0), then lane 15's row total added to rows 1 and 3 and lane
+// 31's running total to rows 2 and 3.  Every lane must be active.
+__device__ __forceinline__ uint32_t wave_incl_scan32(uint32_t v)
+{
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x111, 0xF, 0xF, true);   // row_shr:1
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x112, 0xF, 0xF, true);   // row_shr:2
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x114, 0xF, 0xF, true);   // row_shr:4
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x118, 0xF, 0xF, true);   // row_shr:8
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return v;
+}
+
 // Replays a prefix from the root into an LDS level (lanes = tests).  The
 // state is stepped in place in LDS so that the register index chosen by the
 // instruction addresses LDS, not a private array.
@@ -202,42 +217,73 @@ struct ParentSummary {
     uint32_t dead;     // some test already faulted or cannot fit max_cycles
 };
 
-// Lanes = tests.  Also leaves per test the mask of registers equal to the
-// target (regmatch) in LDS for the value-group scan.
-// my_tgt: this lane's target (lane = test), held in a register by the caller;
-// st: this lane's parent state (valid when act).
+// Lanes = tests.  my_tgt: this lane's target (lane = test), held in a
+// register by the caller; st: this lane's parent state (valid when act).
+// Each register half is compared with the target by a 16-bit test of the
+// XOR (low half) or its magnitude (high half); the ballots are the summary.
 __device__ __forceinline__ ParentSummary summarize_state(const DevTask &task, const PackedState &st, bool act,
-                                                         uint32_t pos, int lane, uint32_t my_tgt, uint8_t *regmatch)
+                                                         uint32_t pos, uint32_t my_tgt)
 {
-    // each register pair against the target, both halves at once
     const uint32_t tt = my_tgt | (my_tgt << 16);
     ParentSummary s;
-    uint32_t pm = 0, rm = 0;
+    uint32_t pm = 0;
 #pragma unroll
-    for (int r = 1; r < 8; r++) {
-        const uint32_t d = st.w[r >> 1] ^ tt;
-        const bool eq = act && ((r & 1) ? (d >> 16) == 0u : (d & 0xFFFFu) == 0u);
-        rm |= (eq ? 1u : 0u) << r;
-        pm |= (__ballot(act && !eq) == 0ull ? 1u : 0u) << r;
+    for (int i = 0; i < 4; i++) {
+        const uint32_t d = st.w[i] ^ tt;
+        if (i > 0) pm |= (__ballot(act && (d & 0xFFFFu) != 0u) == 0ull ? 1u : 0u) << (2 * i);
+        pm |= (__ballot(act && d > 0xFFFFu) == 0ull ? 1u : 0u) << (2 * i + 1);
     }
-    if (act) regmatch[lane] = (uint8_t)rm;
     s.pm = pm;
-    // skipped by a pending BR, or the VM already stopped
-    const uint32_t frozen = (st.w[4] >> 16) & 0xFFu, resume = (st.w[4] >> 8) & 0xFFu, exec = st.w[4] >> 24;
-    const bool noexec = act && (frozen || resume > pos);
+    // skipped by a pending BR (resume past pos) or the VM already stopped
+    // (frozen, the byte above resume: any nonzero value exceeds pos)
+    const uint32_t w4 = st.w[4];
+    const bool noexec = act && ((w4 >> 8) & 0xFFFFu) > pos;
     s.skipm = __ballot(noexec);
-    const bool cyc = act && (exec + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
-    s.dead = __ballot(act && (frozen || cyc)) != 0ull ? 1u : 0u;
+    const bool cyc = act && ((w4 >> 24) + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
+    s.dead = __ballot(act && (((w4 >> 16) & 0xFFu) != 0u || cyc)) != 0ull ? 1u : 0u;
     return s;
 }
 
 __device__ __forceinline__ ParentSummary summarize_parent(const DevTask &task, const TestState *P, uint32_t pos,
-                                                          int lane, uint32_t my_tgt, uint8_t *regmatch)
+                                                          int lane, uint32_t my_tgt)
 {
     const bool act = lane < task.T;
     PackedState st = {};
     if (act) st = load_ps(&P[lane]);
-    return summarize_state(task, st, act, pos, lane, my_tgt, regmatch);
+    return summarize_state(task, st, act, pos, my_tgt);
+}
+
+// Registers r1..r7 (bits 1..7) of one test's state row equal to that test's
+// target tg; read from the row when a scan needs it (a test the last
+// instruction skips, a zero divisor), not kept per parent.
+__device__ __forceinline__ uint32_t row_regmatch(const TestState *row, uint32_t tg)
+{
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(row);
+    const uint32_t t2 = tg * 0x10001u;
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t d = q[i] ^ t2;
+        m |= ((d & 0xFFFFu) == 0u ? 1u : 0u) << (2 * i);
+        m |= (d <= 0xFFFFu ? 1u : 0u) << (2 * i + 1);
+    }
+    return m & 0xFEu;
+}
+
+// The same for a row every lane reads (a test position the whole wave is
+// at): its words read once into scalar registers, the compares scalar.
+__device__ __forceinline__ uint32_t row_regmatch_uni(const TestState *row, uint32_t tg)
+{
+    const uint32_t *q = reinterpret_cast<const uint32_t *>(row);
+    const uint32_t t2 = tg * 0x10001u;
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint32_t d = uni(q[i]) ^ t2;
+        m |= ((d & 0xFFFFu) == 0u ? 1u : 0u) << (2 * i);
+        m |= (d <= 0xFFFFu ? 1u : 0u) << (2 * i + 1);
+    }
+    return m & 0xFEu;
 }
 
 // The candidate indices of a scanned parent at positions s..D-2: a parent at
@@ -260,13 +306,16 @@ struct WaveLds {
     uint32_t queue[128];                  // pooled survivors
     uint32_t cq[128];                     // scan_parent's candidates
     uint32_t live[MAXLEV], idx[MAXLEV];   // the suffix walk: live masks, candidate indices
-    uint8_t srm[NSLOT][MAXT];             // per slot and test, registers equal to the target
     unsigned long long skip[NSLOT];       // per slot, tests the last instruction does not run on
     uint32_t meta[NSLOT][2];              // per slot, prefix index; live-mask index | dead << 8
     uint32_t path[NSLOT][MAXLEV];         // per slot, candidate indices at positions s..D-2
     unsigned long long wc[4];             // work counters (WaveCounters)
     uint32_t evl[64];                     // per lane, phase-A group evaluations
     uint32_t tst[MAXT][2];                // survivor-phase test statistics
+    // per lane, phase A at test position 2 (the only reordered position it
+    // runs, at most 3 tests): groups that reached it (low half), groups it
+    // ended (high half); folded into tst[2] when the wave leaves
+    unsigned long long pa2[64];
 };
 struct Pool {                      // wave-uniform bookkeeping
     uint32_t qn;                   // entries queued
@@ -318,7 +367,6 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
     }
     const uint64_t skl = skor ? sv.skip[s] : 0ull;
     const TestState *P = sv.slot[s];
-    const uint8_t *rmv = sv.srm[s];
     // survivor-phase reach / end counts from one drain in eight are enough
     // to order the tests, at an eighth of their cost
     const bool tst8 = tst && (pool.nd & 7u) == 0u;
@@ -329,7 +377,8 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
     bool any_div = __any(divmod && ok != 0u);
     uint32_t n_ev = 0;
     const uint32_t aimm2 = (aimm & 0xFFFFu) * 0x10001u;
-    for (int tt = t0; tt < T; tt++) {
+    for (int t_ = t0; t_ < T; t_++) {
+        int tt = (int)uni((uint32_t)t_);
         const uint64_t al = __ballot(ok != 0u);
         if (al == 0ull) break;
         const uint32_t before = (uint32_t)__popcll(al);
@@ -375,7 +424,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
                     tst[tt + 1][1] += mid - after;
                 }
             }
-            tt++;
+            t_ = tt + 1;
             continue;
         }
         const bool sk = (skl >> tt) & 1ull;
@@ -397,7 +446,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
         if (q == 12u) match = (tw >> (16 + bit)) & 1u;
         if (any_div && divmod && !sk) {
             if (y == 0u) {
-                ok &= rmv[tt];
+                ok &= row_regmatch(&sv.slot[s][tt], tg);
                 flt = true;
                 match = true;
             } else {
@@ -407,7 +456,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
             }
         }
         if (sk)
-            ok &= rmv[tt];
+            ok &= row_regmatch(&sv.slot[s][tt], tg);
         else if (!match)
             ok = 0;
         if (tst8) {
@@ -453,7 +502,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
 __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const CandDev &cd,
                                                  const TestState *P,
                                                  uint32_t live, uint32_t prefix_idx, const JPath &jpath,
-                                                 int lane, WaveCounters &wc, const uint8_t *regmatch,
+                                                 int lane, WaveCounters &wc,
                                                  WaveLds &sv, Pool &pool, uint32_t (*tst)[2],
                                                  const ParentSummary &ps)
 {
@@ -485,9 +534,8 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         // the rd values (rm_and), groups left without one are not told
         // apart here, so such parents are not counted further
         if (rm_and == 0xFFu) ev += nal;
-        const unsigned long long alive = (tst && t >= 2) ? wave_sum((unsigned long long)nal) : 0ull;
         if ((ps.skipm >> t) & 1ull) {
-            const uint32_t rmt = uni(regmatch[t]);
+            const uint32_t rmt = row_regmatch_uni(&P[t], task.tgt[t]);
             okd &= rmt;
             okm &= rmt;
             rm_and &= rmt;
@@ -510,31 +558,34 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         keep |= (((x + addi_imm) & 0xFFFFu) == tg ? 1u : 0u) << 11;
         kl &= keep;
         if (__any(okd != 0u || okm != 0u)) {
+            // a zero divisor stops the VM and rd keeps its value: the test
+            // passes for the rd values whose register already matches (the
+            // test's match mask, the same for every lane); no branch on it
             const bool z = y == 0u;
-            if (z) {
-                const uint32_t rmt = regmatch[t];
-                okd &= rmt;
-                okm &= rmt;
-                fault = fault || okd != 0u || okm != 0u;
-            } else {
-                const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
-                const int32_t qt = sdiv16_q(sx, sy);
-                if (((uint32_t)qt & 0xFFFFu) != tg) okd = 0;
-                if (((uint32_t)(sx - qt * sy) & 0xFFFFu) != tg) okm = 0;
-            }
+            const uint32_t rmt = __any(z) ? row_regmatch_uni(&P[t], tg) : 0u;
+            const int32_t sx = (int16_t)(uint16_t)x, sy = z ? 1 : (int32_t)(int16_t)(uint16_t)y;
+            const int32_t qt = sdiv16_q(sx, sy);
+            const bool dq = ((uint32_t)qt & 0xFFFFu) == tg, dr = ((uint32_t)(sx - qt * sy) & 0xFFFFu) == tg;
+            okd &= z ? rmt : (dq ? 0xFFu : 0u);
+            okm &= z ? rmt : (dr ? 0xFFu : 0u);
+            fault = fault || (z && (okd != 0u || okm != 0u));
         }
-        if (tst && t >= 2) {   // positions 2.. are reordered by these counts
-            const unsigned long long after =
-                wave_sum((unsigned long long)((uint32_t)__popc(kl) + (okd != 0u ? 1u : 0u) + (okm != 0u ? 1u : 0u)));
-            if (lane == 0) {
-                tst[t][0] += (uint32_t)alive;
-                tst[t][1] += (uint32_t)(alive - after);
-            }
+        if (tst && t == 2) {   // position 2 is reordered by these counts (phase A runs at most 3 tests)
+            const uint32_t left = (uint32_t)__popc(kl) + (okd != 0u ? 1u : 0u) + (okm != 0u ? 1u : 0u);
+            __hip_atomic_fetch_add(&sv.pa2[lane], (unsigned long long)nal | ((unsigned long long)(nal - left) << 32),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
         }
     }
     if (ev) __hip_atomic_fetch_add(&sv.evl[lane], ev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
     if (rm_and == 0u) return;   // no rd value is possible for any group
-    if (!__any(kl != 0u || okd != 0u || okm != 0u)) return;
+    // the kinds with a group still matching on this lane, and where this
+    // lane's entries go among the parent's survivors (lane order, then kind)
+    const uint32_t sm = kl | (okd != 0u ? 1u << (OP_DIV - OP_ADD) : 0u) | (okm != 0u ? 1u << (OP_MOD - OP_ADD) : 0u);
+    const uint32_t c = (uint32_t)__popc(sm);
+    const uint32_t incl = wave_incl_scan32(c);
+    const uint32_t total = uni(__builtin_amdgcn_readlane(incl, 63));
+    if (total == 0u) return;
+    const uint32_t excl = incl - c;
     // phase B is pooled: the survivors wait in the wave's queue with those of
     // earlier parents (whose states stay in their slots) and are tested 64
     // at a time by pool_drain.  Queue entry: kind | bit << 4 | rd mask << 10
@@ -547,28 +598,30 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     }
     if (lane + 1 < a.k) sv.path[cur][lane] = jp_at(jpath, lane);
     const uint32_t next = (cur + 1u) & (NSLOT - 1u);
-    const uint64_t lt = (1ull << lane) - 1ull;
-    uint32_t done = 0;   // kinds pushed, in kind order
+    uint32_t pushed = 0;   // this parent's entries queued so far (the first `pushed` in lane order)
     for (;;) {
-#pragma unroll
-        for (int q = 0; q < NBKINDS; q++) {
-            const uint64_t bits = __ballot(q == OP_DIV - OP_ADD   ? okd != 0u
-                                           : q == OP_MOD - OP_ADD ? okm != 0u
-                                                                  : ((kl >> q) & 1u) != 0u);
-            const uint32_t c = (uint32_t)__popcll(bits);
-            if (done == (uint32_t)q && pool.qn + c <= 128u) {
-                if (bits >> lane & 1ull) {
-                    const uint32_t okq = q == OP_DIV - OP_ADD ? okd : (q == OP_MOD - OP_ADD ? okm : rm_and);
-                    const bool fq = (q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD) && fault;
-                    sv.queue[pool.qn + (uint32_t)__popcll(bits & lt)] =
-                        (uint32_t)q | ((uint32_t)lane << 4) | (okq << 10) | (fq ? 1u << 18 : 0u) | (cur << 19);
+        if (pushed < total) {
+            // as many entries as the queue has room for
+            const uint32_t take = min(128u - pool.qn, total - pushed);
+            if (c != 0u && excl < pushed + take && excl + c > pushed) {
+                uint32_t mk = sm, g = excl;
+                while (mk) {
+                    const uint32_t q = (uint32_t)__builtin_ctz(mk);
+                    mk &= mk - 1u;
+                    if (g >= pushed && g < pushed + take) {
+                        const bool dm = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
+                        const uint32_t okq = q == OP_DIV - OP_ADD ? okd : (dm ? okm : rm_and);
+                        sv.queue[pool.qn + (g - pushed)] =
+                            q | ((uint32_t)lane << 4) | (okq << 10) | ((dm && fault) ? 1u << 18 : 0u) | (cur << 19);
+                    }
+                    g++;
                 }
-                pool.qn += c;
-                pool.pend += c << (8u * cur);
-                done = (uint32_t)q + 1u;
             }
+            pool.qn += take;
+            pool.pend += take << (8u * cur);
+            pushed += take;
         }
-        const bool all = done == (uint32_t)NBKINDS;
+        const bool all = pushed == total;
         // drain 64 when that many wait; once this parent is queued, drain
         // the rest too if the next parent's slot still has entries
         uint32_t cnt = 0;
@@ -733,7 +786,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
 }
 
 // k == 0: the prefix is the whole kernel; check its registers directly.
-__device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestState *P, uint32_t live,
+__device__ __forceinline__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestState *P, uint32_t live,
                             uint32_t prefix_idx, int lane, WaveCounters &wc)
 {
     const int T = task.T;
@@ -772,7 +825,7 @@ __device__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestSt
 // scanned parents (max(lch.levels, 2) - 1 of them); the scanned parents
 // themselves (level k - 1) sit in the NSLOT slots of the wave's WaveLds,
 // where they stay while their survivors wait in the pooled queue.
-template <bool MULTI>
+template <bool MULTI, bool K23>
 __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs &a1, const DevTask &task1)
 {
     extern __shared__ TestState s_dyn[];
@@ -784,6 +837,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     for (int i = lane; i < MAXT * 2; i += 64) (&sv.tst[0][0])[i] = 0u;
     if (lane < 4) sv.wc[lane] = 0ull;
     sv.evl[lane] = 0u;
+    sv.pa2[lane] = 0ull;
     uint32_t (*tst)[2] = (!MULTI && lch.tstat) ? sv.tst : nullptr;
     // per-mask tables live in LDS: no global round trip per parent
     for (int i = threadIdx.x; i < 128; i += SCAN_WG) {
@@ -888,15 +942,17 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
         // the scanned parents (level k - 1) sit in the slots: for k == 1 the
         // prefix itself, replayed into the current slot every time
         auto lvl = [&](int L) -> TestState * { return L == k - 1 ? sv.slot[pool.cur] : lev(L); };
-        if (k == 1 || pi != cur_i) {
-            replay_prefix(task, P, a.s, k == 1 ? lvl(0) : lev(0), lane);
+        if ((!K23 && k == 1) || pi != cur_i) {
+            replay_prefix(task, P, a.s, (!K23 && k == 1) ? lvl(0) : lev(0), lane);
             cur_i = k == 1 ? 0xFFFFFFFFu : pi;
             cur_js = 0xFFFFFFFFu;
             wave_sync();
         }
-        if (k == 0) {
-            leaf_parent(task, a, lev(0), live0, pi, lane, wc);
-            continue;
+        if constexpr (!K23) {
+            if (k == 0) {
+                leaf_parent(task, a, lev(0), live0, pi, lane, wc);
+                continue;
+            }
         }
         if (lane == 0) sv.live[0] = live0;
         // level of the scanned parents' first level: 0 (the prefix itself,
@@ -921,22 +977,46 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
             // the block's candidate words, one per lane, in one coalesced read
             cw = (uint32_t)lane < nb ? cand_at(cand, lvp, mfp, (int)j0 + lane) : 0u;
         }
+        if (K23 || (MULTI && (k == 2 || k == 3))) {
+            // every child of the block is a scanned parent (kernel position
+            // D-2): stepped from the block's parent state, held in registers,
+            // into the current slot (without flags: nothing at the last
+            // position reads them), summarised from registers, scanned
+            const bool act = lane < task.T;
+            PackedState pl = {};
+            if (act) pl = load_ps(&lev(Lu - 1)[lane]);
+            const uint32_t cpos = (uint32_t)(a.D - 2);
+            for (uint32_t ci = 0; ci < nb; ci++) {
+                const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
+                TestState *PP = sv.slot[pool.cur];
+                PackedState st = pl;
+                if (act) {
+                    step_packed<false>(st, c, cpos);
+                    store_ps(&PP[lane], st);
+                }
+                const ParentSummary ps = summarize_state(task, st, act, (uint32_t)(a.D - 1), my_tgt);
+                const uint32_t lvL = live_after(lvp, c);
+                JPath jpath;
+                jpath.lds = nullptr;
+                jpath.j0 = k == 3 ? js : j0 + ci;
+                jpath.j1 = j0 + ci;
+                wave_sync();
+                if (ps.pm == 0u)
+                    scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
+                else
+                    scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
+                wave_sync();
+            }
+            if (lane == 0) wc.c[3] += nb;
+            continue;
+        }
+        if constexpr (K23) continue;
+        // the general walk (every k; the kernels above take k = 2 and 3):
+        // depth first from the block's children down to the scanned parents
         for (uint32_t ci = 0; ci < nb; ci++) {
-            // when the child is itself the scanned parent it is stepped and
-            // summarised from registers (stored once, for the scan)
-            ParentSummary ps_child = {};
-            const bool fused = k >= 2 && Lu == k - 1;
             if (k >= 2) {
                 const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
-                const bool act = lane < task.T;
-                PackedState st = {};
-                if (act) {
-                    st = load_ps(&lev(Lu - 1)[lane]);
-                    step_packed(st, c, (uint32_t)(a.s + Lu - 1));
-                    store_ps(&lvl(Lu)[lane], st);
-                }
-                if (fused)
-                    ps_child = summarize_state(task, st, act, (uint32_t)(a.D - 1), lane, my_tgt, sv.srm[pool.cur]);
+                step_level(task, lev(Lu - 1), lvl(Lu), c, a.s + Lu - 1, lane);
                 if (lane == 0) {
                     sv.live[Lu] = live_after(lvp, c);
                     sv.idx[Lu] = j0 + ci;
@@ -954,14 +1034,11 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                     {
                         const TestState *PP = lvl(L);
                         const uint32_t lvL = uni(sv.live[L]);
-                        const ParentSummary ps =
-                            fused ? ps_child
-                                  : summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt, sv.srm[pool.cur]);
+                        const ParentSummary ps = summarize_parent(task, PP, (uint32_t)(a.D - 1), lane, my_tgt);
                         if (lane == 0) wc.c[3]++;
                         wave_sync();
                         if (ps.pm == 0u)
-                            scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.srm[pool.cur], sv, pool,
-                                             tst, ps);
+                            scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
                         else
                             scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
                     }
@@ -999,6 +1076,14 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     wave_sync();
     const unsigned long long evs = wave_sum((unsigned long long)sv.evl[lane]);
     if (lane == 0) sv.wc[2] += evs;
+    if (tst) {
+        const unsigned long long p2 = sv.pa2[lane];
+        const unsigned long long reach = wave_sum(p2 & 0xFFFFFFFFull), ended = wave_sum(p2 >> 32);
+        if (lane == 0) {
+            tst[2][0] += (uint32_t)reach;
+            tst[2][1] += (uint32_t)ended;
+        }
+    }
     wave_sync();
     if (lane < 4) atomicAdd(&lch.counters[lane], sv.wc[lane]);
     if (!MULTI && lch.tstat) {
@@ -1020,12 +1105,19 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
 // (188, none): 4.70 s of kernel time against 4.97 and 4.80.
 __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false>(lch, a1, task1);
+    scan_body<false, false>(lch, a1, task1);
+}
+
+// One-item launches of 2- and 3-instruction suffixes (the block's children
+// are the scanned parents): the same body without the general walk.
+__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_k23(ScanLaunch lch, ScanArgs a1, DevTask task1)
+{
+    scan_body<false, true>(lch, a1, task1);
 }
 
 __global__ __launch_bounds__(SCAN_WG, 3) void scan_kernel_multi(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<true>(lch, a1, task1);
+    scan_body<true, false>(lch, a1, task1);
 }
 
 // The kernel argument of a one-item launch: its work item with the launch's
@@ -1602,6 +1694,9 @@ hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
 {
     const size_t lds = scan_dyn_lds(levels);
     if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_multi, SCAN_WG, lds);
+    // a one-item launch's levels are its suffix length k (k >= 1)
+    if (levels == 2u || levels == 3u)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_k23, SCAN_WG, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_one, SCAN_WG, lds);
 }
 
@@ -1614,7 +1709,7 @@ void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &ta
     args->ptrs[0] = &args->l;
     args->ptrs[1] = &args->a1;
     args->ptrs[2] = &args->task;
-    p->func = reinterpret_cast<void *>(scan_kernel_one);
+    p->func = reinterpret_cast<void *>((it.k == 2 || it.k == 3) ? scan_kernel_k23 : scan_kernel_one);
     p->gridDim = dim3((unsigned)grid);
     p->blockDim = dim3(SCAN_WG);
     p->sharedMemBytes = (unsigned)scan_dyn_lds(l.levels);
@@ -1653,7 +1748,9 @@ void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask 
 {
     const size_t lds = scan_dyn_lds(l.levels);
     const ScanArgs a1 = one_item_args(l, host_items[0]);
-    if (l.n_items == 1)
+    if (l.n_items == 1 && (host_items[0].k == 2 || host_items[0].k == 3))
+        hipLaunchKernelGGL(scan_kernel_k23, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
+    else if (l.n_items == 1)
         hipLaunchKernelGGL(scan_kernel_one, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
     else
         hipLaunchKernelGGL(scan_kernel_multi, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);

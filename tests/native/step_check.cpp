@@ -43,11 +43,20 @@ int main()
                     TestState a = s;
                     PackedState p;
                     std::memcpy(&p, &s, sizeof(p));
+                    PackedState pn = p;
                     step_test(a, insn, pos);
                     step_packed(p, insn, pos);
+                    step_packed<false>(pn, insn, pos);
                     cases++;
                     if (std::memcmp(&a, &p, sizeof(p)) != 0) {
                         std::printf("mismatch insn %04x pos %u trial %d\n", insn, pos, trial);
+                        return 1;
+                    }
+                    // the flag-free form: the same state with the flags byte untouched
+                    TestState an = a;
+                    an.flags = s.flags;
+                    if (std::memcmp(&an, &pn, sizeof(pn)) != 0) {
+                        std::printf("flag-free mismatch insn %04x pos %u trial %d\n", insn, pos, trial);
                         return 1;
                     }
                 }
