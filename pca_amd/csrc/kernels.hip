@@ -30,6 +30,10 @@ __device__ __forceinline__ void wave_sync()
 }
 
 __device__ __forceinline__ uint32_t uni(uint32_t v) { return __builtin_amdgcn_readfirstlane(v); }
+// the wave's lanes where p holds, from the compare itself (no widening of
+// the predicate to an integer and second compare, as __ballot(int) does)
+__device__ __forceinline__ uint64_t bal(bool p) { return __builtin_amdgcn_ballot_w64(p); }
+__device__ __forceinline__ bool any_lane(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0ull; }
 // lane 0's value, known uniform to the compiler (SGPRs, uniform branches)
 __device__ __forceinline__ unsigned long long uni64(unsigned long long v)
 {
@@ -162,7 +166,7 @@ __device__ __forceinline__ void test_loop(const DevTask &task, const TestState *
     const int T = task.T;
     bool need = mine;
     for (int t = 0; t < T; t++) {
-        if (!__any(need)) break;
+        if (!any_lane(need)) break;
         if (need) {
             uint32_t v;
             if ((skipm >> t) & 1ull) {
@@ -224,23 +228,29 @@ struct ParentSummary {
 __device__ __forceinline__ ParentSummary summarize_state(const DevTask &task, const PackedState &st, bool act,
                                                          uint32_t pos, uint32_t my_tgt)
 {
+    // every ballot is of one compare; the active lanes are ANDed in as a
+    // mask (a ballot of a combined predicate makes the compiler rebuild it
+    // in a vector register and compare again)
+    const uint64_t am = bal(act);
     const uint32_t tt = my_tgt | (my_tgt << 16);
     ParentSummary s;
     uint32_t pm = 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         const uint32_t d = st.w[i] ^ tt;
-        if (i > 0) pm |= (__ballot(act && (d & 0xFFFFu) != 0u) == 0ull ? 1u : 0u) << (2 * i);
-        pm |= (__ballot(act && d > 0xFFFFu) == 0ull ? 1u : 0u) << (2 * i + 1);
+        if (i > 0) pm |= ((bal((d & 0xFFFFu) != 0u) & am) == 0ull ? 1u : 0u) << (2 * i);
+        pm |= ((bal(d > 0xFFFFu) & am) == 0ull ? 1u : 0u) << (2 * i + 1);
     }
     s.pm = pm;
     // skipped by a pending BR (resume past pos) or the VM already stopped
     // (frozen, the byte above resume: any nonzero value exceeds pos)
-    const uint32_t w4 = st.w[4];
-    const bool noexec = act && ((w4 >> 8) & 0xFFFFu) > pos;
-    s.skipm = __ballot(noexec);
-    const bool cyc = act && ((w4 >> 24) + (noexec ? 2u : 3u) > (uint32_t)task.max_cycles);
-    s.dead = __ballot(act && (((w4 >> 16) & 0xFFu) != 0u || cyc)) != 0ull ? 1u : 0u;
+    const uint32_t w4 = st.w[4], ex = w4 >> 24, mc = (uint32_t)task.max_cycles;
+    const uint64_t nm = bal(((w4 >> 8) & 0xFFFFu) > pos) & am;
+    s.skipm = nm;
+    // out of cycles: the last instruction, OUT and HLT (OUT and HLT alone
+    // on a skipped test) would pass max_cycles
+    const uint64_t cyc = bal(ex + 2u > mc) | (bal(ex + 3u > mc) & ~nm);
+    s.dead = ((bal((w4 & 0xFF0000u) != 0u) | cyc) & am) != 0ull ? 1u : 0u;
     return s;
 }
 
@@ -361,7 +371,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
     uint64_t skor = 0;
 #pragma unroll
     for (uint32_t z = 0; z < NSLOT; z++) {
-        const uint64_t b = __ballot(valid && s == z);
+        const uint64_t b = bal(valid && s == z);
         if (b) skor |= uni64(sv.skip[z]);
         pool.pend -= (uint32_t)__popcll(b) << (8u * z);
     }
@@ -374,16 +384,16 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
     const uint32_t xa = bit >> 3, yb = bit & 7u, i = bit & 7u;
     const uint32_t aimm = (i & 1u) ? (i >> 1) + 1u : 0x10000u - ((i >> 1) + 1u);
     const bool divmod = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
-    bool any_div = __any(divmod && ok != 0u);
+    bool any_div = any_lane(divmod && ok != 0u);
     uint32_t n_ev = 0;
     const uint32_t aimm2 = (aimm & 0xFFFFu) * 0x10001u;
     for (int t_ = t0; t_ < T; t_++) {
         int tt = (int)uni((uint32_t)t_);
-        const uint64_t al = __ballot(ok != 0u);
+        const uint64_t al = bal(ok != 0u);
         if (al == 0ull) break;
         const uint32_t before = (uint32_t)__popcll(al);
         n_ev += before;
-        if (any_div) any_div = __any(divmod && ok != 0u);
+        if (any_div) any_div = any_lane(divmod && ok != 0u);
         if (!any_div && tt + 1 < T && ((skor >> tt) & 3ull) == 0ull) {
             // tests tt and tt+1 at once: each 16-bit half of a register
             // holds one test's operand, and the packed 16-bit ops compute
@@ -412,11 +422,11 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
                 m1 = (tw1 >> (16 + bit)) & 1u;
             }
             // the second test is evaluated for the lanes the first kept
-            const uint32_t mid = (uint32_t)__popcll(__ballot(ok != 0u && m0));
+            const uint32_t mid = (uint32_t)__popcll(bal(ok != 0u && m0));
             n_ev += mid;
             if (!(m0 && m1)) ok = 0;
             if (tst8) {
-                const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
+                const uint32_t after = (uint32_t)__popcll(bal(ok != 0u));
                 if (lane == 0) {
                     tst[tt][0] += before;
                     tst[tt][1] += before - mid;
@@ -460,7 +470,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
         else if (!match)
             ok = 0;
         if (tst8) {
-            const uint32_t after = (uint32_t)__popcll(__ballot(ok != 0u));
+            const uint32_t after = (uint32_t)__popcll(bal(ok != 0u));
             if (lane == 0) {
                 tst[tt][0] += before;
                 tst[tt][1] += before - after;
@@ -529,7 +539,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
     int t = 0;
     for (; t < TA; t++) {
         const uint32_t nal = (uint32_t)__popc(kl) + (okd != 0u ? 1u : 0u) + (okm != 0u ? 1u : 0u);
-        if (!__any(nal != 0u)) break;
+        if (!any_lane(nal != 0u)) break;
         // group evaluations of this test; once a skipped test has narrowed
         // the rd values (rm_and), groups left without one are not told
         // apart here, so such parents are not counted further
@@ -557,12 +567,12 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         keep |= (((uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) & 0xFFFFu) == tg ? 1u : 0u) << 10;
         keep |= (((x + addi_imm) & 0xFFFFu) == tg ? 1u : 0u) << 11;
         kl &= keep;
-        if (__any(okd != 0u || okm != 0u)) {
+        if (any_lane((okd | okm) != 0u)) {
             // a zero divisor stops the VM and rd keeps its value: the test
             // passes for the rd values whose register already matches (the
             // test's match mask, the same for every lane); no branch on it
             const bool z = y == 0u;
-            const uint32_t rmt = __any(z) ? row_regmatch_uni(&P[t], tg) : 0u;
+            const uint32_t rmt = any_lane(z) ? row_regmatch_uni(&P[t], tg) : 0u;
             const int32_t sx = (int16_t)(uint16_t)x, sy = z ? 1 : (int32_t)(int16_t)(uint16_t)y;
             const int32_t qt = sdiv16_q(sx, sy);
             const bool dq = ((uint32_t)qt & 0xFFFFu) == tg, dr = ((uint32_t)(sx - qt * sy) & 0xFFFFu) == tg;
@@ -676,7 +686,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
         bool fault = false;
         bool pending = valid;
         for (;;) {
-            const uint64_t pb = __ballot(pending);
+            const uint64_t pb = bal(pending);
             if (pb == 0ull) break;
             const uint32_t op0 = __builtin_amdgcn_readlane(op, (int)__ffsll((long long)pb) - 1);
             const bool mine = pending && op == op0;
@@ -728,7 +738,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
             bool rd_ok = true;
             bool pending = true;
             for (;;) {
-                const uint64_t pb = __ballot(pending);
+                const uint64_t pb = bal(pending);
                 if (pb == 0ull) break;
                 const uint32_t op0 = __builtin_amdgcn_readlane(op, (int)__ffsll((long long)pb) - 1);
                 const bool mine = pending && op == op0;
@@ -759,7 +769,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
             alive = true;   // another register already matches (or T < 2): full test
         }
         // push survivors to the queue in lane order
-        const uint64_t sb = __ballot(alive);
+        const uint64_t sb = bal(alive);
         if (alive) queue[qn + (uint32_t)__popcll(sb & ((1ull << lane) - 1ull))] = (uint16_t)j;
         qn += (uint32_t)__popcll(sb);
         if (qn >= 64) {
@@ -795,9 +805,9 @@ __device__ __forceinline__ void leaf_parent(const DevTask &task, const ScanArgs 
     if (act) st = P[lane];
     const uint32_t tgt = act ? task.tgt[lane] : 0u;
     uint32_t pm = 0;
-    for (int r = 1; r < 8; r++) pm |= (__ballot(act && st.r[r] != tgt) == 0ull ? 1u : 0u) << r;
+    for (int r = 1; r < 8; r++) pm |= (bal(act && st.r[r] != tgt) == 0ull ? 1u : 0u) << r;
     const bool cyc = act && ((uint32_t)st.exec + 2u > (uint32_t)task.max_cycles);
-    const uint32_t dead = __ballot(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
+    const uint32_t dead = bal(act && (st.frozen || cyc)) != 0ull ? 1u : 0u;
     if (lane == 0) {
         wc.c[3]++;
         wc.c[0] += (unsigned)__popc(live & 0xFEu);
@@ -1353,7 +1363,7 @@ __global__ __launch_bounds__(SCAN_WG) void expand_survive_kernel(DevTask task, E
                     }
                 }
             }
-            const unsigned long long b = __ballot(keep);
+            const unsigned long long b = bal(keep);
             kept += (unsigned long long)__popcll(b);
             const uint64_t bit0 = base + (uint64_t)j0;
             // 64 consecutive children -> up to two 32-bit words
@@ -1393,7 +1403,7 @@ __global__ __launch_bounds__(SCAN_WG) void expand_compact_kernel(ExpandArgs a)
                 const uint64_t bit = base + (uint64_t)j;
                 keep = (a.keep_bits[bit >> 5] >> (bit & 31u)) & 1u;
             }
-            const unsigned long long b = __ballot(keep);
+            const unsigned long long b = bal(keep);
             if (keep) {
                 const unsigned long long below = b & ((1ull << lane) - 1ull);
                 pca_prefix_t c = P;
