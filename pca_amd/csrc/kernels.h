@@ -232,6 +232,9 @@ struct ZeroNodeArgs {
 void zero_node_params(uint32_t *const ptr[4], const uint32_t n[4], ZeroNodeArgs *args, hipKernelNodeParams *p);
 
 void launch_table_init(unsigned long long *table, uint64_t cap, hipStream_t st);
+// The frontier and match kernels, for their own occupancy queries.
+enum class AuxKernel : int { ExpandInsert = 0, ExpandSurvive, ExpandCompact, Match, Count };
+hipError_t aux_occupancy(AuxKernel which, int *blocks_per_cu);
 // LDS bytes of a scan launch with `levels` state levels per wave
 size_t scan_dyn_lds(uint32_t levels);
 hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu);

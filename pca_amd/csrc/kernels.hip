@@ -1710,6 +1710,22 @@ hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_one, SCAN_WG, lds);
 }
 
+hipError_t aux_occupancy(AuxKernel which, int *blocks_per_cu)
+{
+    switch (which) {
+    case AuxKernel::ExpandInsert:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, expand_insert_kernel, SCAN_WG, 0);
+    case AuxKernel::ExpandSurvive:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, expand_survive_kernel, SCAN_WG, 0);
+    case AuxKernel::ExpandCompact:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, expand_compact_kernel, SCAN_WG, 0);
+    case AuxKernel::Match:
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, match_kernel, SCAN_WG, 0);
+    default:
+        return hipErrorInvalidValue;
+    }
+}
+
 void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &task, int grid, ScanNodeArgs *args,
                       hipKernelNodeParams *p)
 {

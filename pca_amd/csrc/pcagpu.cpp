@@ -165,6 +165,22 @@ int grid_for(pcagpu *h, uint64_t units, bool multi, uint32_t levels)
     return (int)std::max<uint64_t>(wgs, 1);
 }
 
+int grid_for_aux(pcagpu *h, uint64_t units, AuxKernel which)
+{
+    uint32_t wpc = h->waves_per_cu;
+    if (!wpc) {
+        uint32_t &c = h->aux_wpc[(int)which];
+        if (!c) {
+            int b = 0;
+            c = (aux_occupancy(which, &b) == hipSuccess && b > 0) ? (uint32_t)b * SCAN_WAVES : 16u;
+        }
+        wpc = c;
+    }
+    const uint64_t waves = (uint64_t)h->cu_count * wpc;
+    const uint64_t wgs = (std::min<uint64_t>(units, waves) + SCAN_WAVES - 1) / SCAN_WAVES;
+    return (int)std::max<uint64_t>(wgs, 1);
+}
+
 // Scan tile autotuner.  The tile (children per block, tests run by wave
 // masks) is chosen per suffix length k: the first launches of that shape
 // each try one candidate on real work, and the one with the least kernel
@@ -997,7 +1013,7 @@ int run_match(pcagpu *h, int kk, const std::vector<uint64_t> &qh0, const std::ve
                 a.work = h->d_work.p;
                 HIP_TRY(hipMemsetAsync(h->d_match_count.p, 0, 4, h->stream));
                 HIP_TRY(hipMemsetAsync(h->d_work.p, 0, 8, h->stream));
-                launch_match(h->dt, a, grid_for(h, units), h->stream);
+                launch_match(h->dt, a, grid_for_aux(h, units, AuxKernel::Match), h->stream);
                 HIP_TRY(hipGetLastError());
                 uint32_t cnt = 0;
                 HIP_TRY_S(h->stream, hipMemcpyAsync(&cnt, h->d_match_count.p, 4, hipMemcpyDeviceToHost, h->stream));
@@ -1417,19 +1433,22 @@ int expand_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint6
     a.keep_count = h->d_keep_cnt.p;
     a.counters = h->d_counters.p + 4;
     a.work = h->d_work.p;
-    const int grid = grid_for(h, n);
+    const int grid = grid_for_aux(h, n, AuxKernel::ExpandCompact);
     // insert / survive: slice the child lists of a small frontier so that
     // every resident wave has work, and take runs of units from a large one
-    const uint64_t resident = (uint64_t)grid_for(h, UINT64_MAX / 2) * SCAN_WAVES;
+    // (the runs sized by the insert kernel's grid, which survive shares: the
+    // work counter's units must mean the same for both)
+    const uint64_t resident = (uint64_t)grid_for_aux(h, UINT64_MAX / 2, AuxKernel::ExpandInsert) * SCAN_WAVES;
     a.slices = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(resident / std::max<uint64_t>(n, 1), 1), 16);
-    const int grid_s = grid_for(h, n * a.slices);
+    const int grid_i = grid_for_aux(h, n * a.slices, AuxKernel::ExpandInsert);
+    const int grid_v = grid_for_aux(h, n * a.slices, AuxKernel::ExpandSurvive);
     a.grab = (uint32_t)std::min<uint64_t>(
-        std::max<uint64_t>(n * a.slices / ((uint64_t)grid_s * SCAN_WAVES * 8), 1), 32);
+        std::max<uint64_t>(n * a.slices / ((uint64_t)grid_i * SCAN_WAVES * 8), 1), 32);
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
-    launch_expand_insert(dt, a, grid_s, h->stream);
+    launch_expand_insert(dt, a, grid_i, h->stream);
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_keep_cnt.p, 0, (n + 1) * sizeof(uint64_t), h->stream));
-    launch_expand_survive(dt, a, grid_s, h->stream);
+    launch_expand_survive(dt, a, grid_v, h->stream);
     launch_exclusive_scan(h->d_keep_cnt.p, n, h->d_scan_tmp.p, h->stream);
     uint64_t kept = 0;
     unsigned long long ctr[2] = {0, 0};

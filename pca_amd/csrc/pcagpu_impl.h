@@ -115,6 +115,7 @@ struct pcagpu {
     uint32_t waves_per_cu = 0;           // opts.waves_per_cu, or 0: from the kernel's occupancy
     // resident scan waves per CU by [multi-item][state levels], 0 = not yet asked
     uint32_t scan_wpc[2][pca::MAXLEV + 1] = {};
+    uint32_t aux_wpc[(int)pca::AuxKernel::Count] = {};   // resident waves per CU of the frontier / match kernels
 
     const GrammarDev *d_grammar = nullptr;   // per device, shared, never freed
     const GrammarDev *d_grammar_bl = nullptr;   // branchless tables (leaf completion), same lifetime
@@ -229,6 +230,8 @@ constexpr uint64_t kAutoChildren = 1ull << 27;
 // items; levels: state levels per wave), or of the other persistent kernels
 // (levels 0: the one-item scan figure at 3 levels)
 int grid_for(pcagpu *h, uint64_t units, bool multi = false, uint32_t levels = 0);
+// persistent grid of a frontier or match kernel, from that kernel's own occupancy
+int grid_for_aux(pcagpu *h, uint64_t units, pca::AuxKernel which);
 size_t handle_device_bytes(const pcagpu *h);
 // queue prefixes for the depth without flushing (the caller scans them)
 int stage_prefixes(pcagpu *h, const pca_prefix_t *p, uint32_t n, uint64_t first_rank);

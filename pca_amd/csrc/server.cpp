@@ -122,6 +122,9 @@ struct pcagpu_server {
     // one step's prefixes and scan blocks of every item, copied in one go
     // (pinned host side), and the items' hit records, read back in one go
     PinnedBuf h_stage, h_hits;
+    // the multi-item launch's item list, staged here (pinned, outliving the
+    // launch) for its asynchronous upload
+    PinnedBuf h_items;
     DevBuf<unsigned char> d_stage;
     std::unique_ptr<HostPool> pool;   // staging and resolution threads
     std::vector<ScanItem> items;
@@ -420,6 +423,7 @@ extern "C" void pcagpu_server_close(pcagpu_server_t *s)
     if (s->h_ctl) (void)hipHostFree(s->h_ctl);
     s->h_stage.release();
     s->h_hits.release();
+    s->h_items.release();
     s->d_stage.release();
     if (s->ev0) (void)hipEventDestroy(s->ev0);
     if (s->ev1) (void)hipEventDestroy(s->ev1);
@@ -633,9 +637,15 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             const int grid = grid_for(in_launch[0]->h, n_units, n_items > 1, lv);
             l.grab = scan_grab(k, n_units, grid);   // runs of units when they are single prefixes
             l.pad = 0;
-            if (n_items > 1)
-                HIP_TRY(hipMemcpyAsync(s->d_items.p, items, n_items * sizeof(ScanItem), hipMemcpyHostToDevice,
-                                       s->stream));
+            if (n_items > 1) {
+                // never an asynchronous copy from pageable memory: the items
+                // are a local vector, which an error return would free while
+                // the copy is still queued
+                const size_t ib = n_items * sizeof(ScanItem);
+                if ((rc = s->h_items.ensure(ib))) return rc;
+                std::memcpy(s->h_items.p, items, ib);
+                HIP_TRY(hipMemcpyAsync(s->d_items.p, s->h_items.p, ib, hipMemcpyHostToDevice, s->stream));
+            }
             launch_scan(l, items, task0, grid, s->stream);
             HIP_TRY(hipGetLastError());
             s->st.launches++;
