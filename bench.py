@@ -516,7 +516,10 @@ def main():
                                 "achieved_lower_bound": st["suffixes"] * ((D - s) * 8 + 7) / kernel_s / 1e12,
                                 "frac_lower_bound": st["suffixes"] * ((D - s) * 8 + 7) / kernel_s / PEAK_INT_OPS,
                                 "unit": "Tint-ops/s"},
-        "kernel": {"name": "scan_kernel", "avg_launch_ms": avg_launch * 1e3, "launches": launches,
+        # one-item launches of 2- and 3-instruction suffixes run scan_kernel_k23,
+        # other suffix lengths scan_kernel_one
+        "kernel": {"name": "scan_kernel_k23" if D - s in (2, 3) else "scan_kernel_one",
+                   "avg_launch_ms": avg_launch * 1e3, "launches": launches,
                    "candidates": cand, "suffixes": st["suffixes"], "tests_evaluated": st["tests_evaluated"],
                    "parents": st["parents"], "raw_hits": st["raw_hits"],
                    "kernel_candidates_per_s": cand / kernel_s if kernel_s else None,
