@@ -1,4 +1,5 @@
-// host_pool.h -- a fixed set of host worker threads.
+// host_pool.h -- fixed sets of host worker threads: HostPool runs one
+// parallel loop at a time for its caller; JobQueue runs background jobs.
 //
 // run(n, f) calls f(0..n-1) spread over the workers and the calling thread
 // and returns when every call is done; one run at a time (the owner calls
@@ -91,6 +92,77 @@ private:
     size_t n_ = 0, pending_ = 0;
     uint64_t gen_ = 0;
     std::atomic<size_t> next_{0};
+    bool stop_ = false;
+};
+
+// Background jobs: submit() queues a job and returns at once; a free
+// worker runs it.  wait_all() returns when every job submitted so far has
+// finished.  Jobs do not submit jobs.  Each worker runs `init` once when it
+// starts.
+class JobQueue {
+public:
+    JobQueue(unsigned workers, std::function<void()> init = {}) : init_(std::move(init))
+    {
+        for (unsigned i = 0; i < workers; i++) threads_.emplace_back([this] { loop(); });
+    }
+    JobQueue(const JobQueue &) = delete;
+    JobQueue &operator=(const JobQueue &) = delete;
+    ~JobQueue()
+    {
+        wait_all();
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+        }
+        wake_.notify_all();
+        for (std::thread &t : threads_) t.join();
+    }
+    size_t workers() const { return threads_.size(); }
+
+    void submit(std::function<void()> job)
+    {
+        if (threads_.empty()) {   // no workers: run it here
+            job();
+            return;
+        }
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            jobs_.push_back(std::move(job));
+            in_flight_++;
+        }
+        wake_.notify_one();
+    }
+
+    void wait_all()
+    {
+        std::unique_lock<std::mutex> lk(mu_);
+        idle_.wait(lk, [this] { return in_flight_ == 0; });
+    }
+
+private:
+    void loop()
+    {
+        if (init_) init_();
+        for (;;) {
+            std::function<void()> job;
+            {
+                std::unique_lock<std::mutex> lk(mu_);
+                wake_.wait(lk, [&] { return stop_ || !jobs_.empty(); });
+                if (jobs_.empty()) return;   // stop_ and nothing left
+                job = std::move(jobs_.front());
+                jobs_.erase(jobs_.begin());
+            }
+            job();
+            std::lock_guard<std::mutex> lk(mu_);
+            if (--in_flight_ == 0) idle_.notify_all();
+        }
+    }
+    std::function<void()> init_;
+    std::vector<std::thread> threads_;
+    std::mutex mu_;
+    std::condition_variable wake_, idle_;
+    std::vector<std::function<void()>> jobs_;
+    size_t in_flight_ = 0;
     bool stop_ = false;
 };
 

@@ -1075,6 +1075,42 @@ void node_keys(const pcagpu *h, const std::vector<Node> &nodes, const std::vecto
     }
 }
 
+// A hash of each node's OEP key (the same per-test rows node_keys writes,
+// chained through h_row_hash), for nodes[idx[a..b)]: equal keys give equal
+// hashes, so only nodes of equal hash need their keys compared.
+void node_hashes(const pcagpu *h, const std::vector<Node> &nodes, const std::vector<size_t> &idx, size_t a,
+                 size_t b, uint64_t *out)
+{
+    TestState base[MAXT], st[MAXT];
+    uint32_t cached = 0xFFFFFFFFu;
+    for (size_t q = a; q < b; q++) {
+        const Path &x = nodes[idx[q]].path;
+        uint16_t insns[PCAGPU_MAX_KDEPTH];
+        const int n = node_insns(h, x, insns, nullptr);
+        if (x.prefix != cached) {
+            for (int t = 0; t < h->T; t++) {
+                base[t] = h->dt.root[t];
+                for (int i = 0; i < h->s; i++) step_test(base[t], insns[i], (uint32_t)i);
+            }
+            cached = x.prefix;
+        }
+        // a light multiply-xor chain over the rows' 20 bytes (a collision
+        // only costs a byte-for-byte compare), mixed once at the end
+        uint64_t hf = 0x243F6A8885A308D3ull;
+        for (int t = 0; t < h->T; t++) {
+            st[t] = base[t];
+            for (int i = h->s; i < n; i++) step_test(st[t], insns[i], (uint32_t)i);
+            uint64_t w[3];
+            std::memcpy(w, &st[t], 16);
+            w[2] = (uint64_t)st[t].flags;
+            hf = (hf ^ w[0]) * 0x9E3779B97F4A7C15ull;
+            hf = (hf ^ w[1]) * 0xC2B2AE3D27D4EB4Full;
+            hf = (hf ^ w[2]) * 0x165667B19E3779F9ull;
+        }
+        out[q - a] = h_mix64(hf) | 1ull;
+    }
+}
+
 // f(lo, hi) over [0, n) in contiguous runs on up to 8 host threads (one
 // when n is small)
 template <typename F>
@@ -1109,36 +1145,66 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
     // further part (a task can have hundreds of thousands of them).
     bool mask_used[256] = {};
     std::vector<uint32_t> live_of(h->raw.size());
+    parallel_runs(h->raw.size(), [&](size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) {
+            const HostHit &x = h->raw[i];
+            Path p;
+            p.prefix = (uint32_t)x.prefix;
+            p.len = (uint8_t)k;
+            for (int q = 0; q < k; q++) p.j[q] = x.j[q];
+            uint16_t insns[PCAGPU_MAX_KDEPTH];
+            node_insns(h, p, insns, &live_of[i]);
+        }
+    });
     for (size_t i = 0; i < h->raw.size(); i++) {
         const HostHit &x = h->raw[i];
-        Path p;
-        p.prefix = (uint32_t)x.prefix;
-        p.len = (uint8_t)k;
-        for (int q = 0; q < k; q++) p.j[q] = x.j[q];
-        uint16_t insns[PCAGPU_MAX_KDEPTH];
-        node_insns(h, p, insns, &live_of[i]);
         if (!(x.flags & 1) && (x.mask & live_of[i] & 0xFEu)) mask_used[x.mask] = true;
     }
-    std::vector<Node> nodes;
-    for (size_t i = 0; i < h->raw.size(); i++) {
-        const HostHit &x = h->raw[i];
-        if (!mask_used[x.mask]) continue;
-        Node nd;
-        nd.path.prefix = (uint32_t)x.prefix;
-        nd.path.len = (uint8_t)k;
-        for (int q = 0; q < k; q++) nd.path.j[q] = x.j[q];
-        nd.live = live_of[i];
-        nd.mask = x.mask;
-        nd.dead = x.flags & 1;
-        nd.skey = ((uint64_t)nd.path.prefix << 24) | ((uint64_t)(k >= 1 ? nd.path.j[0] : 0) << 12) |
-                  (uint64_t)(k >= 2 ? nd.path.j[1] : 0);
-        nodes.push_back(nd);
+    std::vector<size_t> sel;
+    for (size_t i = 0; i < h->raw.size(); i++)
+        if (mask_used[h->raw[i].mask]) sel.push_back(i);
+    std::vector<Node> nodes(sel.size());
+    parallel_runs(sel.size(), [&](size_t lo, size_t hi) {
+        for (size_t q = lo; q < hi; q++) {
+            const size_t i = sel[q];
+            const HostHit &x = h->raw[i];
+            Node &nd = nodes[q];
+            nd.path.prefix = (uint32_t)x.prefix;
+            nd.path.len = (uint8_t)k;
+            for (int p = 0; p < k; p++) nd.path.j[p] = x.j[p];
+            nd.live = live_of[i];
+            nd.mask = x.mask;
+            nd.dead = x.flags & 1;
+            nd.skey = ((uint64_t)nd.path.prefix << 24) | ((uint64_t)(k >= 1 ? nd.path.j[0] : 0) << 12) |
+                      (uint64_t)(k >= 2 ? nd.path.j[1] : 0);
+        }
+    });
+    // canonical (DFS rank) order; candidate indices are below 4096.  Runs
+    // sorted on host threads, then merged pairwise.
+    auto by_rank = [k](const Node &a, const Node &b) { return k <= 2 ? a.skey < b.skey : a.path < b.path; };
+    {
+        std::mutex mu;
+        std::vector<std::pair<size_t, size_t>> runs;
+        parallel_runs(nodes.size(), [&](size_t lo, size_t hi) {
+            std::sort(nodes.begin() + lo, nodes.begin() + hi, by_rank);
+            std::lock_guard<std::mutex> lk(mu);
+            runs.emplace_back(lo, hi);
+        });
+        std::sort(runs.begin(), runs.end());
+        while (runs.size() > 1) {   // merge neighbouring runs pairwise
+            std::vector<std::pair<size_t, size_t>> next;
+            for (size_t r = 0; r < runs.size(); r += 2) {
+                if (r + 1 == runs.size()) {
+                    next.push_back(runs[r]);
+                    break;
+                }
+                std::inplace_merge(nodes.begin() + runs[r].first, nodes.begin() + runs[r].second,
+                                   nodes.begin() + runs[r + 1].second, by_rank);
+                next.emplace_back(runs[r].first, runs[r + 1].second);
+            }
+            runs.swap(next);
+        }
     }
-    // canonical (DFS rank) order; candidate indices are below 4096
-    if (k <= 2)
-        std::sort(nodes.begin(), nodes.end(), [](const Node &a, const Node &b) { return a.skey < b.skey; });
-    else
-        std::sort(nodes.begin(), nodes.end(), [](const Node &a, const Node &b) { return a.path < b.path; });
     std::vector<TestState> st((size_t)(T > 0 ? T : 1));
     std::vector<uint8_t> key((size_t)KL);
 
@@ -1218,27 +1284,60 @@ extern "C" long pcagpu_end_depth(pcagpu_t *h, pca_solution_t *out, long cap)
         }
     }
     // ---- last level: among hit nodes with a surviving parent, the first of
-    // each state (every node of a target-matching state is a hit)
+    // each state (every node of a target-matching state is a hit).  Only a
+    // node that can be output (no dead test, a live matching register) needs
+    // the verdict, and only an earlier node of the same state can take it
+    // away: every visited node's key is hashed on host threads; the nodes
+    // whose hash equals an outputtable node's are then keyed in full, in
+    // rank order per hash, and compared byte for byte.
     std::vector<char> keep(nodes.size(), 0);
     {
-        // the first-visitor test over the remaining nodes: keys computed on
-        // host threads in runs, inserted in rank order
-        std::vector<size_t> idx;
+        std::vector<size_t> idx;   // visited nodes, rank order
         idx.reserve(nodes.size());
         for (size_t i = 0; i < nodes.size(); i++) {
             if (k >= 2 && !survive[parent_of(nodes[i].path)]) continue;
             if (k == 0) { keep[i] = 1; continue; }
             idx.push_back(i);
         }
-        ExactKeySet set(KL);
-        const size_t chunk = (size_t)1 << 16;
-        std::vector<uint8_t> keys(std::min(idx.size(), chunk) * (size_t)KL + 1);
-        for (size_t c0 = 0; c0 < idx.size(); c0 += chunk) {
-            const size_t c1 = std::min(idx.size(), c0 + chunk);
-            parallel_runs(c1 - c0, [&](size_t lo, size_t hi) {
-                node_keys(h, nodes, idx, c0 + lo, c0 + hi, keys.data() + lo * (size_t)KL, KL);
-            });
-            for (size_t q = c0; q < c1; q++) keep[idx[q]] = (char)set.insert(keys.data() + (q - c0) * (size_t)KL);
+        std::vector<uint64_t> hs(idx.size());
+        parallel_runs(idx.size(), [&](size_t lo, size_t hi) { node_hashes(h, nodes, idx, lo, hi, hs.data() + lo); });
+        auto outputtable = [&](size_t i) { return !nodes[i].dead && (nodes[i].mask & nodes[i].live & 0xFEu) != 0u; };
+        // the outputtable nodes' hashes in a small open-addressed table; a
+        // slot's list collects, in rank order, every visited node of that hash
+        size_t cap = 16;
+        size_t n_out = 0;
+        for (size_t q = 0; q < idx.size(); q++) n_out += outputtable(idx[q]) ? 1 : 0;
+        while (cap < 2 * n_out) cap <<= 1;
+        std::vector<uint64_t> tab(cap, 0);
+        auto slot_of = [&](uint64_t hv, bool add) -> long {
+            for (size_t j = (size_t)(h_mix64(hv) & (cap - 1));; j = (j + 1) & (cap - 1)) {
+                if (tab[j] == hv) return (long)j;
+                if (tab[j] == 0) {
+                    if (!add) return -1;
+                    tab[j] = hv;
+                    return (long)j;
+                }
+            }
+        };
+        for (size_t q = 0; q < idx.size(); q++)
+            if (outputtable(idx[q])) (void)slot_of(hs[q], true);
+        std::vector<std::vector<size_t>> lists(cap);
+        for (size_t q = 0; q < idx.size(); q++) {
+            const long j = slot_of(hs[q], false);
+            if (j >= 0) lists[(size_t)j].push_back(q);
+        }
+        std::vector<uint8_t> kq((size_t)KL);
+        for (const std::vector<size_t> &l : lists) {
+            if (l.empty()) continue;
+            std::vector<std::vector<uint8_t>> seen;   // distinct keys of this hash so far
+            for (size_t q : l) {
+                node_key(h, nodes[idx[q]].path, kq.data(), st.data());
+                bool first = true;
+                for (const auto &v : seen)
+                    if (std::memcmp(v.data(), kq.data(), (size_t)KL) == 0) { first = false; break; }
+                if (first) seen.push_back(kq);
+                if (first && outputtable(idx[q])) keep[idx[q]] = 1;
+            }
         }
     }
     long total = 0;

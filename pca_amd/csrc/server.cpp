@@ -26,6 +26,7 @@
 // resolution, only scheduled together.
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -71,6 +72,16 @@ struct PinnedBuf {
     }
 };
 
+// What resolving a slot's finished depth led to.
+struct Outcome {
+    int rc = 0;          // < 0: the request fails
+    bool done = false;   // optimum proven, or none up to the budget
+    int optimal = -1;
+    long nsol = 0;
+    std::vector<pca_solution_t> sols;
+    uint64_t ns = 0;     // host time the resolution took
+};
+
 struct Slot {
     pcagpu *h = nullptr;
     bool busy = false;
@@ -81,6 +92,12 @@ struct Slot {
     size_t next = 0;
     uint32_t staged = 0;     // prefixes of this step's launch
     size_t byte_cap = 0;     // device bytes this slot may keep between depths
+    // a finished depth resolved in the background: 0 none, 1 running, 2 the
+    // outcome is ready (res_state and res_out under pcagpu_server::res_mu);
+    // while it runs, the step touches nothing of the slot but busy and h
+    int res_state = 0;
+    Outcome res_out;
+    size_t bytes_seen = 0;   // device bytes when last counted outside a resolution
     // the depth's prefixes: the host-produced list, or the handle's
     // device-resident frontier (its pinned host view)
     bool dev() const { return h->dev_front; }
@@ -126,7 +143,12 @@ struct pcagpu_server {
     // launch) for its asynchronous upload
     PinnedBuf h_items;
     DevBuf<unsigned char> d_stage;
-    std::unique_ptr<HostPool> pool;   // staging and resolution threads
+    std::unique_ptr<HostPool> pool;   // staging threads
+    // finished depths are resolved on these threads while the steps go on
+    // launching the other slots' work
+    std::unique_ptr<JobQueue> resolver;
+    std::mutex res_mu;
+    std::condition_variable res_cv;
     std::vector<ScanItem> items;
     pcagpu_server_stats_t st;
 
@@ -143,11 +165,22 @@ namespace {
 
 double clock_s(const pcagpu_server *s) { return now_s() - s->t0; }
 
-size_t slots_bytes(const pcagpu_server *s)
+bool resolving(pcagpu_server *s, const Slot &sl)
+{
+    std::lock_guard<std::mutex> lk(s->res_mu);
+    return sl.res_state != 0;
+}
+
+// (a slot being resolved counts with its bytes from before: its handle's
+// buffers are the resolver's while it runs)
+size_t slots_bytes(pcagpu_server *s)
 {
     size_t b = 0;
-    for (const Slot &sl : s->slots)
-        if (sl.h) b += handle_device_bytes(sl.h);
+    for (Slot &sl : s->slots) {
+        if (!sl.h) continue;
+        if (!resolving(s, sl)) sl.bytes_seen = handle_device_bytes(sl.h);
+        b += sl.bytes_seen;
+    }
     return b;
 }
 
@@ -214,15 +247,6 @@ void fail_request(pcagpu_server *s, Slot &sl)
     sl.frontier.clear();
 }
 
-// What resolving a slot's finished depth led to.
-struct Outcome {
-    int rc = 0;          // < 0: the request fails
-    bool done = false;   // optimum proven, or none up to the budget
-    int optimal = -1;
-    long nsol = 0;
-    std::vector<pca_solution_t> sols;
-};
-
 // Depth finished: resolve it and, while no program is found and budget is
 // left, start the next depth.  Touches only the slot (its handle, stream and
 // frontier), so finished slots are resolved on several host threads.
@@ -269,6 +293,43 @@ int apply(pcagpu_server *s, Slot &sl, const Outcome &o)
 
 int advance(pcagpu_server *s, Slot &sl) { return apply(s, sl, resolve_depths(sl)); }
 
+// Resolves a slot's finished depth on a resolver thread.
+void resolve_in_background(pcagpu_server *s, Slot &sl)
+{
+    {
+        std::lock_guard<std::mutex> lk(s->res_mu);
+        sl.res_state = 1;
+    }
+    Slot *p = &sl;   // slots never move: the vector holds max_slots from the start
+    s->resolver->submit([s, p] {
+        const double t = now_s();
+        Outcome o = resolve_depths(*p);
+        o.ns = (uint64_t)((now_s() - t) * 1e9);
+        std::lock_guard<std::mutex> lk(s->res_mu);
+        p->res_out = std::move(o);
+        p->res_state = 2;
+        s->res_cv.notify_all();
+    });
+}
+
+// Applies the outcomes of the background resolutions that have finished
+// (on the calling thread, the one that steps the server).
+void apply_ready(pcagpu_server *s)
+{
+    for (Slot &sl : s->slots) {
+        Outcome o;
+        {
+            std::lock_guard<std::mutex> lk(s->res_mu);
+            if (sl.res_state != 2) continue;
+            o = std::move(sl.res_out);
+            sl.res_out = Outcome();
+            sl.res_state = 0;
+        }
+        s->st.resolve_ns += o.ns;
+        if (apply(s, sl, o) < 0) fail_request(s, sl);
+    }
+}
+
 // f(0..n-1) on the server's host threads (the calling one included; only
 // it when `parallel` is false).
 template <class F>
@@ -299,7 +360,7 @@ int open_slot(pcagpu_server *s, const pca_task_t *task, Slot **out)
 
 // True if one more slot fits max_slots and the HBM budget (the first slot
 // always does, so a small budget serialises instead of stalling).
-bool slot_fits(const pcagpu_server *s)
+bool slot_fits(pcagpu_server *s)
 {
     if (s->slots.size() >= s->opts.max_slots) return false;
     if (s->slots.empty()) return true;
@@ -393,6 +454,13 @@ extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_serve
         pcagpu_server_close(s);
         return PCAGPU_ENOMEM;
     }
+    s->resolver.reset(new (std::nothrow) JobQueue(std::min<uint32_t>(s->opts.max_slots, 4u),
+                                                  [dev] { (void)hipSetDevice(dev); }));
+    if (!s->resolver) {
+        pcagpu_server_close(s);
+        return PCAGPU_ENOMEM;
+    }
+    s->slots.reserve(s->opts.max_slots);
     // warm slots: opened now on a placeholder task (one input, one test) so
     // the first requests are only re-tasked
     pca_task_t ph;
@@ -417,6 +485,7 @@ extern "C" int pcagpu_server_open(const pcagpu_server_opts_t *opts, pcagpu_serve
 extern "C" void pcagpu_server_close(pcagpu_server_t *s)
 {
     if (!s) return;
+    s->resolver.reset();   // waits for the resolutions in flight
     for (Slot &sl : s->slots) pcagpu_close(sl.h);
     s->d_items.release();
     s->d_ctl.release();
@@ -459,6 +528,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         acc += (uint64_t)((t - tp) * 1e9);
         tp = t;
     };
+    apply_ready(s);
     int rc = admit(s);
     lap(s->st.admit_ns);
     if (rc < 0) return rc;
@@ -470,7 +540,13 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     uint32_t levels = 1;
     std::vector<Slot *> in_launch;
     uint32_t active = 0;
-    for (const Slot &sl : s->slots) active += sl.busy && sl.next < sl.fsize();
+    // (a slot being resolved is neither staged nor looked at)
+    std::vector<char> res_now(s->slots.size());
+    for (size_t i = 0; i < s->slots.size(); i++) res_now[i] = resolving(s, s->slots[i]) ? 1 : 0;
+    for (size_t i = 0; i < s->slots.size(); i++) {
+        const Slot &sl = s->slots[i];
+        active += !res_now[i] && sl.busy && sl.next < sl.fsize();
+    }
     const uint32_t share = std::max(s->opts.prefixes_per_item, s->opts.step_prefixes / std::max(active, 1u));
     auto align16 = [](size_t b) { return (b + 15) & ~(size_t)15; };
     // staging is host work per slot (tile, prefix fitting, the batch and its
@@ -484,11 +560,13 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     };
     std::vector<Slot *> act;
     size_t left = 0;
-    for (Slot &sl : s->slots)
-        if (sl.busy && sl.next < sl.fsize()) {
+    for (size_t i = 0; i < s->slots.size(); i++) {
+        Slot &sl = s->slots[i];
+        if (!res_now[i] && sl.busy && sl.next < sl.fsize()) {
             act.push_back(&sl);
             left += sl.fsize() - sl.next;
         }
+    }
     std::vector<Staged> stg(act.size());
     // (threads only pay off with work to share: many slots or big batches)
     run_parallel(s, act.size(), [&](size_t i) {
@@ -561,8 +639,10 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     // the stream is drained so that no copy of this step is still running
     // when the next step stages), and the server goes on with the rest.
     auto launch_and_collect = [&]() -> int {
-    s->step_no++;
-    if (s->opts.fail_step && s->step_no == s->opts.fail_step) return PCAGPU_ENOMEM;
+    if (!s->items.empty()) {
+        s->step_no++;
+        if (s->opts.fail_step && s->step_no == s->opts.fail_step) return PCAGPU_ENOMEM;
+    }
     // ---- every item's prefixes and blocks into one pinned buffer, one copy
     if (!s->items.empty()) {
         if ((rc = s->h_stage.ensure(stage_bytes)) || (rc = s->d_stage.ensure(s->h_stage.n))) return rc;
@@ -737,16 +817,24 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
         return rc;
     }
     lap(s->st.stage_ns);   // hit collection
-    // ---- tasks whose depth is complete: resolved in parallel, applied in
-    // slot order
-    std::vector<Slot *> fin;
-    for (Slot &sl : s->slots)
-        if (sl.busy && sl.next >= sl.fsize()) fin.push_back(&sl);
-    std::vector<Outcome> res(fin.size());
-    run_parallel(s, fin.size(), [&](size_t i) { res[i] = resolve_depths(*fin[i]); });
-    for (size_t i = 0; i < fin.size(); i++)
-        if (apply(s, *fin[i], res[i]) < 0) fail_request(s, *fin[i]);
-    lap(s->st.resolve_ns);
+    // ---- tasks whose depth is complete: resolved on the resolver threads
+    // (exact OEP, CPU re-verification, the next depth's frontier) while the
+    // next steps launch the other tasks' work; their outcomes are applied at
+    // the start or end of a later step, in slot order
+    for (size_t i = 0; i < s->slots.size(); i++) {
+        Slot &sl = s->slots[i];
+        if (!res_now[i] && sl.busy && sl.next >= sl.fsize()) resolve_in_background(s, sl);
+    }
+    // with nothing launched this step, wait a little for a resolution
+    // instead of returning at once to a caller that steps again
+    if (s->items.empty()) {
+        std::unique_lock<std::mutex> lk(s->res_mu);
+        bool running = false;
+        for (const Slot &sl : s->slots) running = running || sl.res_state == 1;
+        if (running) s->res_cv.wait_for(lk, std::chrono::milliseconds(1));
+    }
+    apply_ready(s);
+    lap(s->st.wait_ns);
     int pending = (int)s->queue.size();
     for (const Slot &sl : s->slots) pending += sl.busy;
     return pending;
