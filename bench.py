@@ -516,9 +516,9 @@ def main():
                                 "achieved_lower_bound": st["suffixes"] * ((D - s) * 8 + 7) / kernel_s / 1e12,
                                 "frac_lower_bound": st["suffixes"] * ((D - s) * 8 + 7) / kernel_s / PEAK_INT_OPS,
                                 "unit": "Tint-ops/s"},
-        # one-item launches of 2- and 3-instruction suffixes run scan_kernel_k23,
+        # one-item launches of 2- and 3-instruction suffixes run scan_kernel_k2 / k3,
         # other suffix lengths scan_kernel_one
-        "kernel": {"name": "scan_kernel_k23" if D - s in (2, 3) else "scan_kernel_one",
+        "kernel": {"name": {2: "scan_kernel_k2", 3: "scan_kernel_k3"}.get(D - s, "scan_kernel_one"),
                    "avg_launch_ms": avg_launch * 1e3, "launches": launches,
                    "candidates": cand, "suffixes": st["suffixes"], "tests_evaluated": st["tests_evaluated"],
                    "parents": st["parents"], "raw_hits": st["raw_hits"],

@@ -335,7 +335,7 @@ struct Pool {                      // wave-uniform bookkeeping
 };
 
 // The record is written field by field (no private copy).
-__device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx, const JPath &jpath,
+__device__ __forceinline__ void emit_hit(const ScanArgs &a, int k, uint32_t prefix_idx, const JPath &jpath,
                                          uint32_t jlast, uint32_t mask, uint32_t flags)
 {
     const uint32_t slot = atomicAdd(a.hit_count, 1u);
@@ -346,7 +346,7 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
         h->flags = (uint8_t)flags;
         h->pad = 0;
         for (int q = 0; q < PCAGPU_MAX_KDEPTH; q++)
-            h->j[q] = (uint16_t)(q + 1 < a.k ? jp_at(jpath, q) : (q + 1 == a.k ? jlast : 0u));
+            h->j[q] = (uint16_t)(q + 1 < k ? jp_at(jpath, q) : (q + 1 == k ? jlast : 0u));
     }
 }
 
@@ -357,7 +357,7 @@ __device__ __forceinline__ void emit_hit(const ScanArgs &a, uint32_t prefix_idx,
 // parent's prefix and suffix path.  A test that some entry's parent skips
 // (pending BR, VM stopped) keeps that entry's rd values whose register
 // already equals the target.
-__device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &a, const CandDev &cd,
+__device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &a, int k, const CandDev &cd,
                                            WaveLds &sv, uint32_t cnt, int t0, int lane, WaveCounters &wc,
                                            uint32_t (*tst)[2], Pool &pool)
 {
@@ -488,7 +488,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
         jp.lds = sv.path[s];
         const uint32_t pidx = sv.meta[s][0];
         for (uint32_t r = 1; r < 8; r++)
-            if (okf >> r & 1u) emit_hit(a, pidx, jp, cd.g->vg_j[mm][g][r], 1u << r, fl);
+            if (okf >> r & 1u) emit_hit(a, k, pidx, jp, cd.g->vg_j[mm][g][r], 1u << r, fl);
     }
 }
 
@@ -509,7 +509,7 @@ __device__ __forceinline__ void pool_drain(const DevTask &task, const ScanArgs &
 // live mask come from the grammar table (GrammarDev::kind_bits).
 // The parent's state P sits in the wave's slot pool.cur; its surviving
 // groups are queued for the pooled survivor phase (pool_drain).
-__device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, const CandDev &cd,
+__device__ __forceinline__ void scan_parent_bits(const DevTask &task, const ScanArgs &a, int k, const CandDev &cd,
                                                  const TestState *P,
                                                  uint32_t live, uint32_t prefix_idx, const JPath &jpath,
                                                  int lane, WaveCounters &wc,
@@ -606,7 +606,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         sv.meta[cur][1] = (uint32_t)m | (ps.dead ? 0x100u : 0u);
         sv.skip[cur] = ps.skipm;
     }
-    if (lane + 1 < a.k) sv.path[cur][lane] = jp_at(jpath, lane);
+    if (lane + 1 < k) sv.path[cur][lane] = jp_at(jpath, lane);
     const uint32_t next = (cur + 1u) & (NSLOT - 1u);
     uint32_t pushed = 0;   // this parent's entries queued so far (the first `pushed` in lane order)
     for (;;) {
@@ -641,7 +641,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
             cnt = pool.qn;
         if (cnt) {
             wave_sync();
-            pool_drain(task, a, cd, sv, cnt, TA, lane, wc, tst, pool);
+            pool_drain(task, a, k, cd, sv, cnt, TA, lane, wc, tst, pool);
             wave_sync();
             const uint32_t rest = pool.qn - cnt;
             if (cnt == 64u && (uint32_t)lane < rest) sv.queue[lane] = sv.queue[64 + lane];
@@ -656,7 +656,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
 
 // Parent where some register already equals the target: every candidate
 // that leaves it alone is a hit, so go candidate by candidate.
-__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, const CandDev &cd,
+__device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs &a, int k, const CandDev &cd,
                                             const TestState *P,
                                             uint32_t live, uint32_t prefix_idx, const JPath &jpath, int lane,
                                             WaveCounters &wc, uint32_t *queue, const ParentSummary &ps)
@@ -712,7 +712,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
             pending = pending && !mine;
         }
         const uint32_t match = valid ? (other | (rd_ok ? (1u << rd) : 0u)) & 0xFEu : 0u;
-        if (match) emit_hit(a, prefix_idx, jpath, j, match, (dead || fault) ? 1u : 0u);
+        if (match) emit_hit(a, k, prefix_idx, jpath, j, match, (dead || fault) ? 1u : 0u);
     };
 
     uint32_t insn_next = lane < n ? L[lane] : 0u;
@@ -731,7 +731,7 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
         bool alive;
         if (is_cmp || !valid) {
             // CMP writes no register: its hits are the parent's matches
-            if (valid && other) emit_hit(a, prefix_idx, jpath, j, other & 0xFEu, dead);
+            if (valid && other) emit_hit(a, k, prefix_idx, jpath, j, other & 0xFEu, dead);
             alive = false;
         } else if (two && other == 0u) {
             const uint32_t sa = op == OP_ADDI ? rd : rs;
@@ -835,9 +835,12 @@ __device__ __forceinline__ void leaf_parent(const DevTask &task, const ScanArgs 
 // scanned parents (max(lch.levels, 2) - 1 of them); the scanned parents
 // themselves (level k - 1) sit in the NSLOT slots of the wave's WaveLds,
 // where they stay while their survivors wait in the pooled queue.
-template <bool MULTI, bool K23>
+template <bool MULTI, int KF>
 __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs &a1, const DevTask &task1)
 {
+    // KF: the suffix length of every item, fixed at compile time (2 or 3;
+    // 0: read from the item)
+    constexpr bool K23 = KF != 0;
     extern __shared__ TestState s_dyn[];
     __shared__ WaveLds s_wave[SCAN_WAVES];
     __shared__ int s_cnt[128], s_split[128];
@@ -892,7 +895,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     auto flush_pool = [&]() {
         const int ta = taskp->T < (int)a.tile.mask_tests ? taskp->T : (int)a.tile.mask_tests;
         wave_sync();
-        pool_drain(*taskp, a, cand, sv, pool.qn, ta, lane, wc, tst, pool);
+        pool_drain(*taskp, a, KF ? KF : a.k, cand, sv, pool.qn, ta, lane, wc, tst, pool);
         wave_sync();
         pool.qn = 0;
     };
@@ -936,7 +939,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
             my_tgt = lane < taskp->T ? taskp->tgt[lane] : 0u;
         }
         const DevTask &task = *taskp;
-        const int k = a.k;
+        const int k = KF ? KF : a.k;
         const uint64_t u = ug - it_begin;
         uint32_t pi, js = 0, j0 = 0;
         if (k >= 2) {
@@ -1000,6 +1003,12 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
                 TestState *PP = sv.slot[pool.cur];
                 PackedState st = pl;
+                // the parent's status word enters each child as an unknown
+                // value: the tests on its flags, resume and frozen bytes are
+                // made inside the branch of the child's opcode that reads
+                // them, instead of hoisted out of the loop as lane masks
+                // held (and spilled) across it
+                __asm__ volatile("" : "+v"(st.w[4]));
                 if (act) {
                     step_packed<false>(st, c, cpos);
                     store_ps(&PP[lane], st);
@@ -1012,9 +1021,9 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 jpath.j1 = j0 + ci;
                 wave_sync();
                 if (ps.pm == 0u)
-                    scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
+                    scan_parent_bits(task, a, k, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
                 else
-                    scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
+                    scan_parent(task, a, k, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
                 wave_sync();
             }
             if (lane == 0) wc.c[3] += nb;
@@ -1048,9 +1057,9 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                         if (lane == 0) wc.c[3]++;
                         wave_sync();
                         if (ps.pm == 0u)
-                            scan_parent_bits(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
+                            scan_parent_bits(task, a, k, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
                         else
-                            scan_parent(task, a, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
+                            scan_parent(task, a, k, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
                     }
                     wave_sync();
                     if (L == Lu) break;
@@ -1115,19 +1124,25 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
 // (188, none): 4.70 s of kernel time against 4.97 and 4.80.
 __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false, false>(lch, a1, task1);
+    scan_body<false, 0>(lch, a1, task1);
 }
 
 // One-item launches of 2- and 3-instruction suffixes (the block's children
-// are the scanned parents): the same body without the general walk.
-__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_k23(ScanLaunch lch, ScanArgs a1, DevTask task1)
+// are the scanned parents): the same body without the general walk, with
+// the suffix length a constant (the hit paths' per-position selects fold).
+__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_k2(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false, true>(lch, a1, task1);
+    scan_body<false, 2>(lch, a1, task1);
+}
+
+__global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_k3(ScanLaunch lch, ScanArgs a1, DevTask task1)
+{
+    scan_body<false, 3>(lch, a1, task1);
 }
 
 __global__ __launch_bounds__(SCAN_WG, 3) void scan_kernel_multi(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<true, false>(lch, a1, task1);
+    scan_body<true, 0>(lch, a1, task1);
 }
 
 // The kernel argument of a one-item launch: its work item with the launch's
@@ -1705,8 +1720,10 @@ hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
     const size_t lds = scan_dyn_lds(levels);
     if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_multi, SCAN_WG, lds);
     // a one-item launch's levels are its suffix length k (k >= 1)
-    if (levels == 2u || levels == 3u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_k23, SCAN_WG, lds);
+    if (levels == 2u)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_k2, SCAN_WG, lds);
+    if (levels == 3u)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_k3, SCAN_WG, lds);
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_one, SCAN_WG, lds);
 }
 
@@ -1735,7 +1752,7 @@ void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &ta
     args->ptrs[0] = &args->l;
     args->ptrs[1] = &args->a1;
     args->ptrs[2] = &args->task;
-    p->func = reinterpret_cast<void *>((it.k == 2 || it.k == 3) ? scan_kernel_k23 : scan_kernel_one);
+    p->func = reinterpret_cast<void *>(it.k == 2 ? scan_kernel_k2 : (it.k == 3 ? scan_kernel_k3 : scan_kernel_one));
     p->gridDim = dim3((unsigned)grid);
     p->blockDim = dim3(SCAN_WG);
     p->sharedMemBytes = (unsigned)scan_dyn_lds(l.levels);
@@ -1774,8 +1791,10 @@ void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask 
 {
     const size_t lds = scan_dyn_lds(l.levels);
     const ScanArgs a1 = one_item_args(l, host_items[0]);
-    if (l.n_items == 1 && (host_items[0].k == 2 || host_items[0].k == 3))
-        hipLaunchKernelGGL(scan_kernel_k23, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
+    if (l.n_items == 1 && host_items[0].k == 2)
+        hipLaunchKernelGGL(scan_kernel_k2, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
+    else if (l.n_items == 1 && host_items[0].k == 3)
+        hipLaunchKernelGGL(scan_kernel_k3, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
     else if (l.n_items == 1)
         hipLaunchKernelGGL(scan_kernel_one, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
     else

@@ -46,6 +46,10 @@ using namespace pca;
 
 namespace {
 
+// A finished depth with at most this many raw hits resolves in well under a
+// millisecond (the heavy ones, millions of hits, take 0.1-0.2 s).
+constexpr size_t kInlineRawHits = 1 << 14;
+
 // Growable pinned host buffer (contents are not kept when it grows).
 struct PinnedBuf {
     unsigned char *p = nullptr;
@@ -820,10 +824,26 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
     // ---- tasks whose depth is complete: resolved on the resolver threads
     // (exact OEP, CPU re-verification, the next depth's frontier) while the
     // next steps launch the other tasks' work; their outcomes are applied at
-    // the start or end of a later step, in slot order
+    // the start or end of a later step, in slot order.  Light depths (few
+    // raw hits) are resolved here, on the step's host threads, and join the
+    // next step: handed to a resolver they would miss it and wait a step
+    // longer, for work that takes far less than a step.
+    std::vector<Slot *> light;
     for (size_t i = 0; i < s->slots.size(); i++) {
         Slot &sl = s->slots[i];
-        if (!res_now[i] && sl.busy && sl.next >= sl.fsize()) resolve_in_background(s, sl);
+        if (res_now[i] || !sl.busy || sl.next < sl.fsize()) continue;
+        if (sl.h->raw.size() <= kInlineRawHits)
+            light.push_back(&sl);
+        else
+            resolve_in_background(s, sl);
+    }
+    if (!light.empty()) {
+        const double t = now_s();
+        std::vector<Outcome> outs(light.size());
+        run_parallel(s, light.size(), [&](size_t i) { outs[i] = resolve_depths(*light[i]); }, light.size() > 1);
+        s->st.resolve_ns += (uint64_t)((now_s() - t) * 1e9);
+        for (size_t i = 0; i < light.size(); i++)
+            if (apply(s, *light[i], outs[i]) < 0) fail_request(s, *light[i]);
     }
     // with nothing launched this step, wait a little for a resolution
     // instead of returning at once to a caller that steps again
