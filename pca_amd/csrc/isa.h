@@ -222,16 +222,20 @@ PCA_HD void ps_set_reg(PackedState &s, uint32_t k, uint32_t v)
         s.w[3] = (s.w[3] & keep) | nv;
 }
 
+// Whether a test runs the instruction at pos: not frozen, no pending branch
+// past pos.
+PCA_HD bool ps_runs(uint32_t w4, uint32_t pos) { return (w4 & 0xFF0000u) == 0u && ((w4 >> 8) & 0xFFu) <= pos; }
+
 // FLAGS = false: the flags byte is left as it was (for a state whose flags
 // nothing reads: the last kernel position holds no BR, and its value-writing
 // instructions do not read flags); registers, resume, frozen and exec are
-// stepped exactly.
-template <bool FLAGS = true>
+// stepped exactly.  CHECK = false: the caller has checked ps_runs.
+template <bool FLAGS = true, bool CHECK = true>
 PCA_HD void step_packed(PackedState &s, uint32_t insn, uint32_t pos)
 {
     const uint32_t w4 = s.w[4];
-    const uint32_t flags = w4 & 0xFFu, resume = (w4 >> 8) & 0xFFu, frozen = (w4 >> 16) & 0xFFu;
-    if (frozen || resume > pos) return;
+    const uint32_t flags = w4 & 0xFFu;
+    if (CHECK && !ps_runs(w4, pos)) return;
     uint32_t nw4 = w4 + (1u << 24);   // exec++ (a byte: wraps like the uint8_t field)
     const uint32_t op = insn_op(insn), rd = insn_rd(insn), rs = insn_rs(insn), rt = insn_rt(insn);
     if (op == OP_BR) {

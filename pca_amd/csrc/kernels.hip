@@ -1009,10 +1009,10 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 // them, instead of hoisted out of the loop as lane masks
                 // held (and spilled) across it
                 __asm__ volatile("" : "+v"(st.w[4]));
-                if (act) {
-                    step_packed<false>(st, c, cpos);
-                    store_ps(&PP[lane], st);
-                }
+                // one region for the lanes whose test runs the child (the
+                // others keep the parent's words): no per-condition copies
+                if (act && ps_runs(st.w[4], cpos)) step_packed<false, false>(st, c, cpos);
+                if (act) store_ps(&PP[lane], st);
                 const ParentSummary ps = summarize_state(task, st, act, (uint32_t)(a.D - 1), my_tgt);
                 const uint32_t lvL = live_after(lvp, c);
                 JPath jpath;

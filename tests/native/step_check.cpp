@@ -43,10 +43,12 @@ int main()
                     TestState a = s;
                     PackedState p;
                     std::memcpy(&p, &s, sizeof(p));
-                    PackedState pn = p;
+                    PackedState pn = p, pc = p;
                     step_test(a, insn, pos);
                     step_packed(p, insn, pos);
                     step_packed<false>(pn, insn, pos);
+                    // the caller-checked form, as the kernels use it
+                    if (ps_runs(pc.w[4], pos)) step_packed<false, false>(pc, insn, pos);
                     cases++;
                     if (std::memcmp(&a, &p, sizeof(p)) != 0) {
                         std::printf("mismatch insn %04x pos %u trial %d\n", insn, pos, trial);
@@ -57,6 +59,10 @@ int main()
                     an.flags = s.flags;
                     if (std::memcmp(&an, &pn, sizeof(pn)) != 0) {
                         std::printf("flag-free mismatch insn %04x pos %u trial %d\n", insn, pos, trial);
+                        return 1;
+                    }
+                    if (std::memcmp(&pc, &pn, sizeof(pn)) != 0) {
+                        std::printf("caller-checked mismatch insn %04x pos %u trial %d\n", insn, pos, trial);
                         return 1;
                     }
                 }
