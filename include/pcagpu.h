@@ -98,7 +98,8 @@ typedef struct {
     uint64_t oep_collisions;  /* 64-bit key matches with a different state   */
     uint64_t parents;         /* parent states scanned (last-position lists) */
     uint64_t scan_tile;       /* tile of the last scan launch: block_kids |
-                                 mask_tests << 8                             */
+                                 mask_tests << 8 | mode << 16 (0: last
+                                 position shared per block, 1: per child)    */
 } pcagpu_stats_t;
 
 typedef struct {
@@ -109,8 +110,12 @@ typedef struct {
     uint32_t fixed_test_order;/* 1: keep the initial device test order; 0:
                                  reorder tests 2.. after every scan launch by
                                  how often each ends a surviving candidate   */
-    uint32_t scan_tile;       /* 0: autotuned; else block_kids (1..64) |
-                                 mask_tests (1..3) << 8, fixed              */
+    uint32_t scan_tile;       /* low 16 bits 0: autotuned; else block_kids
+                                 (1..64) | mask_tests (1..6) << 8, fixed;
+                                 bits 16..23: 0 decides the last-position
+                                 value groups that do not read the register
+                                 a child writes once per block, 1 scans
+                                 every child on its own (same results)      */
     uint32_t eager_flush;     /* 1: plain stream calls per scan launch; 0: one
                                  replayed HIP graph (copies, clears, kernel,
                                  counter read-back)                          */

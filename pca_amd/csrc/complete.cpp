@@ -93,7 +93,7 @@ int scan_level(pcagpu *h, const DevTask &dt, const CandDev &cd, int cur, uint64_
         it.D = r;
         it.s = r - 1;
         it.k = 1;
-        it.tile = ScanTile{64, 2};
+        it.tile = ScanTile{64, 2, TILE_SHARED};
         it.hits = h->d_hits.p;
         it.hit_count = h->d_hit_count.p;
         it.hit_cap = h->hit_cap;

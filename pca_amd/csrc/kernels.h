@@ -94,11 +94,17 @@ struct ScanBlock {
 constexpr int BLOCK_KIDS_MAX = 64;
 
 // Tile choice of a scan launch (picked by the host autotuner): children per
-// block, and tests run by wave masks before survivors go to the queue.
+// block, tests run on every value group before survivors go to a queue,
+// and how a 2- or 3-instruction suffix scans the last position.
 struct ScanTile {
     uint16_t block_kids;   // 1..64
-    uint16_t mask_tests;   // 1..3
+    uint8_t mask_tests;    // 1..6
+    // 0: shared per block (value groups that do not read the register a
+    // child writes are decided once on the block's node); 1: every child
+    // scanned on its own
+    uint8_t mode;
 };
+constexpr uint8_t TILE_SHARED = 0, TILE_PER_CHILD = 1;
 
 struct ScanArgs {
     const pca_prefix_t *prefixes;

@@ -332,6 +332,7 @@ struct Pool {                      // wave-uniform bookkeeping
     uint32_t pend;                 // entries queued per slot, 8 bits each
     uint32_t cur;                  // slot of the parent being scanned
     uint32_t nd;                   // drains so far
+    uint32_t ns;                   // slots in use: NSLOT, or 2 when slots 2..3 hold ShareLds
 };
 
 // The record is written field by field (no private copy).
@@ -607,7 +608,7 @@ __device__ __forceinline__ void scan_parent_bits(const DevTask &task, const Scan
         sv.skip[cur] = ps.skipm;
     }
     if (lane + 1 < k) sv.path[cur][lane] = jp_at(jpath, lane);
-    const uint32_t next = (cur + 1u) & (NSLOT - 1u);
+    const uint32_t next = cur + 1u == pool.ns ? 0u : cur + 1u;
     uint32_t pushed = 0;   // this parent's entries queued so far (the first `pushed` in lane order)
     for (;;) {
         if (pushed < total) {
@@ -795,6 +796,438 @@ __device__ __forceinline__ void scan_parent(const DevTask &task, const ScanArgs 
     }
 }
 
+// ---------------------------------------------- block-shared last position
+// The children of one block (k = 2, 3) are the states of one node N after
+// one more instruction each.  A child that writes one register x (an ALU,
+// LDI or ADDI instruction) without branching or faulting differs from N
+// only in x, on the tests it runs, and skips at the last position exactly
+// the tests N would skip there (its resume and frozen bytes are N's).  A
+// value group (op, a, b) with a != x and b != x therefore has the same
+// value vector below the child as below N, the same rd values allowed on
+// skipped tests (those rows are N's), and -- except for the rd = x bit of
+// a DIV / MOD group whose divisor is zero on some test, which reads x's
+// value -- the same outcome.  Those groups ("shared lanes") are decided
+// once per block on N over every test; only the 15 lanes 8a + b with
+// a == x or b == x are evaluated for each child, four children packed into
+// one 64-lane pass (lane 16j + p: child j, p < 8 the pair (x, p), p >= 8
+// the pair (p - 8, x)).  A CMP child writes no register: all its lanes are
+// shared.  Children that branch, fault, or already hold a register equal
+// to the target on every test take the per-child path.
+
+// Per-wave buffers of the shared path, overlaid on pool slots 2 and 3 (a
+// launch on this path cycles its pool through slots 0 and 1 only).
+constexpr uint32_t GQ_CAP = 96;
+constexpr uint32_t SH_KIDS = 32;   // shared-path children recorded per pass over a block
+constexpr int SH_TA = 4;           // tests run on every group before the survivors queue (at most)
+// One recorded child: its instruction and candidate index, the tests on
+// which the register it writes equals the target, and that register's
+// value on the first SH_TA tests.
+struct ShareKid {
+    uint32_t cc;              // instruction | candidate index at D-2 << 16
+    uint32_t xm;              // bit t: x equals the target on test t
+    uint16_t xv[SH_TA];
+};
+struct ShareLds {
+    ShareKid kid[SH_KIDS];
+    uint32_t gq[GQ_CAP];      // packed survivors: q | bit << 4 | ok << 10 | fault << 18 | child << 19
+    uint32_t fn[64];          // per lane: N's groups matching every test (kind bits 0..12; 3, 4: DIV,
+                              // MOD) | DIV rd mask << 13 | MOD rd mask << 20 (bits 1..7 of each)
+                              // | DIV value match << 27 | MOD value match << 28
+    uint32_t xs[8];           // [0]: N's group tests over all lanes; [x]: over the lanes reading x
+    uint32_t zb[8];           // per register: tests (not skipped) on which N holds 0 there
+};
+static_assert(sizeof(ShareLds) <= 2 * MAXT * sizeof(TestState), "ShareLds fits in two pool slots");
+
+// The value groups of one lane still matching: kind bits (ADD..ASR, ADDI,
+// LDI; DIV and MOD kept as rd masks), and for a node scan whether DIV / MOD
+// matched on every test whose divisor is nonzero (the rd = x bit of a
+// child).
+struct LaneKinds {
+    uint32_t kl, okd, okm;
+    bool fault, vmd, vmm;
+};
+
+// One test of every group of a lane (lbit = 8a + b: the LDI immediate index
+// when below 16, b the ADDI immediate index); x, y the source values, tw
+// the test's target word.  rmz() gives the register-match mask of the row
+// this lane's groups leave alone on a zero divisor (called only when some
+// lane divides by zero).  VM: keep the value-match flags of a node scan.
+template <bool VM, class RMZ>
+__device__ __forceinline__ void lane_test(LaneKinds &g, uint32_t x, uint32_t y, uint32_t tw, uint32_t lbit,
+                                          uint32_t addi_imm, RMZ rmz)
+{
+    const uint32_t tg = tw & 0xFFFFu;
+    const uint32_t sh = y & 15u;
+    uint32_t keep = (lbit < 16u && ((tw >> (16u + lbit)) & 1u)) ? 1u << 12 : 0u;
+    keep |= (((x + y) & 0xFFFFu) == tg ? 1u : 0u) << 0;
+    keep |= (((x - y) & 0xFFFFu) == tg ? 1u : 0u) << 1;
+    keep |= (((x * y) & 0xFFFFu) == tg ? 1u : 0u) << 2;
+    keep |= ((x & y) == tg ? 1u : 0u) << 5;
+    keep |= ((x | y) == tg ? 1u : 0u) << 6;
+    keep |= ((x ^ y) == tg ? 1u : 0u) << 7;
+    keep |= (((x << sh) & 0xFFFFu) == tg ? 1u : 0u) << 8;
+    keep |= ((x >> sh) == tg ? 1u : 0u) << 9;
+    keep |= (((uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) & 0xFFFFu) == tg ? 1u : 0u) << 10;
+    keep |= (((x + addi_imm) & 0xFFFFu) == tg ? 1u : 0u) << 11;
+    g.kl &= keep;
+    if (any_lane((g.okd | g.okm) != 0u || (VM && (g.vmd || g.vmm)))) {
+        const bool z = y == 0u;
+        const uint32_t rmt = any_lane(z) ? rmz() : 0u;
+        const int32_t sx = (int16_t)(uint16_t)x, sy = z ? 1 : (int32_t)(int16_t)(uint16_t)y;
+        const int32_t qt = sdiv16_q(sx, sy);
+        const bool dq = ((uint32_t)qt & 0xFFFFu) == tg, dr = ((uint32_t)(sx - qt * sy) & 0xFFFFu) == tg;
+        g.okd &= z ? rmt : (dq ? 0xFFu : 0u);
+        g.okm &= z ? rmt : (dr ? 0xFFu : 0u);
+        g.fault = g.fault || (z && (g.okd != 0u || g.okm != 0u));
+        if (VM) {
+            g.vmd = g.vmd && (z || dq);
+            g.vmm = g.vmm && (z || dr);
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t addi_imm_of(uint32_t b)
+{
+    return (b & 1u) ? (b >> 1) + 1u : 0x10000u - ((b >> 1) + 1u);
+}
+
+// Every value group of node N (state rows in LDS, live-mask index mN, tests
+// skipped at the last position skipm) on every test, lanes = groups'
+// source pairs.  Returns the group tests the per-child scan evaluates on
+// this lane for one child (phase A up to TA tests while no skipped test
+// has narrowed rd, then every test a group reaches); sets rm_and (AND of
+// the skipped tests' register matches) and fn_any, and when some group
+// matches every test writes the lanes' matches to sh.fn.
+__device__ __forceinline__ uint32_t node_scan(const DevTask &task, const CandDev &cd, const TestState *N, uint32_t mN,
+                                              uint64_t skipm, int TA, int lane, ShareLds &sh, uint32_t &rm_and_out,
+                                              bool &fn_any)
+{
+    const int T = task.T;
+    const uint32_t ra = (uint32_t)lane >> 3, rb = (uint32_t)lane & 7u;
+    const uint32_t addi_imm = addi_imm_of(rb);
+    const uint32_t kb = cd.g->kind_bits[mN][lane];
+    LaneKinds g;
+    g.kl = kb & ~0x18u;
+    g.okd = (kb >> 3 & 1u) ? 0xFEu : 0u;
+    g.okm = (kb >> 4 & 1u) ? 0xFEu : 0u;
+    g.vmd = (kb >> 3 & 1u) != 0u;
+    g.vmm = (kb >> 4 & 1u) != 0u;
+    g.fault = false;
+    uint32_t rm_and = 0xFFu, ev = 0;
+    for (int t = 0; t < T; t++) {
+        if (!any_lane(g.kl != 0u || g.okd != 0u || g.okm != 0u || g.vmd || g.vmm)) break;
+        const uint32_t nal = (uint32_t)__popc(g.kl) + (g.okd != 0u ? 1u : 0u) + (g.okm != 0u ? 1u : 0u);
+        if (t >= TA || rm_and == 0xFFu) ev += nal;
+        const uint32_t tw = task.tgtw[t];
+        if ((skipm >> t) & 1ull) {
+            const uint32_t rmt = row_regmatch_uni(&N[t], tw & 0xFFFFu);
+            g.okd &= rmt;
+            g.okm &= rmt;
+            rm_and &= rmt;
+            if (rm_and == 0u) g.kl = 0u;
+            continue;
+        }
+        const uint32_t x = N[t].r[ra], y = N[t].r[rb];
+        lane_test<true>(g, x, y, tw, (uint32_t)lane, addi_imm,
+                        [&]() { return row_regmatch_uni(&N[t], tw & 0xFFFFu); });
+    }
+    rm_and_out = rm_and;
+    const uint32_t fk = g.kl | ((g.okd != 0u || g.vmd) ? 1u << 3 : 0u) | ((g.okm != 0u || g.vmm) ? 1u << 4 : 0u);
+    fn_any = any_lane(fk != 0u);
+    if (fn_any)
+        sh.fn[lane] = fk | ((g.okd & 0xFEu) << 12) | ((g.okm & 0xFEu) << 19) | (g.vmd ? 1u << 27 : 0u) |
+                      (g.vmm ? 1u << 28 : 0u);
+    return ev;
+}
+
+// Hits of one shared-path child on its shared lanes: N's groups matching
+// every test, on the lanes that do not read x (x = 8: a CMP child, every
+// lane).  xm: tests on which the child's x equals the target (bit t).
+__device__ __forceinline__ void emit_shared(const ScanArgs &a, int k, const CandDev &cd, const ShareLds &sh,
+                                            uint32_t x, uint32_t mP, uint32_t rm_and, uint32_t xm,
+                                            uint32_t prefix_idx, const JPath &jp, uint32_t deadF, int lane)
+{
+    const uint32_t ra = (uint32_t)lane >> 3, rb = (uint32_t)lane & 7u;
+    const bool isx = x < 8u && (ra == x || rb == x);
+    const uint32_t f = isx ? 0u : sh.fn[lane];
+    uint32_t fk = f & 0x1FFFu;
+    if (!any_lane(fk != 0u)) return;
+    const uint32_t zbl = sh.zb[rb];
+    uint32_t okd = (f >> 12) & 0xFEu, okm = (f >> 19) & 0xFEu;
+    if (x < 8u) {
+        // rd = x of a DIV / MOD group: its value matched on every test with
+        // a nonzero divisor, and x equals the target on the skipped tests
+        // (rm_and) and on the tests where the divisor is zero
+        const bool zx = (zbl & ~xm) == 0u && ((rm_and >> x) & 1u);
+        okd = (okd & ~(1u << x)) | ((((f >> 27) & 1u) && zx) ? 1u << x : 0u);
+        okm = (okm & ~(1u << x)) | ((((f >> 28) & 1u) && zx) ? 1u << x : 0u);
+    }
+    while (fk) {
+        const uint32_t q = (uint32_t)__builtin_ctz(fk);
+        fk &= fk - 1u;
+        const bool dm = q == 3u || q == 4u;
+        const uint32_t ok = q == 3u ? okd : (q == 4u ? okm : rm_and);
+        if (!(ok & 0xFEu)) continue;
+        const uint32_t g = cd.g->bits_gidx[mP][q][lane];
+        if (g >= (uint32_t)VGMAX) continue;   // (no such group: cannot happen for a shared lane)
+        const uint32_t okf = cd.g->vg_rdmask[mP][g] & ok;
+        const uint32_t fl = deadF | ((dm && zbl != 0u) ? 1u : 0u);
+        for (uint32_t r = 1; r < 8; r++)
+            if (okf >> r & 1u) emit_hit(a, k, prefix_idx, jp, cd.g->vg_j[mP][g][r], 1u << r, fl);
+    }
+}
+
+// The value the recorded child instruction c writes on test row `row`
+// (the child runs there: not frozen, no pending branch past D-2).
+__device__ __forceinline__ uint32_t kid_value(uint32_t c, const TestState &row)
+{
+    const uint32_t op = insn_op(c), rd = insn_rd(c);
+    const uint32_t av = row.r[op == OP_ADDI ? rd : insn_rs(c)], bv = row.r[insn_rt(c)];
+    uint32_t f;
+    bool fault;
+    return alu_eval(op, av, bv, insn_imm(c), row.flags, &f, &fault) & 0xFFFFu;
+}
+
+// Remaining tests (t0..T-1) of packed survivors gq[0..cnt), one per lane,
+// against their child's state (N with x replaced by the child's value),
+// with an early exit; survivors give one hit per allowed rd.  kid: the
+// pass's first recorded child; lvp: N's live mask; cpos = D - 2.
+__device__ __forceinline__ void group_drain(const DevTask &task, const ScanArgs &a, int k, const CandDev &cd,
+                                            const TestState *N, const ShareKid *kid, uint32_t lvp, uint32_t cpos,
+                                            const uint32_t *gq, uint64_t skipm, uint32_t cnt, int t0, int lane,
+                                            WaveCounters &wc, uint32_t (*tst)[2], Pool &pool, uint32_t prefix_idx,
+                                            uint32_t js, uint32_t deadF)
+{
+    const int T = task.T;
+    const bool valid = (uint32_t)lane < cnt;
+    const uint32_t e = valid ? gq[lane] : 0u;
+    const uint32_t q = e & 15u, bit = (e >> 4) & 63u, jj = (e >> 19) & 3u;
+    uint32_t ok = valid ? (e >> 10) & 0xFFu : 0u;
+    bool flt = (e >> 18) & 1u;
+    const ShareKid &kd = kid[jj];
+    const uint32_t cc = kd.cc;
+    const uint32_t ckid = cc & 0xFFFFu, cj = cc >> 16, xj = insn_rd(ckid);
+    const uint32_t mj = ((lvp | (1u << xj)) >> 1) & 127u;
+    const uint32_t xa = bit >> 3, yb = bit & 7u;
+    const uint32_t aimm = addi_imm_of(yb);
+    const uint32_t aimm2 = (aimm & 0xFFFFu) * 0x10001u;
+    const bool divmod = q == OP_DIV - OP_ADD || q == OP_MOD - OP_ADD;
+    bool any_div = any_lane(divmod && ok != 0u);
+    const bool tst8 = tst && (pool.nd & 7u) == 0u;
+    pool.nd++;
+    uint32_t n_ev = 0;
+    // the child's x on test t: recorded for the first SH_TA tests, else
+    // computed from N's row
+    auto xval = [&](int t) -> uint32_t {
+        const TestState &row = N[t];
+        if (t < SH_TA) return kd.xv[t];
+        const bool runs = row.frozen == 0u && (uint32_t)row.resume <= cpos;
+        return runs ? kid_value(ckid, row) : (uint32_t)row.r[xj];
+    };
+    auto val = [&](uint32_t r, int t) -> uint32_t { return r == xj ? xval(t) : (uint32_t)N[t].r[r]; };
+    for (int t_ = t0; t_ < T; t_++) {
+        const int tt = (int)uni((uint32_t)t_);
+        const uint64_t al = bal(ok != 0u);
+        if (al == 0ull) break;
+        const uint32_t before = (uint32_t)__popcll(al);
+        n_ev += before;
+        if (any_div) any_div = any_lane(divmod && ok != 0u);
+        if (!any_div && tt + 1 < T && ((skipm >> tt) & 3ull) == 0ull) {
+            const uint32_t tw0 = task.tgtw[tt], tw1 = task.tgtw[tt + 1];
+            const uint32_t tg2 = (tw0 & 0xFFFFu) | (tw1 << 16);
+            const uint32_t x2 = val(xa, tt) | (val(xa, tt + 1) << 16);
+            const uint32_t y2 = val(yb, tt) | (val(yb, tt + 1) << 16);
+            const u16x2 X = __builtin_bit_cast(u16x2, x2), Y = __builtin_bit_cast(u16x2, y2);
+            const u16x2 S = Y & (unsigned short)15;
+            uint32_t v = __builtin_bit_cast(uint32_t, X + Y);
+            v = q == 1u ? __builtin_bit_cast(uint32_t, X - Y) : v;
+            v = q == 2u ? __builtin_bit_cast(uint32_t, X * Y) : v;
+            v = q == 5u ? (x2 & y2) : v;
+            v = q == 6u ? (x2 | y2) : v;
+            v = q == 7u ? (x2 ^ y2) : v;
+            v = q == 8u ? __builtin_bit_cast(uint32_t, X << S) : v;
+            v = q == 9u ? __builtin_bit_cast(uint32_t, X >> S) : v;
+            v = q == 10u ? __builtin_bit_cast(uint32_t, __builtin_bit_cast(i16x2, x2) >> __builtin_bit_cast(i16x2, S))
+                         : v;
+            v = q == 11u ? __builtin_bit_cast(uint32_t, X + __builtin_bit_cast(u16x2, aimm2)) : v;
+            const uint32_t d = v ^ tg2;
+            bool m0 = (d & 0xFFFFu) == 0u, m1 = (d >> 16) == 0u;
+            if (q == 12u) {
+                m0 = (tw0 >> (16 + bit)) & 1u;
+                m1 = (tw1 >> (16 + bit)) & 1u;
+            }
+            const uint32_t mid = (uint32_t)__popcll(bal(ok != 0u && m0));
+            n_ev += mid;
+            if (!(m0 && m1)) ok = 0;
+            if (tst8) {
+                const uint32_t after = (uint32_t)__popcll(bal(ok != 0u));
+                if (lane == 0) {
+                    tst[tt][0] += before;
+                    tst[tt][1] += before - mid;
+                    tst[tt + 1][0] += mid;
+                    tst[tt + 1][1] += mid - after;
+                }
+            }
+            t_ = tt + 1;
+            continue;
+        }
+        const bool sk = (skipm >> tt) & 1ull;
+        const uint32_t tw = task.tgtw[tt];
+        const uint32_t tg = tw & 0xFFFFu;
+        if (sk) {
+            // the child did not run this test: its row is N's
+            ok &= row_regmatch_uni(&N[tt], tg);
+        } else {
+            const uint32_t x = val(xa, tt), y = val(yb, tt);
+            const uint32_t sh = y & 15u;
+            uint32_t v = x + y;
+            v = q == 1u ? x - y : v;
+            v = q == 2u ? x * y : v;
+            v = q == 5u ? (x & y) : v;
+            v = q == 6u ? (x | y) : v;
+            v = q == 7u ? (x ^ y) : v;
+            v = q == 8u ? x << sh : v;
+            v = q == 9u ? x >> sh : v;
+            v = q == 10u ? (uint32_t)((int32_t)(int16_t)(uint16_t)x >> sh) : v;
+            v = q == 11u ? x + aimm : v;
+            bool match = (v & 0xFFFFu) == tg;
+            if (q == 12u) match = (tw >> (16 + bit)) & 1u;
+            if (any_div && divmod) {
+                if (y == 0u) {
+                    // a zero divisor stops the VM: rd keeps the child's value
+                    const uint32_t rn = row_regmatch(&N[tt], tg);
+                    ok &= (rn & ~(1u << xj)) | ((xval(tt) == tg ? 1u : 0u) << xj);
+                    flt = true;
+                    match = true;
+                } else {
+                    const int32_t sx = (int16_t)(uint16_t)x, sy = (int16_t)(uint16_t)y;
+                    const int32_t qt = sdiv16_q(sx, sy);
+                    match = ((uint32_t)(q == OP_DIV - OP_ADD ? qt : sx - qt * sy) & 0xFFFFu) == tg;
+                }
+            }
+            if (!match) ok = 0;
+        }
+        if (tst8) {
+            const uint32_t after = (uint32_t)__popcll(bal(ok != 0u));
+            if (lane == 0) {
+                tst[tt][0] += before;
+                tst[tt][1] += before - after;
+            }
+        }
+    }
+    if (lane == 0) wc.c[2] += n_ev;
+    if (ok != 0u) {
+        const uint32_t g = cd.g->bits_gidx[mj][q][bit];
+        const uint32_t okf = g < (uint32_t)VGMAX ? cd.g->vg_rdmask[mj][g] & ok : 0u;
+        const uint32_t fl = deadF | ((divmod && flt) ? 1u : 0u);
+        JPath jp;
+        jp.lds = nullptr;
+        jp.j0 = k == 3 ? js : cj;
+        jp.j1 = cj;
+        for (uint32_t r = 1; r < 8; r++)
+            if (okf >> r & 1u) emit_hit(a, k, prefix_idx, jp, cd.g->vg_j[mj][g][r], 1u << r, fl);
+    }
+}
+
+// The lanes that read x of gn (<= 4) recorded children kid[0..gn): phase A
+// over the first TA (<= SH_TA) tests, then the survivors through
+// group_drain.
+__device__ __forceinline__ void run_group(const DevTask &task, const ScanArgs &a, int k, const CandDev &cd,
+                                          const TestState *N, ShareLds &sh, WaveLds &sv, const ShareKid *kid,
+                                          uint32_t gn, uint32_t lvp, uint32_t cpos, uint64_t skipm, int TA, int lane,
+                                          WaveCounters &wc, uint32_t (*tst)[2], Pool &pool, uint32_t prefix_idx,
+                                          uint32_t js, uint32_t deadF)
+{
+    const uint32_t j = (uint32_t)lane >> 4, p = (uint32_t)lane & 15u;
+    const bool in = j < gn;
+    const ShareKid &kd = kid[in ? j : 0u];
+    const uint32_t xj = in ? insn_rd(kd.cc) : 0u;
+    const uint32_t mj = ((lvp | (1u << xj)) >> 1) & 127u;
+    const bool lo = p < 8u;
+    const uint32_t o = p & 7u;
+    // (x, x) appears at p = x and p = 8 + x: the second copy is idle
+    const bool valid = in && (lo || o != xj);
+    const uint32_t ra = lo ? xj : o, rb = lo ? o : xj;
+    const uint32_t bit = (ra << 3) | rb;
+    const uint32_t kb = valid ? cd.g->kind_bits[mj][bit] : 0u;
+    LaneKinds g;
+    g.kl = kb & ~0x18u;
+    g.okd = (kb >> 3 & 1u) ? 0xFEu : 0u;
+    g.okm = (kb >> 4 & 1u) ? 0xFEu : 0u;
+    g.fault = g.vmd = g.vmm = false;
+    const uint32_t addi_imm = addi_imm_of(rb);
+    uint32_t rm_and = 0xFFu, ev = 0;
+    for (int t = 0; t < TA; t++) {
+        const uint32_t nal = (uint32_t)__popc(g.kl) + (g.okd != 0u ? 1u : 0u) + (g.okm != 0u ? 1u : 0u);
+        if (!any_lane(nal != 0u)) break;
+        if (rm_and == 0xFFu) ev += nal;
+        const uint32_t tw = task.tgtw[t];
+        if ((skipm >> t) & 1ull) {
+            const uint32_t rmt = row_regmatch_uni(&N[t], tw & 0xFFFFu);
+            g.okd &= rmt;
+            g.okm &= rmt;
+            rm_and &= rmt;
+            continue;
+        }
+        const uint32_t xv = kd.xv[t];
+        const uint32_t nv = N[t].r[o];
+        const uint32_t ov = o == xj ? xv : nv;
+        const uint32_t X = lo ? xv : ov, Y = lo ? ov : xv;
+        lane_test<false>(g, X, Y, tw, bit, addi_imm, [&]() {
+            const uint32_t tg = tw & 0xFFFFu;
+            const uint32_t rn = row_regmatch_uni(&N[t], tg);
+            return (rn & ~(1u << xj)) | ((xv == tg ? 1u : 0u) << xj);
+        });
+        if (tst && t == 2) {
+            const uint32_t left = (uint32_t)__popc(g.kl) + (g.okd != 0u ? 1u : 0u) + (g.okm != 0u ? 1u : 0u);
+            __hip_atomic_fetch_add(&sv.pa2[lane], (unsigned long long)nal | ((unsigned long long)(nal - left) << 32),
+                                   __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+        }
+    }
+    if (ev) __hip_atomic_fetch_add(&sv.evl[lane], ev, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WAVEFRONT);
+    if (rm_and == 0u) return;
+    const uint32_t sm = g.kl | (g.okd != 0u ? 1u << 3 : 0u) | (g.okm != 0u ? 1u << 4 : 0u);
+    const uint32_t c = (uint32_t)__popc(sm);
+    const uint32_t incl = wave_incl_scan32(c);
+    const uint32_t total = uni(__builtin_amdgcn_readlane(incl, 63));
+    if (total == 0u) return;
+    const uint32_t excl = incl - c;
+    uint32_t qn = 0, pushed = 0;
+    for (;;) {
+        if (pushed < total) {
+            const uint32_t take = min(GQ_CAP - qn, total - pushed);
+            if (c != 0u && excl < pushed + take && excl + c > pushed) {
+                uint32_t mk = sm, gi = excl;
+                while (mk) {
+                    const uint32_t q = (uint32_t)__builtin_ctz(mk);
+                    mk &= mk - 1u;
+                    if (gi >= pushed && gi < pushed + take) {
+                        const uint32_t okq = q == 3u ? g.okd : (q == 4u ? g.okm : rm_and);
+                        sh.gq[qn + (gi - pushed)] = q | (bit << 4) | (okq << 10) |
+                                                    (((q == 3u || q == 4u) && g.fault) ? 1u << 18 : 0u) | (j << 19);
+                    }
+                    gi++;
+                }
+            }
+            qn += take;
+            pushed += take;
+        }
+        const bool all = pushed == total;
+        const uint32_t cnt = qn >= 64u ? 64u : (all ? qn : 0u);
+        if (cnt) {
+            wave_sync();
+            group_drain(task, a, k, cd, N, kid, lvp, cpos, sh.gq, skipm, cnt, TA, lane, wc, tst, pool, prefix_idx, js,
+                        deadF);
+            wave_sync();
+            const uint32_t rest = qn - cnt;
+            if ((uint32_t)lane < rest) sh.gq[lane] = sh.gq[64 + lane];
+            qn = rest;
+            wave_sync();
+        }
+        if (all && qn == 0u) break;
+    }
+}
+
 // k == 0: the prefix is the whole kernel; check its registers directly.
 __device__ __forceinline__ void leaf_parent(const DevTask &task, const ScanArgs &a, const TestState *P, uint32_t live,
                             uint32_t prefix_idx, int lane, WaveCounters &wc)
@@ -835,11 +1268,13 @@ __device__ __forceinline__ void leaf_parent(const DevTask &task, const ScanArgs 
 // scanned parents (max(lch.levels, 2) - 1 of them); the scanned parents
 // themselves (level k - 1) sit in the NSLOT slots of the wave's WaveLds,
 // where they stay while their survivors wait in the pooled queue.
-template <bool MULTI, int KF>
+template <bool MULTI, int KF, bool SHARED>
 __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs &a1, const DevTask &task1)
 {
     // KF: the suffix length of every item, fixed at compile time (2 or 3;
-    // 0: read from the item)
+    // 0: read from the item).  SHARED: blocks of 2- and 3-instruction
+    // suffixes scan the last position shared per block (items whose tile
+    // asks for it; the pool then uses slots 0 and 1)
     constexpr bool K23 = KF != 0;
     extern __shared__ TestState s_dyn[];
     __shared__ WaveLds s_wave[SCAN_WAVES];
@@ -867,6 +1302,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     auto lev = [&](int L) -> TestState * { return s_dyn + ((uint32_t)w * nreg + (uint32_t)L) * MAXT; };
     Pool pool;
     pool.qn = pool.pend = pool.cur = pool.nd = 0;
+    pool.ns = SHARED ? 2u : NSLOT;
 
     WaveCounters wc;
     wc.c = sv.wc;
@@ -999,6 +1435,202 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
             PackedState pl = {};
             if (act) pl = load_ps(&lev(Lu - 1)[lane]);
             const uint32_t cpos = (uint32_t)(a.D - 2);
+            if (SHARED && a.tile.mode == TILE_SHARED && nb != 0u) {
+                // the block's node N (level Lu - 1, in LDS and in pl)
+                const TestState *N = lev(Lu - 1);
+                ShareLds &sh = *reinterpret_cast<ShareLds *>(&sv.slot[2][0]);
+                const uint32_t lpos = (uint32_t)(a.D - 1);
+                const int TA0 = task.T < (int)a.tile.mask_tests ? task.T : (int)a.tile.mask_tests;
+                const int TA = TA0 < SH_TA ? TA0 : SH_TA;
+                const uint64_t am = bal(act);
+                const uint32_t w4 = pl.w[4];
+                // what every shared-path child inherits from N: the tests
+                // it skips at the last position, its frozen tests, and
+                // whether it is dead (frozen, or out of cycles once it has
+                // run its own instruction)
+                const uint64_t skipN = bal(((w4 >> 8) & 0xFFFFu) > lpos) & am;
+                const uint64_t frzN = bal((w4 & 0xFF0000u) != 0u) & am;
+                const uint32_t exP = ((w4 >> 24) + ((act && ps_runs(w4, cpos)) ? 1u : 0u)) & 0xFFu;
+                const uint32_t mc = (uint32_t)task.max_cycles;
+                const uint64_t cycF = (bal(exP + 2u > mc) | (bal(exP + 3u > mc) & ~skipN)) & am;
+                const uint32_t deadF = (frzN | cycF) != 0ull ? 1u : 0u;
+                const uint32_t pmN = summarize_state(task, pl, act, lpos, my_tgt).pm;
+                uint32_t rm_andN = 0xFFu;
+                bool fn_any = false;
+                const uint32_t evn = node_scan(task, cand, N, (lvp >> 1) & 127u, skipN, TA, lane, sh, rm_andN, fn_any);
+                if (fn_any) {
+                    // per register, the tests (not skipped) where N holds 0
+                    uint32_t zr = 0;
+#pragma unroll
+                    for (uint32_t r = 0; r < 8; r++) {
+                        const uint64_t zm = bal(ps_reg(pl, r) == 0u) & am & ~skipN;
+                        zr = lane == (int)r ? (uint32_t)zm : zr;
+                    }
+                    if (lane < 8) sh.zb[lane] = zr;
+                }
+                // N's group tests over all lanes and over the lanes reading
+                // each register (a child is credited the lanes it shares)
+                const uint32_t ev_all = (uint32_t)wave_sum((unsigned long long)evn);
+                wave_sync();
+                sh.gq[lane] = evn;
+                wave_sync();
+                if (lane < 8) {
+                    uint32_t xsum = 0;
+                    for (int i = 0; i < 8; i++) xsum += sh.gq[lane * 8 + i] + (i != lane ? sh.gq[i * 8 + lane] : 0u);
+                    sh.xs[lane] = lane == 0 ? ev_all : xsum;
+                }
+                wave_sync();
+                const uint32_t jsb = k == 3 ? js : 0u;
+                const uint32_t mN = (lvp >> 1) & 127u;
+                // the tests on which the children run (not frozen, no
+                // pending branch past D-2)
+                const bool runs2 = act && ps_runs(w4, cpos);
+                const uint64_t runs2m = bal(runs2);
+                for (uint32_t base = 0; base < nb; base += SH_KIDS) {
+                    const uint32_t nh = min(nb - base, SH_KIDS);
+                    // pass 1: sort the children; record the ones that
+                    // write a register (their value from N's row: one ALU
+                    // operation on the uniform opcode, no general step),
+                    // count the shared ones per register written (hist: 7
+                    // bits per register, slot 0 for CMP), defer the
+                    // per-child ones (bit ci - base of gmask)
+                    uint32_t nx = 0, gmask = 0, cmask = 0;
+                    uint64_t hist = 0;
+                    for (uint32_t ci = base; ci < base + nh; ci++) {
+                        const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
+                        const uint32_t op = insn_op(c), x = insn_rd(c);
+                        const uint32_t cbit = 1u << (ci - base);
+                        if (op == OP_BR) {
+                            gmask |= cbit;
+                            continue;
+                        }
+                        if (op == OP_CMP) {
+                            // writes no register: every lane shared
+                            if (pmN & 0xFEu) {
+                                gmask |= cbit;
+                            } else {
+                                cmask |= cbit;
+                                hist += 1ull;
+                            }
+                            continue;
+                        }
+                        const TestState &row = N[act ? lane : 0];
+                        const uint32_t av = row.r[op == OP_ADDI ? x : insn_rs(c)], bv = row.r[insn_rt(c)];
+                        const uint32_t ov = row.r[x];
+                        // a DIV / MOD by zero on a test it runs freezes it
+                        if ((op == OP_DIV || op == OP_MOD) && (bal(bv == 0u) & runs2m) != 0ull) {
+                            gmask |= cbit;
+                            continue;
+                        }
+                        const uint32_t imm = insn_imm(c);
+                        uint32_t v = ov;
+                        switch (op) {
+#define PCA_KV(OPC) \
+    case OPC: v = op_value<OPC>(av, bv, imm); break;
+                        PCA_KV(OP_ADD)
+                        PCA_KV(OP_SUB)
+                        PCA_KV(OP_MUL)
+                        PCA_KV(OP_DIV)
+                        PCA_KV(OP_MOD)
+                        PCA_KV(OP_AND)
+                        PCA_KV(OP_OR)
+                        PCA_KV(OP_XOR)
+                        PCA_KV(OP_SHL)
+                        PCA_KV(OP_SHR)
+                        PCA_KV(OP_ASR)
+                        PCA_KV(OP_LDI)
+                        PCA_KV(OP_ADDI)
+#undef PCA_KV
+                        default: break;
+                        }
+                        const uint32_t xv = runs2 ? v : ov;
+                        const uint64_t hitm = bal(xv == my_tgt) & am;
+                        if (((pmN & ~(1u << x)) | (hitm == am ? 1u << x : 0u)) & 0xFEu) {
+                            gmask |= cbit;
+                            continue;
+                        }
+                        hist += 1ull << (7u * x);
+                        ShareKid &kd = sh.kid[nx];
+                        if (lane == 0) {
+                            kd.cc = c | ((ci + j0) << 16);
+                            kd.xm = (uint32_t)hitm;
+                        }
+                        if (lane < SH_TA) kd.xv[lane] = (uint16_t)xv;
+                        nx++;
+                    }
+                    {
+                        // counters of the shared children: per register x
+                        // written (lane x; lane 0: CMP) their count times
+                        // the candidates, suffixes and shared group tests
+                        // of one such child
+                        const uint32_t xl = (uint32_t)lane & 7u;
+                        const uint32_t cnt = lane < 8 ? (uint32_t)(hist >> (7u * xl)) & 127u : 0u;
+                        const uint32_t mx = xl == 0u ? mN : ((lvp | (1u << xl)) >> 1) & 127u;
+                        const uint32_t credit = sh.xs[0] - (xl == 0u ? 0u : sh.xs[xl]);
+                        const unsigned long long c0 = wave_sum((unsigned long long)(cnt * cand.cand_total[mx]));
+                        const unsigned long long c1 = wave_sum((unsigned long long)(cnt * (uint32_t)cand.count[mx]));
+                        const unsigned long long c2 = wave_sum((unsigned long long)cnt * credit);
+                        if (lane == 0) {
+                            wc.c[0] += c0;
+                            wc.c[1] += c1;
+                            wc.c[2] += c2;
+                        }
+                    }
+                    wave_sync();
+                    // pass 2: the shared lanes' hits (only when some group
+                    // of N matches every test)
+                    if (fn_any) {
+                        for (uint32_t i = 0; i < nx; i++) {
+                            const uint32_t cc = sh.kid[i].cc, xm = sh.kid[i].xm;
+                            const uint32_t x = insn_rd(cc & 0xFFFFu);
+                            JPath jp;
+                            jp.lds = nullptr;
+                            jp.j0 = k == 3 ? js : cc >> 16;
+                            jp.j1 = cc >> 16;
+                            emit_shared(a, k, cand, sh, x, ((lvp | (1u << x)) >> 1) & 127u, rm_andN, xm, pi, jp,
+                                        deadF, lane);
+                        }
+                        for (uint32_t mk = cmask; mk; mk &= mk - 1u) {
+                            const uint32_t ci = base + (uint32_t)__builtin_ctz(mk);
+                            JPath jp;
+                            jp.lds = nullptr;
+                            jp.j0 = k == 3 ? js : j0 + ci;
+                            jp.j1 = j0 + ci;
+                            emit_shared(a, k, cand, sh, 8u, mN, rm_andN, 0u, pi, jp, deadF, lane);
+                        }
+                    }
+                    // pass 3: the lanes that read x, four children a pass
+                    for (uint32_t g0 = 0; g0 < nx; g0 += 4u) {
+                        run_group(task, a, k, cand, N, sh, sv, &sh.kid[g0], min(nx - g0, 4u), lvp, cpos, skipN, TA,
+                                  lane, wc, tst, pool, pi, jsb, deadF);
+                        wave_sync();
+                    }
+                    // pass 4: the per-child path (branches, faults, a
+                    // register already equal to the target)
+                    for (uint32_t mk = gmask; mk; mk &= mk - 1u) {
+                        const uint32_t ci = base + (uint32_t)__builtin_ctz(mk);
+                        const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
+                        PackedState st = pl;
+                        if (act && ps_runs(st.w[4], cpos)) step_packed<false, false>(st, c, cpos);
+                        TestState *PP = sv.slot[pool.cur];
+                        if (act) store_ps(&PP[lane], st);
+                        const ParentSummary ps = summarize_state(task, st, act, lpos, my_tgt);
+                        const uint32_t lvL = live_after(lvp, c);
+                        JPath jpath;
+                        jpath.lds = nullptr;
+                        jpath.j0 = k == 3 ? js : j0 + ci;
+                        jpath.j1 = j0 + ci;
+                        wave_sync();
+                        if (ps.pm == 0u)
+                            scan_parent_bits(task, a, k, cand, PP, lvL, pi, jpath, lane, wc, sv, pool, tst, ps);
+                        else
+                            scan_parent(task, a, k, cand, PP, lvL, pi, jpath, lane, wc, sv.cq, ps);
+                        wave_sync();
+                    }
+                }
+                if (lane == 0) wc.c[3] += nb;
+                continue;
+            }
             for (uint32_t ci = 0; ci < nb; ci++) {
                 const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
                 TestState *PP = sv.slot[pool.cur];
@@ -1124,7 +1756,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
 // (188, none): 4.70 s of kernel time against 4.97 and 4.80.
 __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false, 0>(lch, a1, task1);
+    scan_body<false, 0, false>(lch, a1, task1);
 }
 
 // One-item launches of 2- and 3-instruction suffixes (the block's children
@@ -1132,17 +1764,31 @@ __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_one(ScanLaunch lch, Sc
 // the suffix length a constant (the hit paths' per-position selects fold).
 __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_k2(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false, 2>(lch, a1, task1);
+    scan_body<false, 2, false>(lch, a1, task1);
 }
 
 __global__ __launch_bounds__(SCAN_WG, 6) void scan_kernel_k3(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<false, 3>(lch, a1, task1);
+    scan_body<false, 3, false>(lch, a1, task1);
+}
+
+// The same with the last position shared per block (tile mode TILE_SHARED).
+#ifndef PCA_SHARED_WAVES
+#define PCA_SHARED_WAVES 6
+#endif
+__global__ __launch_bounds__(SCAN_WG, PCA_SHARED_WAVES) void scan_kernel_k2s(ScanLaunch lch, ScanArgs a1, DevTask task1)
+{
+    scan_body<false, 2, true>(lch, a1, task1);
+}
+
+__global__ __launch_bounds__(SCAN_WG, PCA_SHARED_WAVES) void scan_kernel_k3s(ScanLaunch lch, ScanArgs a1, DevTask task1)
+{
+    scan_body<false, 3, true>(lch, a1, task1);
 }
 
 __global__ __launch_bounds__(SCAN_WG, 3) void scan_kernel_multi(ScanLaunch lch, ScanArgs a1, DevTask task1)
 {
-    scan_body<true, 0>(lch, a1, task1);
+    scan_body<true, 0, true>(lch, a1, task1);
 }
 
 // The kernel argument of a one-item launch: its work item with the launch's
@@ -1166,6 +1812,16 @@ ScanArgs one_item_args(const ScanLaunch &l, const ScanItem &it)
     a.counters = l.counters;
     a.work = l.work;
     return a;
+}
+
+// The kernel of a one-item launch: by suffix length and tile mode.
+typedef void (*ScanKernelFn)(ScanLaunch, ScanArgs, DevTask);
+ScanKernelFn scan_kernel_for(const ScanItem &it)
+{
+    const bool shared = it.tile.mode == TILE_SHARED;
+    if (it.k == 2) return shared ? scan_kernel_k2s : scan_kernel_k2;
+    if (it.k == 3) return shared ? scan_kernel_k3s : scan_kernel_k3;
+    return scan_kernel_one;
 }
 
 // ------------------------------------------------------------ wavefront
@@ -1720,10 +2376,18 @@ hipError_t scan_occupancy(bool multi, uint32_t levels, int *blocks_per_cu)
     const size_t lds = scan_dyn_lds(levels);
     if (multi) return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_multi, SCAN_WG, lds);
     // a one-item launch's levels are its suffix length k (k >= 1)
-    if (levels == 2u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_k2, SCAN_WG, lds);
-    if (levels == 3u)
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_k3, SCAN_WG, lds);
+    if (levels == 2u || levels == 3u) {
+        // the shared and per-child kernels of this length: the smaller
+        // occupancy of the two sizes the persistent grid
+        int b0 = 0, b1 = 0;
+        hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &b0, levels == 2u ? scan_kernel_k2s : scan_kernel_k3s, SCAN_WG, lds);
+        if (e != hipSuccess) return e;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&b1, levels == 2u ? scan_kernel_k2 : scan_kernel_k3,
+                                                         SCAN_WG, lds);
+        *blocks_per_cu = b0 < b1 ? b0 : b1;
+        return e;
+    }
     return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks_per_cu, scan_kernel_one, SCAN_WG, lds);
 }
 
@@ -1752,7 +2416,7 @@ void scan_node_params(const ScanLaunch &l, const ScanItem &it, const DevTask &ta
     args->ptrs[0] = &args->l;
     args->ptrs[1] = &args->a1;
     args->ptrs[2] = &args->task;
-    p->func = reinterpret_cast<void *>(it.k == 2 ? scan_kernel_k2 : (it.k == 3 ? scan_kernel_k3 : scan_kernel_one));
+    p->func = reinterpret_cast<void *>(scan_kernel_for(it));
     p->gridDim = dim3((unsigned)grid);
     p->blockDim = dim3(SCAN_WG);
     p->sharedMemBytes = (unsigned)scan_dyn_lds(l.levels);
@@ -1791,12 +2455,8 @@ void launch_scan(const ScanLaunch &l, const ScanItem *host_items, const DevTask 
 {
     const size_t lds = scan_dyn_lds(l.levels);
     const ScanArgs a1 = one_item_args(l, host_items[0]);
-    if (l.n_items == 1 && host_items[0].k == 2)
-        hipLaunchKernelGGL(scan_kernel_k2, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
-    else if (l.n_items == 1 && host_items[0].k == 3)
-        hipLaunchKernelGGL(scan_kernel_k3, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
-    else if (l.n_items == 1)
-        hipLaunchKernelGGL(scan_kernel_one, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
+    if (l.n_items == 1)
+        hipLaunchKernelGGL(scan_kernel_for(host_items[0]), dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
     else
         hipLaunchKernelGGL(scan_kernel_multi, dim3(grid), dim3(SCAN_WG), lds, st, l, a1, task0);
 }

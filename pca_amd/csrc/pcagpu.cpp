@@ -186,22 +186,31 @@ int grid_for_aux(pcagpu *h, uint64_t units, AuxKernel which)
 // each try one candidate on real work, and the one with the least kernel
 // time per parent state scanned is kept for the rest of the handle's life.
 // A launch with too few parents to time does not count as a try.
-constexpr ScanTile kTiles[] = {{64, 2}, {64, 1}, {64, 3}, {32, 2}, {16, 2}};
+constexpr ScanTile kTiles[] = {{64, 2, TILE_SHARED}, {64, 1, TILE_SHARED}, {64, 3, TILE_SHARED},
+                               {32, 2, TILE_SHARED}, {16, 2, TILE_SHARED}};
 constexpr int kNTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
 constexpr uint64_t kTileMinParents = 200000;
 
 ScanTile current_tile(const pcagpu *h, int *cand)
 {
     *cand = -1;
-    if (h->opts.scan_tile) {
+    // bits 16..23 of the option: 1 scans every child on its own
+    const uint8_t mode = ((h->opts.scan_tile >> 16) & 0xFFu) == 1u ? TILE_PER_CHILD : TILE_SHARED;
+    if (h->opts.scan_tile & 0xFFFFu) {
         const uint32_t bk = h->opts.scan_tile & 0xFFu, mt = (h->opts.scan_tile >> 8) & 0xFFu;
         return ScanTile{(uint16_t)std::min<uint32_t>(std::max<uint32_t>(bk, 1u), BLOCK_KIDS_MAX),
-                        (uint16_t)std::min<uint32_t>(std::max<uint32_t>(mt, 1u), 3u)};
+                        (uint8_t)std::min<uint32_t>(std::max<uint32_t>(mt, 1u), 6u), mode};
     }
     const int k = std::min(std::max(h->k, 0), (int)MAXLEV);
-    if (h->tile_best[k] >= 0) return kTiles[h->tile_best[k]];
-    *cand = h->tile_tried[k];
-    return kTiles[*cand];
+    ScanTile t;
+    if (h->tile_best[k] >= 0) {
+        t = kTiles[h->tile_best[k]];
+    } else {
+        *cand = h->tile_tried[k];
+        t = kTiles[*cand];
+    }
+    t.mode = mode;
+    return t;
 }
 
 // (a launch queued before the previous one's feedback arrived may have been
@@ -404,7 +413,8 @@ int settle_scan(pcagpu *h)
         if ((rc = eager_scan(h, l, it, grid, p.n, p.n_blocks, false, false))) return rc;
         HIP_TRY(hipStreamSynchronize(h->stream));
     }
-    h->stats.scan_tile = (uint64_t)p.tile.block_kids | ((uint64_t)p.tile.mask_tests << 8);
+    h->stats.scan_tile =
+        (uint64_t)p.tile.block_kids | ((uint64_t)p.tile.mask_tests << 8) | ((uint64_t)p.tile.mode << 16);
     h->stats.candidates += o->cnt[0];
     h->stats.suffixes += o->cnt[1];
     h->stats.tests_evaluated += o->cnt[2];

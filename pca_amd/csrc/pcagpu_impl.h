@@ -78,6 +78,8 @@ using pca::HitRec;
 using pca::MatchRec;
 using pca::ScanBlock;
 using pca::ScanTile;
+using pca::TILE_PER_CHILD;
+using pca::TILE_SHARED;
 using pca::ScanGroup;
 using pca::ScanItem;
 

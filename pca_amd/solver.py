@@ -33,15 +33,19 @@ def _sol_tuple(s):
 
 class Solver:
     def __init__(self, task, device=0, batch_capacity=0, hit_capacity=0, waves_per_cu=0, fixed_test_order=False,
-                 scan_tile=None, eager_flush=False, test_flags=0):
-        """scan_tile: None (autotuned) or (block_kids 1..64, mask_tests 1..3).
+                 scan_tile=None, eager_flush=False, test_flags=0, per_child=False):
+        """scan_tile: None (autotuned) or (block_kids 1..64, mask_tests 1..6).
+        per_child: scan every block child's last position on its own instead
+        of sharing the value groups that do not read its register (the same
+        results; the slower form, kept to check the shared one).
         test_flags: testing only (see pcagpu_opts_t.test_flags)."""
         if not isinstance(task, Task):
             raise TypeError("task must be a pca_amd.Task")
         self.task = task
         opts = Opts(device=device, batch_capacity=batch_capacity, hit_capacity=hit_capacity,
                     waves_per_cu=waves_per_cu, fixed_test_order=1 if fixed_test_order else 0,
-                    scan_tile=0 if scan_tile is None else (scan_tile[0] | (scan_tile[1] << 8)),
+                    scan_tile=(0 if scan_tile is None else (scan_tile[0] | (scan_tile[1] << 8))) |
+                    ((1 << 16) if per_child else 0),
                     eager_flush=1 if eager_flush else 0, test_flags=test_flags)
         h = C.c_void_p()
         rc = lib().pcagpu_open(C.byref(task), C.byref(opts), C.byref(h))
