@@ -487,6 +487,10 @@ def main():
     avg_launch = kernel_s / launches
     ops = OPS_PER_TEST * st["tests_evaluated"] + OPS_PER_PARENT_TEST * task.num_tests * st["parents"]
     achieved = ops / launches / avg_launch
+    # what the kernel ran: the group tests of a block's node are run once
+    # and credited to every child that shares them
+    ops_run = OPS_PER_TEST * st["tests_executed"] + OPS_PER_PARENT_TEST * task.num_tests * st["parents"]
+    shared = (st["scan_tile"] >> 16) & 0xFF == 0
     line = {
         "metric": "candidates/s (1 GPU)",
         "value": cand / wall,
@@ -505,7 +509,10 @@ def main():
                    "tests": task.num_tests, "kernel_depth": D, "split": s, "prefixes_per_step": B,
                    "device": dev},
         "roofline": {"bound": "valu-int", "achieved": achieved / 1e12, "peak": PEAK_INT_OPS / 1e12,
-                     "unit": "Tint-ops/s", "frac": achieved / PEAK_INT_OPS, "traffic": None},
+                     "unit": "Tint-ops/s", "frac": achieved / PEAK_INT_OPS, "traffic": None,
+                     "work": "group tests deciding every last-position value group (what a per-child scan runs)",
+                     "executed": {"achieved": ops_run / kernel_s / 1e12, "frac": ops_run / kernel_s / PEAK_INT_OPS,
+                                  "work": "group tests the kernel ran (shared node scans once per block)"}},
         # the per-(prefix, suffix) figure of the reference formulation: every
         # suffix replayed whole (k VM steps of 8 ops) plus 7 register compares
         # per test it reaches; counting one test per suffix is a lower bound.
@@ -516,14 +523,18 @@ def main():
                                 "achieved_lower_bound": st["suffixes"] * ((D - s) * 8 + 7) / kernel_s / 1e12,
                                 "frac_lower_bound": st["suffixes"] * ((D - s) * 8 + 7) / kernel_s / PEAK_INT_OPS,
                                 "unit": "Tint-ops/s"},
-        # one-item launches of 2- and 3-instruction suffixes run scan_kernel_k2 / k3,
+        # one-item launches of 2- and 3-instruction suffixes run scan_kernel_k2s / k3s
+        # (last position shared per block; k2 / k3 scan every child on its own),
         # other suffix lengths scan_kernel_one
-        "kernel": {"name": {2: "scan_kernel_k2", 3: "scan_kernel_k3"}.get(D - s, "scan_kernel_one"),
+        "kernel": {"name": {2: "scan_kernel_k2", 3: "scan_kernel_k3"}.get(D - s, "scan_kernel_one") +
+                           ("s" if shared and D - s in (2, 3) else ""),
                    "avg_launch_ms": avg_launch * 1e3, "launches": launches,
                    "candidates": cand, "suffixes": st["suffixes"], "tests_evaluated": st["tests_evaluated"],
                    "parents": st["parents"], "raw_hits": st["raw_hits"],
                    "kernel_candidates_per_s": cand / kernel_s if kernel_s else None,
-                   "tile": {"block_kids": st["scan_tile"] & 0xFF, "mask_tests": st["scan_tile"] >> 8}},
+                   "tests_executed": st["tests_executed"],
+                   "tile": {"block_kids": st["scan_tile"] & 0xFF, "mask_tests": (st["scan_tile"] >> 8) & 0xFF,
+                            "mode": "shared per block" if shared else "per child"}},
     }
     # HBM bytes per launch from the committed rocprofv3 PMC passes of this
     # kernel and workload (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, copy-calibrated)
@@ -534,7 +545,7 @@ def main():
             pj = json.load(f)
         w = pj.get("window", {})
         if (w.get("depth"), w.get("split"), w.get("batch"), w.get("warmup"), w.get("steps")) == \
-                (D, s, B, args.warmup, args.steps):
+                (D, s, B, args.warmup, args.steps) and pj.get("kernel") == line["kernel"]["name"]:
             line["roofline"]["traffic"] = pj["traffic_bytes_per_launch"]
             line["roofline"]["traffic_source"] = "profiles/scan_kernel_is_power2_32_d8s5.json (same window)"
     if not args.no_serving:

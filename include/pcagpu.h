@@ -100,6 +100,11 @@ typedef struct {
     uint64_t scan_tile;       /* tile of the last scan launch: block_kids |
                                  mask_tests << 8 | mode << 16 (0: last
                                  position shared per block, 1: per child)    */
+    uint64_t tests_executed;  /* group tests the scan kernels ran: below
+                                 tests_evaluated (the tests deciding every
+                                 group, which a per-child scan runs) by what
+                                 a block's node decides once for its
+                                 children                                    */
 } pcagpu_stats_t;
 
 typedef struct {

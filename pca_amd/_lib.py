@@ -135,7 +135,8 @@ class SearchState(C.Structure):
 class Stats(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in (
         "candidates", "suffixes", "tests_evaluated", "prefixes", "raw_hits", "kernel_ns",
-        "launches", "frontier_nodes", "frontier_kept", "expand_ns", "oep_collisions", "parents", "scan_tile")]
+        "launches", "frontier_nodes", "frontier_kept", "expand_ns", "oep_collisions", "parents", "scan_tile",
+        "tests_executed")]
 
     def as_dict(self):
         return {n: int(getattr(self, n)) for n, _ in self._fields_}

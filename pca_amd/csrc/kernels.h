@@ -215,6 +215,9 @@ struct ScanLaunch {
     // would otherwise serialise the waves
     uint32_t grab;
     uint32_t pad;
+    // null, or [0] group tests credited to a block's children from their
+    // node's shared scan, [1] group tests that shared scan ran
+    unsigned long long *xcounters;
 };
 
 // Kernel-node form of a one-item scan launch, for a graph-captured flush:

@@ -320,6 +320,7 @@ struct WaveLds {
     uint32_t meta[NSLOT][2];              // per slot, prefix index; live-mask index | dead << 8
     uint32_t path[NSLOT][MAXLEV];         // per slot, candidate indices at positions s..D-2
     unsigned long long wc[4];             // work counters (WaveCounters)
+    unsigned long long wx[2];             // shared-scan counters (ScanLaunch::xcounters)
     uint32_t evl[64];                     // per lane, phase-A group evaluations
     uint32_t tst[MAXT][2];                // survivor-phase test statistics
     // per lane, phase A at test position 2 (the only reordered position it
@@ -900,7 +901,7 @@ __device__ __forceinline__ uint32_t addi_imm_of(uint32_t b)
 // matches every test writes the lanes' matches to sh.fn.
 __device__ __forceinline__ uint32_t node_scan(const DevTask &task, const CandDev &cd, const TestState *N, uint32_t mN,
                                               uint64_t skipm, int TA, int lane, ShareLds &sh, uint32_t &rm_and_out,
-                                              bool &fn_any)
+                                              bool &fn_any, uint32_t &ran)
 {
     const int T = task.T;
     const uint32_t ra = (uint32_t)lane >> 3, rb = (uint32_t)lane & 7u;
@@ -918,6 +919,7 @@ __device__ __forceinline__ uint32_t node_scan(const DevTask &task, const CandDev
         if (!any_lane(g.kl != 0u || g.okd != 0u || g.okm != 0u || g.vmd || g.vmm)) break;
         const uint32_t nal = (uint32_t)__popc(g.kl) + (g.okd != 0u ? 1u : 0u) + (g.okm != 0u ? 1u : 0u);
         if (t >= TA || rm_and == 0xFFu) ev += nal;
+        ran += nal;
         const uint32_t tw = task.tgtw[t];
         if ((skipm >> t) & 1ull) {
             const uint32_t rmt = row_regmatch_uni(&N[t], tw & 0xFFFFu);
@@ -1284,6 +1286,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     WaveLds &sv = s_wave[w];
     for (int i = lane; i < MAXT * 2; i += 64) (&sv.tst[0][0])[i] = 0u;
     if (lane < 4) sv.wc[lane] = 0ull;
+    if (lane < 2) sv.wx[lane] = 0ull;
     sv.evl[lane] = 0u;
     sv.pa2[lane] = 0ull;
     uint32_t (*tst)[2] = (!MULTI && lch.tstat) ? sv.tst : nullptr;
@@ -1455,9 +1458,10 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 const uint64_t cycF = (bal(exP + 2u > mc) | (bal(exP + 3u > mc) & ~skipN)) & am;
                 const uint32_t deadF = (frzN | cycF) != 0ull ? 1u : 0u;
                 const uint32_t pmN = summarize_state(task, pl, act, lpos, my_tgt).pm;
-                uint32_t rm_andN = 0xFFu;
+                uint32_t rm_andN = 0xFFu, ran = 0;
                 bool fn_any = false;
-                const uint32_t evn = node_scan(task, cand, N, (lvp >> 1) & 127u, skipN, TA, lane, sh, rm_andN, fn_any);
+                const uint32_t evn =
+                    node_scan(task, cand, N, (lvp >> 1) & 127u, skipN, TA, lane, sh, rm_andN, fn_any, ran);
                 if (fn_any) {
                     // per register, the tests (not skipped) where N holds 0
                     uint32_t zr = 0;
@@ -1471,6 +1475,8 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 // N's group tests over all lanes and over the lanes reading
                 // each register (a child is credited the lanes it shares)
                 const uint32_t ev_all = (uint32_t)wave_sum((unsigned long long)evn);
+                const unsigned long long ran_all = wave_sum((unsigned long long)ran);
+                if (lane == 0) sv.wx[1] += ran_all;
                 wave_sync();
                 sh.gq[lane] = evn;
                 wave_sync();
@@ -1574,6 +1580,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                             wc.c[0] += c0;
                             wc.c[1] += c1;
                             wc.c[2] += c2;
+                            sv.wx[0] += c2;
                         }
                     }
                     wave_sync();
@@ -1737,6 +1744,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
     }
     wave_sync();
     if (lane < 4) atomicAdd(&lch.counters[lane], sv.wc[lane]);
+    if (SHARED && lch.xcounters && lane < 2) atomicAdd(&lch.xcounters[lane], sv.wx[lane]);
     if (!MULTI && lch.tstat) {
         __syncthreads();
         for (int i = threadIdx.x; i < MAXT * 2; i += SCAN_WG) {

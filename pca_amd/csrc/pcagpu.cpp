@@ -181,6 +181,10 @@ int grid_for_aux(pcagpu *h, uint64_t units, AuxKernel which)
     return (int)std::max<uint64_t>(wgs, 1);
 }
 
+// d_counters: [0..3] a scan launch's counters, [4..5] the wavefront's, [6]
+// the ALU digest's, [XCNT..XCNT+1] the scan's shared-scan counters.
+constexpr int XCNT = 8;
+
 // Scan tile autotuner.  The tile (children per block, tests run by wave
 // masks) is chosen per suffix length k: the first launches of that shape
 // each try one candidate on real work, and the one with the least kernel
@@ -230,7 +234,9 @@ void tile_feedback(pcagpu *h, int cand, int k_launch, float ms, uint64_t parents
 // Read-back area of one scan launch (pinned host memory): counters,
 // hit count, per-test survivor counts.
 struct LaunchOut {
-    unsigned long long cnt[4];
+    // [0..3] the scan counters, [XCNT..XCNT+1] its shared-scan counters
+    // (the words between belong to other kernels and are read back unused)
+    unsigned long long cnt[XCNT + 2];
     unsigned long long ts[2 * MAXT];
     uint32_t hits;
     uint32_t pad;
@@ -253,7 +259,7 @@ int graph_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
         hipGraphNode_t zero, up, ev0, ev1, r0, r1, r2;
         uint32_t *zp_ptr[4] = {reinterpret_cast<uint32_t *>(h->d_work.p), reinterpret_cast<uint32_t *>(h->d_tstat.p),
                                reinterpret_cast<uint32_t *>(h->d_counters.p), h->d_hit_count.p};
-        const uint32_t zp_n[4] = {2, 4 * MAXT, 8, 1};
+        const uint32_t zp_n[4] = {2, 4 * MAXT, 2 * (XCNT + 2), 1};
         ZeroNodeArgs za;
         hipKernelNodeParams zp;
         zero_node_params(zp_ptr, zp_n, &za, &zp);
@@ -302,7 +308,7 @@ int eager_scan(pcagpu *h, const ScanLaunch &l, const ScanItem &it, int grid, uin
     if (up_prefixes || up_blocks) HIP_TRY(hipEventRecord(h->ev_up, h->stream));
     HIP_TRY(hipMemsetAsync(h->d_work.p, 0, sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_tstat.p, 0, 2 * MAXT * sizeof(unsigned long long), h->stream));
-    HIP_TRY(hipMemsetAsync(h->d_counters.p, 0, 4 * sizeof(unsigned long long), h->stream));
+    HIP_TRY(hipMemsetAsync(h->d_counters.p, 0, (XCNT + 2) * sizeof(unsigned long long), h->stream));
     HIP_TRY(hipMemsetAsync(h->d_hit_count.p, 0, sizeof(uint32_t), h->stream));
     HIP_TRY(hipEventRecord(h->ev0, h->stream));
     launch_scan(l, &it, h->dt, grid, h->stream);
@@ -343,6 +349,7 @@ int scan_args(pcagpu *h, const pca_prefix_t *src, uint32_t n, uint64_t units, Sc
     const int grid = grid_for(h, units, false, l->levels);
     l->grab = scan_grab(k, units, grid);
     l->pad = 0;
+    l->xcounters = h->d_counters.p + XCNT;
     return grid;
 }
 
@@ -418,6 +425,7 @@ int settle_scan(pcagpu *h)
     h->stats.candidates += o->cnt[0];
     h->stats.suffixes += o->cnt[1];
     h->stats.tests_evaluated += o->cnt[2];
+    h->stats.tests_executed += o->cnt[2] - o->cnt[XCNT] + o->cnt[XCNT + 1];
     h->stats.parents += o->cnt[3];
     const uint32_t hits = o->hits;
     h->stats.raw_hits += hits;
@@ -684,14 +692,14 @@ int pcagpu_open(const pca_task_t *task, const pcagpu_opts_t *opts, pcagpu_t **ou
     if ((rc = h->d_batch.ensure(h->batch_cap)) ||
         (rc = h->d_blocks.ensure(1)) ||
         (rc = h->d_hits.ensure(h->hit_cap)) ||
-        (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(8)) || (rc = h->d_work.ensure(1)) ||
+        (rc = h->d_hit_count.ensure(1)) || (rc = h->d_counters.ensure(XCNT + 2)) || (rc = h->d_work.ensure(1)) ||
         (rc = h->d_tstat.ensure(2 * MAXT)))
         return fail(rc);
     if (hipHostMalloc((void **)&h->h_batch, (size_t)h->batch_cap * sizeof(pca_prefix_t)) != hipSuccess ||
         hipHostMalloc((void **)&h->h_blocks, sizeof(ScanBlock)) != hipSuccess ||
         hipHostMalloc(&h->h_out, sizeof(LaunchOut)) != hipSuccess)
         return fail(PCAGPU_ENOMEM);
-    if (hipMemset(h->d_counters.p, 0, 8 * sizeof(unsigned long long)) != hipSuccess)
+    if (hipMemset(h->d_counters.p, 0, (XCNT + 2) * sizeof(unsigned long long)) != hipSuccess)
         return fail(PCAGPU_EHIP);
     if ((rc = h->d_task.ensure(1)) || (rc = h->d_items.ensure(64))) return fail(rc);
     if (hipMemcpy(h->d_task.p, &h->dt, sizeof(DevTask), hipMemcpyHostToDevice) != hipSuccess)

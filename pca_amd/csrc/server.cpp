@@ -721,6 +721,7 @@ extern "C" int pcagpu_server_step(pcagpu_server_t *s)
             const int grid = grid_for(in_launch[0]->h, n_units, n_items > 1, lv);
             l.grab = scan_grab(k, n_units, grid);   // runs of units when they are single prefixes
             l.pad = 0;
+            l.xcounters = nullptr;
             if (n_items > 1) {
                 // never an asynchronous copy from pageable memory: the items
                 // are a local vector, which an error return would free while
