@@ -900,11 +900,16 @@ __device__ __forceinline__ uint32_t addi_imm_of(uint32_t b)
 // the skipped tests' register matches) and fn_any, and when some group
 // matches every test writes the lanes' matches to sh.fn.
 __device__ __forceinline__ uint32_t node_scan(const DevTask &task, const CandDev &cd, const TestState *N, uint32_t mN,
-                                              uint64_t skipm, int TA, int lane, ShareLds &sh, uint32_t &rm_and_out,
-                                              bool &fn_any, uint32_t &ran)
+                                              uint64_t skipm, uint32_t act_tests, int TA, int lane, ShareLds &sh,
+                                              uint32_t &rm_and_out, bool &fn_any, uint32_t &ran)
 {
     const int T = task.T;
     const uint32_t ra = (uint32_t)lane >> 3, rb = (uint32_t)lane & 7u;
+    // the tests (not skipped) where this lane's divisor rb is nonzero: the
+    // DIV / MOD value-match flags change only there, so they keep the scan
+    // going only while such a test is left (a divisor that is 0 on every
+    // remaining test, r0 say, leaves them as they are)
+    const uint32_t nzd = act_tests & ~(uint32_t)skipm & ~sh.zb[rb];
     const uint32_t addi_imm = addi_imm_of(rb);
     const uint32_t kb = cd.g->kind_bits[mN][lane];
     LaneKinds g;
@@ -915,8 +920,10 @@ __device__ __forceinline__ uint32_t node_scan(const DevTask &task, const CandDev
     g.vmm = (kb >> 4 & 1u) != 0u;
     g.fault = false;
     uint32_t rm_and = 0xFFu, ev = 0;
-    for (int t = 0; t < T; t++) {
-        if (!any_lane(g.kl != 0u || g.okd != 0u || g.okm != 0u || g.vmd || g.vmm)) break;
+    int t = 0;
+    for (; t < T; t++) {
+        const bool vm_open = (g.vmd || g.vmm) && (nzd >> t) != 0u;
+        if (!any_lane(g.kl != 0u || g.okd != 0u || g.okm != 0u || vm_open)) break;
         const uint32_t nal = (uint32_t)__popc(g.kl) + (g.okd != 0u ? 1u : 0u) + (g.okm != 0u ? 1u : 0u);
         if (t >= TA || rm_and == 0xFFu) ev += nal;
         ran += nal;
@@ -933,6 +940,10 @@ __device__ __forceinline__ uint32_t node_scan(const DevTask &task, const CandDev
         lane_test<true>(g, x, y, tw, (uint32_t)lane, addi_imm,
                         [&]() { return row_regmatch_uni(&N[t], tw & 0xFFFFu); });
     }
+    // the skipped tests after an early exit still narrow rd (a settled
+    // DIV / MOD value match may still give its child a hit)
+    for (; t < T; t++)
+        if ((skipm >> t) & 1ull) rm_and &= row_regmatch_uni(&N[t], task.tgt[t]);
     rm_and_out = rm_and;
     const uint32_t fk = g.kl | ((g.okd != 0u || g.vmd) ? 1u << 3 : 0u) | ((g.okm != 0u || g.vmm) ? 1u << 4 : 0u);
     fn_any = any_lane(fk != 0u);
@@ -1460,9 +1471,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 const uint32_t pmN = summarize_state(task, pl, act, lpos, my_tgt).pm;
                 uint32_t rm_andN = 0xFFu, ran = 0;
                 bool fn_any = false;
-                const uint32_t evn =
-                    node_scan(task, cand, N, (lvp >> 1) & 127u, skipN, TA, lane, sh, rm_andN, fn_any, ran);
-                if (fn_any) {
+                {
                     // per register, the tests (not skipped) where N holds 0
                     uint32_t zr = 0;
 #pragma unroll
@@ -1471,7 +1480,10 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                         zr = lane == (int)r ? (uint32_t)zm : zr;
                     }
                     if (lane < 8) sh.zb[lane] = zr;
+                    wave_sync();
                 }
+                const uint32_t evn = node_scan(task, cand, N, (lvp >> 1) & 127u, skipN, (uint32_t)am, TA, lane, sh,
+                                               rm_andN, fn_any, ran);
                 // N's group tests over all lanes and over the lanes reading
                 // each register (a child is credited the lanes it shares)
                 const uint32_t ev_all = (uint32_t)wave_sum((unsigned long long)evn);
