@@ -92,12 +92,15 @@ struct ScanBlock {
     uint16_t j0;
 };
 constexpr int BLOCK_KIDS_MAX = 64;
+// a block of a shared-tile launch of 2- or 3-instruction suffixes may hold
+// up to 128 children (its node scanned once for all of them)
+constexpr int BLOCK_KIDS_SHARED = 128;
 
 // Tile choice of a scan launch (picked by the host autotuner): children per
 // block, tests run on every value group before survivors go to a queue,
 // and how a 2- or 3-instruction suffix scans the last position.
 struct ScanTile {
-    uint16_t block_kids;   // 1..64
+    uint16_t block_kids;   // 1..64 (1..128 for shared 2- and 3-instruction suffixes)
     uint8_t mask_tests;    // 1..6
     // 0: shared per block (value groups that do not read the register a
     // child writes are decided once on the block's node); 1: every child

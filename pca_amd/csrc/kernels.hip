@@ -1500,6 +1500,17 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                 wave_sync();
                 const uint32_t jsb = k == 3 ? js : 0u;
                 const uint32_t mN = (lvp >> 1) & 127u;
+                // a shared block holds up to 128 children: the second 64
+                // candidate words, one per lane (kid_word picks by index)
+                uint32_t cw2 = 0;
+                if (nb > 64u) {
+                    const int mfp2 = a.D - a.s - (k == 2 ? 1 : 2);
+                    cw2 = (uint32_t)lane + 64u < nb ? cand_at(cand, lvp, mfp2, (int)j0 + 64 + lane) : 0u;
+                }
+                auto kid_word = [&](uint32_t ci) -> uint32_t {
+                    return ci < 64u ? __builtin_amdgcn_readlane(cw, (int)ci)
+                                    : __builtin_amdgcn_readlane(cw2, (int)(ci - 64u));
+                };
                 // the tests on which the children run (not frozen, no
                 // pending branch past D-2)
                 const bool runs2 = act && ps_runs(w4, cpos);
@@ -1515,7 +1526,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                     uint32_t nx = 0, gmask = 0, cmask = 0;
                     uint64_t hist = 0;
                     for (uint32_t ci = base; ci < base + nh; ci++) {
-                        const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
+                        const uint32_t c = kid_word(ci);
                         const uint32_t op = insn_op(c), x = insn_rd(c);
                         const uint32_t cbit = 1u << (ci - base);
                         if (op == OP_BR) {
@@ -1628,7 +1639,7 @@ __device__ __forceinline__ void scan_body(const ScanLaunch &lch, const ScanArgs 
                     // register already equal to the target)
                     for (uint32_t mk = gmask; mk; mk &= mk - 1u) {
                         const uint32_t ci = base + (uint32_t)__builtin_ctz(mk);
-                        const uint32_t c = __builtin_amdgcn_readlane(cw, (int)ci);
+                        const uint32_t c = kid_word(ci);
                         PackedState st = pl;
                         if (act && ps_runs(st.w[4], cpos)) step_packed<false, false>(st, c, cpos);
                         TestState *PP = sv.slot[pool.cur];

@@ -190,9 +190,12 @@ constexpr int XCNT = 8;
 // each try one candidate on real work, and the one with the least kernel
 // time per parent state scanned is kept for the rest of the handle's life.
 // A launch with too few parents to time does not count as a try.
+// (blocks of 128 children are cut to 64 where only 64 fit: per-child
+// scans and suffixes of other lengths)
 constexpr ScanTile kTiles[] = {{64, 2, TILE_SHARED}, {64, 1, TILE_SHARED}, {64, 3, TILE_SHARED},
-                               {32, 2, TILE_SHARED}, {16, 2, TILE_SHARED}};
+                               {128, 3, TILE_SHARED}, {128, 2, TILE_SHARED}};
 constexpr int kNTiles = (int)(sizeof(kTiles) / sizeof(kTiles[0]));
+static_assert(kNTiles <= 8, "pcagpu::tile_cost holds 8 tries");
 constexpr uint64_t kTileMinParents = 200000;
 
 ScanTile current_tile(const pcagpu *h, int *cand)
@@ -200,9 +203,11 @@ ScanTile current_tile(const pcagpu *h, int *cand)
     *cand = -1;
     // bits 16..23 of the option: 1 scans every child on its own
     const uint8_t mode = ((h->opts.scan_tile >> 16) & 0xFFu) == 1u ? TILE_PER_CHILD : TILE_SHARED;
+    // blocks above 64 children only where the shared last position scans them
+    const uint32_t bk_max = (mode == TILE_SHARED && (h->k == 2 || h->k == 3)) ? BLOCK_KIDS_SHARED : BLOCK_KIDS_MAX;
     if (h->opts.scan_tile & 0xFFFFu) {
         const uint32_t bk = h->opts.scan_tile & 0xFFu, mt = (h->opts.scan_tile >> 8) & 0xFFu;
-        return ScanTile{(uint16_t)std::min<uint32_t>(std::max<uint32_t>(bk, 1u), BLOCK_KIDS_MAX),
+        return ScanTile{(uint16_t)std::min<uint32_t>(std::max<uint32_t>(bk, 1u), bk_max),
                         (uint8_t)std::min<uint32_t>(std::max<uint32_t>(mt, 1u), 6u), mode};
     }
     const int k = std::min(std::max(h->k, 0), (int)MAXLEV);
@@ -214,6 +219,7 @@ ScanTile current_tile(const pcagpu *h, int *cand)
         t = kTiles[*cand];
     }
     t.mode = mode;
+    t.block_kids = (uint16_t)std::min<uint32_t>(t.block_kids, bk_max);
     return t;
 }
 
