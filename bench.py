@@ -267,6 +267,43 @@ def suite_bench(n_tasks, budget, seed=11):
             "candidates": st["candidates"], "same_results": a == b}
 
 
+def suite_depth5_bench(time_limit, seed=11, n_tasks=1000):
+    """Config-3 suite past budget 4: the tasks with no program of at most 4
+    instructions, each searched at depth 5 on one resident handle with a
+    per-task time limit (stops between flushes; a task that hits it has not
+    finished depth 5).  Every solution is re-run on the CPU VM."""
+    import pca_amd
+    from pca_amd import taskgen
+    tasks = [taskgen.random_task(seed, i) for i in range(n_tasks)]
+    out = {"generator": f"taskgen.random_task(seed {seed})", "tasks": n_tasks, "time_limit_s": time_limit, "cases": []}
+    with pca_amd.Solver(tasks[0]) as sv:
+        t0 = time.perf_counter()
+        hard = []
+        for i, t in enumerate(tasks):
+            sv.retask(t)
+            if sv.solve(4, cap=16).optimal_depth < 0:
+                hard.append(i)
+        out["budget4_s"] = time.perf_counter() - t0
+        for i in hard:
+            sv.retask(tasks[i])
+            sv.reset_stats()
+            t1 = time.perf_counter()
+            r = sv.solve(5, start_depth=5, cap=1 << 20, time_limit=time_limit)
+            dt = time.perf_counter() - t1
+            st = sv.stats()
+            finished = dt < time_limit
+            out["cases"].append({"task": i, "inputs": tasks[i].num_inputs, "tests": tasks[i].num_tests,
+                                 "result": "optimum 5" if r.optimal_depth == 5 else
+                                 ("none <= 5" if finished else "depth 5 unfinished"),
+                                 "solutions": r.num_solutions, "seconds": dt, "candidates_per_s": st["candidates"] / dt,
+                                 "cpu_verified": all(x[4] for x in r.solutions)})
+            print(f"suite depth 5: task {i} {out['cases'][-1]['result']}", file=sys.stderr, flush=True)
+    out["unsolved_at_4"] = len(hard)
+    for key in ("optimum 5", "none <= 5", "depth 5 unfinished"):
+        out[key.replace(" ", "_").replace("<=", "le")] = sum(1 for c in out["cases"] if c["result"] == key)
+    return out
+
+
 def value_bench(batch=512, steps=30, seed=5):
     """Value network on the GPU (BF16 MFMA kernels): training and inference
     throughput in search states per second on synthetic states (random
@@ -468,6 +505,9 @@ def main():
     ap.add_argument("--suite-deep-budget", type=int, default=4,
                     help="a second suite field at this depth budget (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--suite-depth5", type=float, default=0.0,
+                    help="config-3 tasks unsolved at budget 4 searched at depth 5 with this time limit per task "
+                         "(seconds; 0: skip, minutes when on)")
     ap.add_argument("--no-optimum", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -556,6 +596,8 @@ def main():
         if args.suite_deep_budget:
             # the deeper budget: the tasks whose optimum needs it dominate
             line["suite_deep"] = suite_bench(args.suite, args.suite_deep_budget)
+    if args.suite_depth5 > 0:
+        line["suite_depth5"] = suite_depth5_bench(args.suite_depth5)
     if not args.no_value:
         line["value_net"] = value_bench()
         print("value net done", file=sys.stderr, flush=True)
