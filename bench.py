@@ -505,6 +505,9 @@ def main():
     ap.add_argument("--suite-deep-budget", type=int, default=4,
                     help="a second suite field at this depth budget (0: skip)")
     ap.add_argument("--serving-requests", type=int, default=200)
+    ap.add_argument("--guided", type=int, default=0,
+                    help="config-5 style: guided vs breadth-first search on this many tasks of 9-10-instruction "
+                         "sources, the exhaustive solve as judge (0: skip; minutes when on)")
     ap.add_argument("--suite-depth5", type=float, default=0.0,
                     help="config-3 tasks unsolved at budget 4 searched at depth 5 with this time limit per task "
                          "(seconds; 0: skip, minutes when on)")
@@ -596,6 +599,11 @@ def main():
         if args.suite_deep_budget:
             # the deeper budget: the tasks whose optimum needs it dominate
             line["suite_deep"] = suite_bench(args.suite, args.suite_deep_budget)
+    if args.guided > 0:
+        from pca_amd import value as V
+        net, info = V.train_on_solved()
+        line["guided"] = {"training": info, **V.guided_suite(net, n_tasks=args.guided)}
+        print("guided search done", file=sys.stderr, flush=True)
     if args.suite_depth5 > 0:
         line["suite_depth5"] = suite_depth5_bench(args.suite_depth5)
     if not args.no_value:

@@ -135,7 +135,7 @@ def _interp(prog, ins, max_cycles=64):
     return None
 
 
-def random_task(seed, index):
+def random_task(seed, index, length=(3, 8)):
     """Task #index of the seeded synthetic suite (see module docstring).
 
     Control-flow opcodes (BR, JMP, CALL, RET) are drawn like the others but
@@ -145,7 +145,7 @@ def random_task(seed, index):
     and HLT end it."""
     rng = Xoshiro256((seed * 1000003 + index) & M64)
     n_in = rng.randint(1, 2)
-    length = rng.randint(3, 8)
+    length = rng.randint(length[0], length[1])
     prog = [(23 << 11) | ((i + 1) << 8) | i for i in range(n_in)]
     for _ in range(length):
         op = rng.randint(0, 24)

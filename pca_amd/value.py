@@ -443,6 +443,110 @@ def guided_search(task, max_depth, net=None, budget=2000, seed=0, r_exhaustive=0
     return None, expansions
 
 
+def train_on_solved(n_tasks=1200, length=3, seed=101, steps=1500, batch=512, negatives=24, log=None):
+    """The network trained on states of exhaustively solved tasks: dependency
+    chains of `length` grammar instructions (taskgen.chain_task) solved on
+    the GPU (one resident handle, re-tasked), those with an optimum of at
+    least 2 labelled by labelled_states (reach, and the distance to a
+    solution the solve proves).  Returns (net, info)."""
+    import time as _time
+    from . import taskgen
+    from .solver import Solver
+    t0 = _time.perf_counter()
+    recs, nvalid, depths, labels, remaining = [], [], [], [], []
+    kept = 0
+    sv = None
+    try:
+        for i in range(n_tasks):
+            t = taskgen.chain_task(seed, i, length)
+            if sv is None:
+                sv = Solver(t)
+            else:
+                sv.retask(t)
+            r = sv.solve(length, cap=4096)
+            if r.optimal_depth < 2:
+                continue
+            kept += 1
+            d = r.optimal_depth
+            kernels = [list(x[0])[t.num_inputs:t.num_inputs + d] for x in r.solutions]
+            p, dp, lb, rm = labelled_states(t, d, kernels, negatives_per_node=negatives, seed=len(recs))
+            rec, _ = state_records(t, p)
+            recs.append(rec)
+            nvalid += [t.num_tests] * len(p)
+            depths += dp
+            labels += lb
+            remaining += rm
+    finally:
+        if sv is not None:
+            sv.close()
+    data_s = _time.perf_counter() - t0
+    net = ValueNet(seed=0)
+    t1 = _time.perf_counter()
+    losses = net.train(np.concatenate(recs), np.array(nvalid, dtype=np.uint8), np.array(depths, dtype=np.int32),
+                       np.array(labels, dtype=np.float32), steps=steps, batch=batch, lr=1e-3,
+                       remaining=np.array(remaining, dtype=np.int32), log=log)
+    return net, {"tasks": n_tasks, "tasks_kept": kept, "states": len(labels), "positives": int(sum(labels)),
+                 "data_s": data_s, "train_s": _time.perf_counter() - t1, "loss_first": float(losses[0]),
+                 "loss_last": float(losses[-1])}
+
+
+def guided_suite(net, n_tasks=100, src_len=(9, 10), max_depth=4, budget=60, seed=29, min_opt=2, scan=8000):
+    """Guided against breadth-first best-first search on generated tasks
+    whose source programs have src_len instructions (taskgen.random_task),
+    with the exhaustive GPU solve to max_depth as the judge and fallback.
+    Most such programs reduce to 0 or 1 instructions; the first n_tasks
+    whose proven optimum is at least min_opt (of at most `scan` generated)
+    are compared: whether each search found a CPU-verified program within
+    `budget` expansions, its length and its expansions."""
+    import time as _time
+    from . import taskgen
+    from .solver import Solver
+    cases = []
+    hist = {}
+    scanned = 0
+    ex_total = 0.0
+    with Solver(taskgen.random_task(seed, 0, src_len)) as sv:
+        for i in range(scan):
+            if len(cases) >= n_tasks:
+                break
+            t = taskgen.random_task(seed, i, src_len)
+            sv.retask(t)
+            t0 = _time.perf_counter()
+            r = sv.solve(max_depth, cap=16)
+            ex_s = _time.perf_counter() - t0
+            ex_total += ex_s
+            scanned += 1
+            hist[str(r.optimal_depth)] = hist.get(str(r.optimal_depth), 0) + 1
+            if r.optimal_depth < min_opt:
+                continue
+            row = {"task": i, "optimum": r.optimal_depth, "exhaustive_s": ex_s}
+            if True:
+                for name, m in (("guided", net), ("breadth_first", None)):
+                    t0 = _time.perf_counter()
+                    res, exp = guided_search(t, max_depth, net=m, budget=budget)
+                    ok = res is not None and verify_program(t, res[0], res[1])
+                    row[name] = {"found": res is not None, "verified": ok, "length": len(res[0]) if res else None,
+                                 "optimal": bool(res is not None and len(res[0]) == r.optimal_depth),
+                                 "expansions": exp, "seconds": _time.perf_counter() - t0}
+            cases.append(row)
+    solved = cases
+    out = {"tasks": len(cases), "generated": scanned, "source_length": list(src_len), "max_depth": max_depth,
+           "budget": budget, "generator": f"taskgen.random_task(seed {seed}, length {src_len})",
+           "optimum_histogram_generated": hist, "optimum_min": min_opt}
+    for name in ("guided", "breadth_first"):
+        found = [c for c in solved if c[name]["found"]]
+        out[name] = {"found": len(found), "verified": sum(1 for c in found if c[name]["verified"]),
+                     "optimal": sum(1 for c in solved if c[name]["optimal"]),
+                     "mean_expansions_found": (sum(c[name]["expansions"] for c in found) / len(found)) if found else None,
+                     "seconds": sum(c[name]["seconds"] for c in solved)}
+    both = [c for c in solved if c["guided"]["optimal"] and c["breadth_first"]["optimal"]]
+    out["both_optimal"] = len(both)
+    out["expansions_on_both"] = {n: sum(c[n]["expansions"] for c in both) for n in ("guided", "breadth_first")}
+    out["exhaustive_s"] = ex_total
+    out["exhaustive_s_compared"] = sum(c["exhaustive_s"] for c in cases)
+    return out
+
+
 def verify_program(task, prog, out_reg):
     """Runs IN scaffold + prog + OUT out_reg + HLT on the CPU VM over every
     test (pcagpu_task_evaluate); True when all pass."""
