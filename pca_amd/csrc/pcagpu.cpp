@@ -188,7 +188,8 @@ constexpr int XCNT = 8;
 // Scan tile autotuner.  The tile (children per block, tests run by wave
 // masks) is chosen per suffix length k: the first launches of that shape
 // each try one candidate on real work, and the one with the least kernel
-// time per parent state scanned is kept for the rest of the handle's life.
+// time per unit of work (group tests and parent states, weighted as in the
+// roofline) is kept for the rest of the handle's life.
 // A launch with too few parents to time does not count as a try.
 // (blocks of 128 children are cut to 64 where only 64 fit: per-child
 // scans and suffixes of other lengths)
@@ -225,11 +226,15 @@ ScanTile current_tile(const pcagpu *h, int *cand)
 
 // (a launch queued before the previous one's feedback arrived may have been
 // given the same candidate: only the candidate being tried is credited)
-void tile_feedback(pcagpu *h, int cand, int k_launch, float ms, uint64_t parents)
+void tile_feedback(pcagpu *h, int cand, int k_launch, float ms, uint64_t parents, uint64_t tests)
 {
     const int k = std::min(std::max(k_launch, 0), (int)MAXLEV);
     if (cand < 0 || cand != h->tile_tried[k] || parents < kTileMinParents) return;
-    h->tile_cost[k][cand] = (double)ms * 1e6 / (double)parents;
+    // kernel time per unit of the launch's work (9 ops per group test
+    // deciding a last-position value group, 15 per parent and test): the
+    // tries run on different batches, whose work per parent differs
+    const double work = 9.0 * (double)tests + 15.0 * (double)h->dt.T * (double)parents;
+    h->tile_cost[k][cand] = (double)ms * 1e6 / work;
     if (++h->tile_tried[k] < kNTiles) return;
     int best = 0;
     for (int i = 1; i < kNTiles; i++)
@@ -447,7 +452,7 @@ int settle_scan(pcagpu *h)
             h->raw.push_back(x);
         }
     }
-    tile_feedback(h, p.cand, p.k, ms, o->cnt[3]);
+    tile_feedback(h, p.cand, p.k, ms, o->cnt[3], o->cnt[2]);
     if (h->opts.fixed_test_order) return 0;
     unsigned long long ts[2 * MAXT];
     std::memcpy(ts, o->ts, sizeof(ts));
